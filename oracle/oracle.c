@@ -1,0 +1,1432 @@
+/* CPU restatement of the reference ADER-WENO solver.  TEST INFRASTRUCTURE ONLY:
+ * see oracle.h.  Reference file:line citations are relative to the reference
+ * tree (haranjackson/PyPDE).
+ *
+ * Operation-order conventions shared with the HIP kernels:
+ *   - dot products:  acc = a[0]*b[0]; acc = fma(a[j], b[j], acc) for j = 1..;
+ *   - every other expression is evaluated as written (-ffp-contract=off);
+ *   - tables are computed in long double and rounded once to double.
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MAXN 6
+#define MAXV 8
+#define MAXD 3
+#define MAXND 216 /* MAXN^MAXD */
+
+/* Constants of the reference: types.h:18, weno.h:8-10, dg.h:8-9,
+ * NumericalDiff.h:9. */
+static const double WENO_LAMS = 1.0, WENO_LAMC = 1e5, WENO_EPS = 1e-14;
+static const int DG_IT = 50;
+static const double DG_TOL = 6e-6;
+static const double FD_EPS = 1.4901161193847656e-08; /* sqrt(2^-52) */
+
+/* ------------------------------------------------------------------ tables */
+
+typedef long double ld;
+
+/* Gauss-Legendre nodes/weights on [-1,1] by Newton iteration on P_N from
+ * Chebyshev-like starting points; the reference gets the same quantities from
+ * the companion-matrix eigenvalues plus one Newton step (legendre.cpp:136-197).
+ * Returned ascending, mapped to [0,1] (basis.cpp:7-20). */
+static void gl_nodes(int n, ld *x, ld *w) {
+  for (int i = 0; i < n; i++) {
+    ld z = cosl(3.14159265358979323846264338327950288L * ((ld)i + 0.75L) /
+                ((ld)n + 0.5L));
+    ld dp = 1.0L;
+    for (int it = 0; it < 100; it++) {
+      ld p0 = 1.0L, p1 = z;
+      for (int k = 2; k <= n; k++) {
+        ld p2 = ((ld)(2 * k - 1) * z * p1 - (ld)(k - 1) * p0) / (ld)k;
+        p0 = p1;
+        p1 = p2;
+      }
+      if (n == 1) p0 = 1.0L;
+      dp = (ld)n * (z * p1 - p0) / (z * z - 1.0L);
+      ld z1 = z;
+      z = z1 - p1 / dp;
+      if (fabsl(z - z1) <= 4.0L * LDBL_EPSILON) break;
+    }
+    {
+      ld p0 = 1.0L, p1 = z;
+      for (int k = 2; k <= n; k++) {
+        ld p2 = ((ld)(2 * k - 1) * z * p1 - (ld)(k - 1) * p0) / (ld)k;
+        p0 = p1;
+        p1 = p2;
+      }
+      if (n == 1) p0 = 1.0L;
+      dp = (ld)n * (z * p1 - p0) / (z * z - 1.0L);
+    }
+    x[n - 1 - i] = (1.0L + z) / 2.0L;
+    w[n - 1 - i] = 1.0L / ((1.0L - z * z) * dp * dp);
+  }
+}
+
+/* Power-basis coefficients of the Lagrange polynomials at the nodes
+ * (basis.cpp:22-53): c[j][k] is the s^k coefficient of psi_j. */
+static void lagrange_coeffs(int n, const ld *x, ld c[MAXN][MAXN]) {
+  for (int j = 0; j < n; j++) {
+    ld p[MAXN];
+    for (int k = 0; k < n; k++) p[k] = 0.0L;
+    p[0] = 1.0L;
+    int deg = 0;
+    for (int m = 0; m < n; m++) {
+      if (m == j) continue;
+      ld den = x[j] - x[m];
+      for (int k = deg + 1; k >= 1; k--) p[k] = (p[k - 1] - x[m] * p[k]) / den;
+      p[0] = (-x[m] * p[0]) / den;
+      deg++;
+    }
+    for (int k = 0; k < n; k++) c[j][k] = p[k];
+  }
+}
+
+static ld poly_eval(const ld *c, int len, ld s) {
+  ld r = 0.0L;
+  for (int k = len - 1; k >= 0; k--) r = r * s + c[k];
+  return r;
+}
+
+/* Antiderivative integral of psi_j over [a, a+1]. */
+static ld poly_int(const ld *c, int n, ld a) {
+  ld P[MAXN + 1];
+  P[0] = 0.0L;
+  for (int k = 0; k < n; k++) P[k + 1] = c[k] / (ld)(k + 1);
+  return poly_eval(P, n + 1, a + 1.0L) - poly_eval(P, n + 1, a);
+}
+
+/* Gauss-Jordan inverse with partial pivoting. */
+static int ld_inverse(int n, ld A[MAXN][MAXN], ld R[MAXN][MAXN]) {
+  ld M[MAXN][2 * MAXN];
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < 2 * n; j++)
+      M[i][j] = j < n ? A[i][j] : (ld)(j - n == i);
+  for (int k = 0; k < n; k++) {
+    int p = k;
+    for (int i = k + 1; i < n; i++)
+      if (fabsl(M[i][k]) > fabsl(M[p][k])) p = i;
+    if (M[p][k] == 0.0L) return -1;
+    if (p != k)
+      for (int j = 0; j < 2 * n; j++) {
+        ld t = M[k][j];
+        M[k][j] = M[p][j];
+        M[p][j] = t;
+      }
+    ld piv = M[k][k];
+    for (int j = 0; j < 2 * n; j++) M[k][j] /= piv;
+    for (int i = 0; i < n; i++) {
+      if (i == k) continue;
+      ld f = M[i][k];
+      for (int j = 0; j < 2 * n; j++) M[i][j] -= f * M[k][j];
+    }
+  }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) R[i][j] = M[i][n + j];
+  return 0;
+}
+
+typedef struct {
+  int N, D, V, Nd, Nd1, NT; /* N, ndim, nvar, N^D, N^(D-1), N^(D+1) */
+  double x[MAXN], w[MAXN], end0[MAXN], end1[MAXN], der[MAXN][MAXN];
+  double wmat[4][MAXN][MAXN], winv[4][MAXN][MAXN], sig[MAXN][MAXN];
+  double dgmat[MAXN][MAXN], dginv[MAXN][MAXN];
+  double flc[MAXN], flm[MAXN][MAXN][MAXN]; /* flm[k-1] = M_k */
+  int nsten, soff[4];
+  double slam[4];
+  /* per-node tables (node idx = row-major multi-index k_0..k_{D-1}) */
+  int kk[MAXND][MAXD];
+  double WP[MAXND];  /* prod_d W[k_d]                      (stepper.cpp:31-40) */
+  double IWP[MAXND]; /* 1 / WP */
+} Tab;
+
+int oracle_tables(int N, double *nodes, double *wghts, double *endvals,
+                  double *dervals, double *weno_mat, double *weno_inv,
+                  double *sig, double *dgmat, double *dginv, double *fl_c,
+                  double *fl_m) {
+  if (N < 1 || N > MAXN) return 1;
+  int n = N;
+  ld x[MAXN], w[MAXN], c[MAXN][MAXN];
+  gl_nodes(n, x, w);
+  lagrange_coeffs(n, x, c);
+  if (nodes)
+    for (int i = 0; i < n; i++) nodes[i] = (double)x[i];
+  if (wghts)
+    for (int i = 0; i < n; i++) wghts[i] = (double)w[i];
+  /* ENDVALS[e][j] = psi_j(e) (basis.cpp:55-65) */
+  ld e0[MAXN], e1[MAXN];
+  for (int j = 0; j < n; j++) {
+    e0[j] = poly_eval(c[j], n, 0.0L);
+    e1[j] = poly_eval(c[j], n, 1.0L);
+    if (endvals) {
+      endvals[j] = (double)e0[j];
+      endvals[n + j] = (double)e1[j];
+    }
+  }
+  /* derivative coefficients */
+  ld dc[MAXN][MAXN];
+  for (int j = 0; j < n; j++) {
+    for (int k = 0; k < n; k++) dc[j][k] = 0.0L;
+    for (int k = 1; k < n; k++) dc[j][k - 1] = c[j][k] * (ld)k;
+  }
+  /* DERVALS[i][j] = psi_j'(x_i) (basis.cpp:67-76) */
+  if (dervals)
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++)
+        dervals[i * n + j] = (double)poly_eval(dc[j], n, x[i]);
+  /* WENO stencil matrices (weno_matrices.cpp:8-33) and their inverses */
+  int FN2 = (n - 1) / 2, CN2 = n / 2;
+  int off[4] = {-(n - 1), 0, -CN2, -FN2};
+  for (int s = 0; s < 4; s++) {
+    ld A[MAXN][MAXN], R[MAXN][MAXN];
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) A[i][j] = poly_int(c[j], n, (ld)(i + off[s]));
+    if (ld_inverse(n, A, R)) return 2;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        if (weno_mat) weno_mat[(s * n + i) * n + j] = (double)A[i][j];
+        if (weno_inv) weno_inv[(s * n + i) * n + j] = (double)R[i][j];
+      }
+  }
+  /* SIG[i][j] = sum_{a=1}^{n-1} int_0^1 psi_i^(a) psi_j^(a) (weno_matrices.cpp:36-51) */
+  if (sig) {
+    ld S[MAXN][MAXN];
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) S[i][j] = 0.0L;
+    ld di[MAXN][MAXN];
+    for (int j = 0; j < n; j++)
+      for (int k = 0; k < n; k++) di[j][k] = c[j][k];
+    for (int a = 1; a < n; a++) {
+      for (int j = 0; j < n; j++) {
+        for (int k = 1; k < n; k++) di[j][k - 1] = di[j][k] * (ld)k;
+        di[j][n - 1] = 0.0L;
+      }
+      for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+          ld acc = 0.0L;
+          for (int p = 0; p < n; p++)
+            for (int q = 0; q < n; q++)
+              acc += di[i][p] * di[j][q] / (ld)(p + q + 1);
+          S[i][j] += acc;
+        }
+    }
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) sig[i * n + j] = (double)S[i][j];
+  }
+  /* DG_MAT[i][j] = psi_i(1) psi_j(1) - int_0^1 psi_i' psi_j
+   * (dg.cpp:38-40 with dg_matrices.cpp:28-55, the quadrature being exact) */
+  ld G[MAXN][MAXN], GI[MAXN][MAXN];
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      ld acc = 0.0L;
+      for (int p = 0; p < n; p++)
+        for (int q = 0; q < n; q++) acc += dc[i][p] * c[j][q] / (ld)(p + q + 1);
+      G[i][j] = e1[i] * e1[j] - acc;
+    }
+  if (ld_inverse(n, G, GI)) return 3;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      if (dgmat) dgmat[i * n + j] = (double)G[i][j];
+      if (dginv) dginv[i * n + j] = (double)GI[i][j];
+    }
+  /* Faddeev-LeVerrier on Tw = diag(1/W) DG_MAT: det(xI - Tw) = x^n + sum c_k x^k,
+   * adj(xI - Tw) = sum_{k=1}^{n} M_k x^{n-k}.  Used by the stiff predictor's
+   * Newton solve (this build's own method; the reference uses Newton-Krylov,
+   * newton_krylov.cpp:144-183). */
+  {
+    ld T[MAXN][MAXN], Mk[MAXN][MAXN], TM[MAXN][MAXN], cc[MAXN + 1];
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) T[i][j] = G[i][j] / w[i];
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) Mk[i][j] = (ld)(i == j);
+    cc[n] = 1.0L;
+    for (int k = 1; k <= n; k++) {
+      if (k > 1) {
+        for (int i = 0; i < n; i++)
+          for (int j = 0; j < n; j++) Mk[i][j] = TM[i][j] + (i == j ? cc[n - k + 1] : 0.0L);
+      }
+      for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+          ld acc = 0.0L;
+          for (int p = 0; p < n; p++) acc += T[i][p] * Mk[p][j];
+          TM[i][j] = acc;
+        }
+      ld tr = 0.0L;
+      for (int i = 0; i < n; i++) tr += TM[i][i];
+      cc[n - k] = -tr / (ld)k;
+      if (fl_m)
+        for (int i = 0; i < n; i++)
+          for (int j = 0; j < n; j++)
+            fl_m[((k - 1) * n + i) * n + j] = (double)Mk[i][j];
+    }
+    if (fl_c)
+      for (int k = 0; k < n; k++) fl_c[k] = (double)cc[k];
+  }
+  return 0;
+}
+
+static int tab_init(Tab *T, int N, int D, int V) {
+  if (N < 1 || N > MAXN || D < 1 || D > MAXD || V < 1 || V > MAXV) return 1;
+  memset(T, 0, sizeof(*T));
+  T->N = N;
+  T->D = D;
+  T->V = V;
+  T->Nd = 1;
+  for (int d = 0; d < D; d++) T->Nd *= N;
+  if (T->Nd > MAXND) return 1;
+  T->Nd1 = T->Nd / N;
+  T->NT = T->Nd * N;
+  double endv[2 * MAXN], derv[MAXN * MAXN], wm[4 * MAXN * MAXN],
+      wi[4 * MAXN * MAXN], sg[MAXN * MAXN], dg[MAXN * MAXN], dgi[MAXN * MAXN],
+      flc[MAXN], flm[MAXN * MAXN * MAXN];
+  if (oracle_tables(N, T->x, T->w, endv, derv, wm, wi, sg, dg, dgi, flc, flm))
+    return 2;
+  for (int i = 0; i < N; i++) {
+    T->end0[i] = endv[i];
+    T->end1[i] = endv[N + i];
+    T->flc[i] = flc[i];
+    for (int j = 0; j < N; j++) {
+      T->der[i][j] = derv[i * N + j];
+      T->sig[i][j] = sg[i * N + j];
+      T->dgmat[i][j] = dg[i * N + j];
+      T->dginv[i][j] = dgi[i * N + j];
+      for (int s = 0; s < 4; s++) {
+        T->wmat[s][i][j] = wm[(s * N + i) * N + j];
+        T->winv[s][i][j] = wi[(s * N + i) * N + j];
+      }
+      for (int k = 0; k < N; k++) T->flm[k][i][j] = flm[(k * N + i) * N + j];
+    }
+  }
+  /* stencils in the reference's order (weno.cpp:47-55) */
+  int FN2 = (N - 1) / 2, CN2 = N / 2;
+  T->nsten = 2;
+  T->soff[0] = 0;
+  T->slam[0] = WENO_LAMS;
+  T->soff[1] = N - 1;
+  T->slam[1] = WENO_LAMS;
+  if (N > 2) {
+    T->soff[2] = FN2;
+    T->slam[2] = WENO_LAMC;
+    T->nsten = 3;
+    if (N % 2 == 0) {
+      T->soff[3] = CN2;
+      T->slam[3] = WENO_LAMC;
+      T->nsten = 4;
+    }
+  }
+  for (int idx = 0; idx < T->Nd; idx++) {
+    int r = idx;
+    for (int d = D - 1; d >= 0; d--) {
+      T->kk[idx][d] = r % N;
+      r /= N;
+    }
+    double p = T->w[T->kk[idx][0]];
+    for (int d = 1; d < D; d++) p = p * T->w[T->kk[idx][d]];
+    T->WP[idx] = p;
+    T->IWP[idx] = 1.0 / p;
+  }
+  return 0;
+}
+
+/* ---------------------------------------------------------------- helpers */
+
+static int ipow(int b, int e) {
+  int r = 1;
+  for (int i = 0; i < e; i++) r *= b;
+  return r;
+}
+
+/* Derivative along axis d at node idx of a nodal tensor q (Nd x V rows):
+ * DERVALS row k_d against the line through idx, then / dX[d]
+ * (evaluations.cpp:23-47). */
+static void node_deriv(const Tab *T, const double *q, int idx, int d,
+                       double dx, double *out) {
+  int N = T->N, V = T->V;
+  int stride = ipow(N, T->D - 1 - d);
+  int kd = T->kk[idx][d];
+  int base = idx - kd * stride;
+  for (int v = 0; v < V; v++) {
+    double acc = T->der[kd][0] * q[base * V + v];
+    for (int j = 1; j < N; j++)
+      acc = fma(T->der[kd][j], q[(base + j * stride) * V + v], acc);
+    out[v] = acc / dx;
+  }
+}
+
+/* ---------------------------------------------------------- eigenvalues */
+
+/* Householder vector for x[0..m-1] (Golub & Van Loan, Alg. 5.1.1): P x = alpha e_1,
+ * P = I - beta v v^T with v[0] = 1. */
+static double house(const double *x, int m, double *v) {
+  double sigma = 0.0;
+  for (int i = 1; i < m; i++) sigma = fma(x[i], x[i], sigma);
+  v[0] = 1.0;
+  if (sigma == 0.0) {
+    for (int i = 1; i < m; i++) v[i] = 0.0;
+    return 0.0;
+  }
+  double mu = sqrt(x[0] * x[0] + sigma);
+  double v0 = (x[0] <= 0.0) ? x[0] - mu : -sigma / (x[0] + mu);
+  double beta = 2.0 * (v0 * v0) / (sigma + v0 * v0);
+  for (int i = 1; i < m; i++) v[i] = x[i] / v0;
+  return beta;
+}
+
+/* rows r0..r0+m-1, columns c0..c1: H = P H */
+static void house_left(double *H, int n, const double *v, int m, double beta,
+                       int r0, int c0, int c1) {
+  for (int j = c0; j <= c1; j++) {
+    double s = H[r0 * n + j];
+    for (int i = 1; i < m; i++) s = fma(v[i], H[(r0 + i) * n + j], s);
+    s = beta * s;
+    for (int i = 0; i < m; i++) H[(r0 + i) * n + j] = fma(-s, v[i], H[(r0 + i) * n + j]);
+  }
+}
+
+/* rows r0..r1, columns c0..c0+m-1: H = H P */
+static void house_right(double *H, int n, const double *v, int m, double beta,
+                        int r0, int r1, int c0) {
+  for (int i = r0; i <= r1; i++) {
+    double s = H[i * n + c0];
+    for (int j = 1; j < m; j++) s = fma(v[j], H[i * n + c0 + j], s);
+    s = beta * s;
+    for (int j = 0; j < m; j++) H[i * n + c0 + j] = fma(-s, v[j], H[i * n + c0 + j]);
+  }
+}
+
+/* max |lambda| of a real n x n matrix, the quantity the reference takes from
+ * Eigen's EigenSolver (system.cpp:28-43; Eigen RealSchur scales by the largest
+ * |entry|, RealSchur.h:258-276).  Method: scale, Householder reduction to
+ * Hessenberg form, then Francis double-shift QR steps (Golub & Van Loan,
+ * Alg. 7.5.1-7.5.2) with deflation on negligible subdiagonals and ad hoc
+ * exceptional shifts after 10 and 20 iterations without deflation. */
+double oracle_max_abs_eig(const double *A, int n) {
+  double H[MAXV * MAXV];
+  double scale = 0.0;
+  for (int i = 0; i < n * n; i++) scale = fmax(scale, fabs(A[i]));
+  if (!(scale >= DBL_MIN)) return (scale != scale) ? scale : 0.0;
+  for (int i = 0; i < n * n; i++) H[i] = A[i] / scale;
+  double v[3], x[MAXV];
+  for (int k = 0; k + 2 < n; k++) {
+    int m = n - k - 1;
+    double vv[MAXV];
+    for (int i = 0; i < m; i++) x[i] = H[(k + 1 + i) * n + k];
+    double beta = house(x, m, vv);
+    if (beta != 0.0) {
+      house_left(H, n, vv, m, beta, k + 1, k, n - 1);
+      house_right(H, n, vv, m, beta, 0, n - 1, k + 1);
+    }
+  }
+  double anorm = 0.0;
+  for (int i = 0; i < n * n; i++) anorm = anorm + fabs(H[i]);
+  double best = 0.0;
+  int hi = n - 1, iter = 0, total = 0;
+  while (hi >= 0) {
+    int lo = hi;
+    while (lo > 0) {
+      double s = fabs(H[(lo - 1) * n + lo - 1]) + fabs(H[lo * n + lo]);
+      if (s == 0.0) s = anorm;
+      if (fabs(H[lo * n + lo - 1]) <= DBL_EPSILON * s) {
+        H[lo * n + lo - 1] = 0.0;
+        break;
+      }
+      lo--;
+    }
+    if (lo == hi) {
+      best = fmax(best, fabs(H[hi * n + hi]));
+      hi -= 1;
+      iter = 0;
+      continue;
+    }
+    if (lo == hi - 1) {
+      double a = H[(hi - 1) * n + hi - 1], b = H[(hi - 1) * n + hi];
+      double c = H[hi * n + hi - 1], dd = H[hi * n + hi];
+      double p = 0.5 * (a - dd);
+      double bc = b * c;
+      double disc = p * p + bc;
+      if (disc >= 0.0) {
+        double z = sqrt(disc);
+        z = (p >= 0.0) ? p + z : p - z;
+        double l1 = dd + z;
+        double l2 = (z != 0.0) ? dd - bc / z : dd;
+        best = fmax(best, fmax(fabs(l1), fabs(l2)));
+      } else {
+        double re = dd + p;
+        best = fmax(best, sqrt(re * re - disc));
+      }
+      hi -= 2;
+      iter = 0;
+      continue;
+    }
+    if (total >= 30 * n) return NAN;
+    iter++;
+    total++;
+    double s, t;
+    if (iter == 10 || iter == 20) {
+      double ww = fabs(H[hi * n + hi - 1]) + fabs(H[(hi - 1) * n + hi - 2]);
+      s = 1.5 * ww;
+      t = ww * ww;
+    } else {
+      s = H[(hi - 1) * n + hi - 1] + H[hi * n + hi];
+      t = H[(hi - 1) * n + hi - 1] * H[hi * n + hi] - H[(hi - 1) * n + hi] * H[hi * n + hi - 1];
+    }
+    double h00 = H[lo * n + lo], h01 = H[lo * n + lo + 1], h10 = H[(lo + 1) * n + lo];
+    double h11 = H[(lo + 1) * n + lo + 1], h21 = H[(lo + 2) * n + lo + 1];
+    x[0] = h00 * h00 + h01 * h10 - s * h00 + t;
+    x[1] = h10 * (h00 + h11 - s);
+    x[2] = h10 * h21;
+    for (int k = lo; k <= hi - 2; k++) {
+      double beta = house(x, 3, v);
+      if (beta != 0.0) {
+        int q = (k - 1 > lo) ? k - 1 : lo;
+        house_left(H, n, v, 3, beta, k, q, hi);
+        int r = (k + 3 < hi) ? k + 3 : hi;
+        house_right(H, n, v, 3, beta, lo, r, k);
+      }
+      x[0] = H[(k + 1) * n + k];
+      x[1] = H[(k + 2) * n + k];
+      if (k < hi - 2) x[2] = H[(k + 3) * n + k];
+    }
+    double beta = house(x, 2, v);
+    if (beta != 0.0) {
+      house_left(H, n, v, 2, beta, hi - 1, hi - 2, hi);
+      house_right(H, n, v, 2, beta, lo, hi, hi - 1);
+    }
+  }
+  return best * scale;
+}
+
+/* A = dF/dQ by forward differences + B (system.cpp:6-26, NumericalDiff.h:6-44):
+ * h_i = max(sqrt(eps)|q_i|, sqrt(eps)), column i = (F(q + h_i e_i) - F(q)) / h_i.
+ * f0 (optional) receives F(q). */
+static void system_matrix(oracle_F F, oracle_B B, const double *q,
+                          const double *dq, int d, int V, double *A,
+                          double *f0) {
+  double qq[MAXV], dqq[MAXD * MAXV], v1[MAXV], v2[MAXV];
+  memcpy(qq, q, V * sizeof(double));
+  memcpy(dqq, dq, MAXD * V * sizeof(double));
+  if (F) {
+    F(v1, qq, dqq, d);
+    for (int i = 0; i < V; i++) {
+      double qi0 = qq[i];
+      double h = fmax(FD_EPS * fabs(qi0), FD_EPS);
+      qq[i] = qi0 + h;
+      F(v2, qq, dqq, d);
+      qq[i] = qi0;
+      for (int r = 0; r < V; r++) A[r * V + i] = (v2[r] - v1[r]) / h;
+    }
+    if (f0) memcpy(f0, v1, V * sizeof(double));
+  } else {
+    for (int i = 0; i < V * V; i++) A[i] = 0.0;
+  }
+  if (B) {
+    double b[MAXV * MAXV];
+    B(b, qq, d);
+    for (int i = 0; i < V * V; i++) A[i] = A[i] + b[i];
+  }
+}
+
+void oracle_system_matrix(oracle_F F, oracle_B B, const double *q,
+                          const double *dq, int d, int ndim, int V,
+                          double *out) {
+  double dqq[MAXD * MAXV] = {0};
+  memcpy(dqq, dq, ndim * V * sizeof(double));
+  system_matrix(F, B, q, dqq, d, V, out, NULL);
+}
+
+static double max_abs_eigs(oracle_F F, oracle_B B, const double *q,
+                           const double *dq, int d, int V, double *f0) {
+  double A[MAXV * MAXV];
+  system_matrix(F, B, q, dq, d, V, A, f0);
+  return oracle_max_abs_eig(A, V);
+}
+
+double oracle_exp(double x) {
+  /* Same method as pde_exp in pypde_amd/systems/pde_prelude.h (which system
+   * sources call); duplicated here so the checker is self-contained. */
+  if (x > 709.782712893384) return 1.0 / 0.0;
+  if (x < -745.1332191019412) return 0.0;
+  double k = rint(x * 1.4426950408889634);
+  double r = fma(-k, 0.6931471805599453, x);
+  r = fma(-k, 2.3190468138462996e-17, r);
+  static const double c[13] = {
+      2.08767569878681e-09, 2.505210838544172e-08, 2.755731922398589e-07,
+      2.7557319223985893e-06, 2.48015873015873e-05, 0.0001984126984126984,
+      0.001388888888888889, 0.008333333333333333, 0.041666666666666664,
+      0.16666666666666666, 0.5, 1.0, 1.0};
+  double p = 1.6059043836821613e-10;
+  for (int i = 0; i < 13; i++) p = fma(p, r, c[i]);
+  return ldexp(p, (int)k);
+}
+
+/* ------------------------------------------------------------ boundaries */
+
+/* Ghost index map (boundaries.cpp:7-23); reflective walls (code 2) are this
+ * build's addition: mirror image across the wall, signs from `reflect`. */
+static int ghost_src(int i, int N, int type, int n, int *flip) {
+  int r = i - N;
+  *flip = 0;
+  if (r < 0) {
+    if (type == 0) return 0;
+    if (type == 1) return n + r;
+    *flip = 1;
+    return -r - 1;
+  }
+  if (r >= n) {
+    if (type == 0) return n - 1;
+    if (type == 1) return r - n;
+    *flip = 1;
+    return 2 * n - r - 1;
+  }
+  return r;
+}
+
+int oracle_boundaries(double *ub, const double *u, const int *nX, int ndim,
+                      const int *bc, const double *reflect, int N, int V) {
+  int nb[MAXD], ind[MAXD];
+  long ncell = 1;
+  for (int d = 0; d < ndim; d++) {
+    nb[d] = nX[d] + 2 * N;
+    ncell *= nb[d];
+    ind[d] = 0;
+  }
+  for (long c = 0; c < ncell; c++) {
+    long src = 0;
+    double sgn[MAXV];
+    for (int v = 0; v < V; v++) sgn[v] = 1.0;
+    for (int d = 0; d < ndim; d++) {
+      int flip;
+      int s = ghost_src(ind[d], N, bc[d], nX[d], &flip);
+      if (s < 0 || s >= nX[d]) return 1;
+      src = src * nX[d] + s;
+      if (flip && reflect)
+        for (int v = 0; v < V; v++) sgn[v] = sgn[v] * reflect[d * V + v];
+    }
+    for (int v = 0; v < V; v++) ub[c * V + v] = u[src * V + v] * sgn[v];
+    for (int d = ndim - 1; d >= 0; d--) {
+      if (++ind[d] < nb[d]) break;
+      ind[d] = 0;
+    }
+  }
+  return 0;
+}
+
+/* ----------------------------------------------------------------- WENO */
+
+/* One line window d[0..2N-2] (stride `st`) -> N nodal values (weno.cpp:26-59):
+ * for each stencil c = M^-1 data, p = c^T SIG c + EPS, o = LAM / p^8,
+ * num += o c, den += o; result num / den. */
+static void weno_line(const Tab *T, const double *d, long st, double *out) {
+  int N = T->N;
+  double num[MAXN], den = 0.0;
+  for (int k = 0; k < N; k++) num[k] = 0.0;
+  for (int s = 0; s < T->nsten; s++) {
+    const double *dd = d + (long)T->soff[s] * st;
+    double c[MAXN];
+    for (int k = 0; k < N; k++) {
+      double acc = T->winv[s][k][0] * dd[0];
+      for (int j = 1; j < N; j++) acc = fma(T->winv[s][k][j], dd[(long)j * st], acc);
+      c[k] = acc;
+    }
+    double p = 0.0;
+    for (int k = 0; k < N; k++) {
+      double t = T->sig[k][0] * c[0];
+      for (int j = 1; j < N; j++) t = fma(T->sig[k][j], c[j], t);
+      p = (k == 0) ? c[0] * t : fma(c[k], t, p);
+    }
+    p = p + WENO_EPS;
+    double p2 = p * p;
+    double p4 = p2 * p2;
+    double p8 = p4 * p4;
+    double o = T->slam[s] / p8;
+    for (int k = 0; k < N; k++) num[k] = fma(o, c[k], num[k]);
+    den = den + o;
+  }
+  for (int k = 0; k < N; k++) out[k] = num[k] / den;
+}
+
+/* Dimension-by-dimension reconstruction (weno.cpp:61-121).  rec has shape
+ * `shape` (cells) x V; result shape shrinks by 2(N-1) per axis and gains N^D
+ * node indices. */
+static double *weno_sweeps(const Tab *T, const double *ub, const int *shape_in) {
+  int N = T->N, V = T->V, D = T->D;
+  int shape[MAXD];
+  long ncell = 1;
+  for (int d = 0; d < D; d++) {
+    shape[d] = shape_in[d];
+    ncell *= shape[d];
+  }
+  double *rec = (double *)malloc(sizeof(double) * ncell * V);
+  memcpy(rec, ub, sizeof(double) * ncell * V);
+  for (int d = 0; d < D; d++) {
+    long n1 = 1, n3 = 1;
+    for (int i = 0; i < d; i++) n1 *= shape[i];
+    long n2 = shape[d] - 2 * (N - 1);
+    for (int i = d + 1; i < D; i++) n3 *= shape[i];
+    long n4 = ipow(N, d);
+    double *rec2 = (double *)malloc(sizeof(double) * n1 * n2 * n3 * n4 * N * V);
+    long stride = n3 * n4 * V;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (long i1 = 0; i1 < n1; i1++)
+      for (long i3 = 0; i3 < n3; i3++)
+        for (long i4 = 0; i4 < n4; i4++) {
+          long indu = ((i1 * shape[d] * n3 + i3) * n4 + i4) * V;
+          long indw = ((i1 * n2 * n3 + i3) * n4 + i4) * N * V;
+          for (long i = 0; i < n2; i++)
+            for (int v = 0; v < V; v++) {
+              double out[MAXN];
+              weno_line(T, rec + indu + i * stride + v, stride, out);
+              for (int k = 0; k < N; k++)
+                rec2[indw + i * N * stride + k * V + v] = out[k];
+            }
+        }
+    free(rec);
+    rec = rec2;
+    shape[d] = (int)n2;
+  }
+  return rec;
+}
+
+int oracle_weno(double *wh, const double *u, const int *nX, int ndim, int N,
+                int V) {
+  Tab T;
+  if (tab_init(&T, N, ndim, V)) return 1;
+  long nout = 1;
+  for (int d = 0; d < ndim; d++) {
+    if (nX[d] - 2 * (N - 1) < 1) return 2;
+    nout *= nX[d] - 2 * (N - 1);
+  }
+  double *r = weno_sweeps(&T, u, nX);
+  memcpy(wh, r, sizeof(double) * nout * T.Nd * V);
+  free(r);
+  return 0;
+}
+
+/* ------------------------------------------------------------- timestep */
+
+/* CFL step (stepper.cpp:35-76) over every reconstructed cell, the ghost layer
+ * included (iterator.cpp:76): cell mean q and gradient dq by the tensor Gauss
+ * weights, MAX = max_cells sum_d lambda_d / dX_d, dt = CFL / MAX, x0.2 while
+ * count <= 5, clamped to tf - t. */
+static double cell_speed(const Tab *T, oracle_F F, oracle_B B,
+                         const double *wc, const double *dX) {
+  int V = T->V, D = T->D, Nd = T->Nd;
+  double q[MAXV], dq[MAXD * MAXV] = {0}, tmp[MAXV];
+  for (int v = 0; v < V; v++) {
+    double acc = T->WP[0] * wc[v];
+    for (int idx = 1; idx < Nd; idx++) acc = fma(T->WP[idx], wc[idx * V + v], acc);
+    q[v] = acc;
+  }
+  for (int d = 0; d < D; d++) {
+    double acc[MAXV];
+    for (int idx = 0; idx < Nd; idx++) {
+      node_deriv(T, wc, idx, d, dX[d], tmp);
+      for (int v = 0; v < V; v++)
+        acc[v] = (idx == 0) ? T->WP[0] * tmp[v] : fma(T->WP[idx], tmp[v], acc[v]);
+    }
+    for (int v = 0; v < V; v++) dq[d * V + v] = acc[v];
+  }
+  double sum = 0.0;
+  for (int d = 0; d < D; d++) {
+    double lam = max_abs_eigs(F, B, q, dq, d, V, NULL);
+    sum = sum + lam / dX[d];
+  }
+  return sum;
+}
+
+double oracle_timestep(oracle_F F, oracle_B B, const double *wh, long nw,
+                       const double *dX, int ndim, int N, int V, double CFL,
+                       double tf, double t, long count) {
+  Tab T;
+  if (tab_init(&T, N, ndim, V)) return NAN;
+  double MAX = 0.0;
+#pragma omp parallel for reduction(max : MAX) schedule(static)
+  for (long c = 0; c < nw; c++) {
+    double s = cell_speed(&T, F, B, wh + c * T.Nd * V, dX);
+    if (s > MAX) MAX = s;
+  }
+  double dt = CFL / MAX;
+  if (count <= 5) dt = dt * 0.2;
+  if (t + dt > tf) return tf - t;
+  return dt;
+}
+
+/* ------------------------------------------------------------ predictor */
+
+typedef struct {
+  const Tab *T;
+  oracle_F F;
+  oracle_B B;
+  oracle_S S;
+  oracle_M M;
+  const double *dX;
+  double dt;
+  double *dq; /* workspace: MAXND * MAXD * MAXV */
+  double *f;  /* workspace: MAXD * MAXND * MAXV */
+  int b_quirk;
+} Ctx;
+
+/* B(q) applied to a gradient dq.  Default: the matrix-vector product of the
+ * documented PDE (solvers.py:108-121).  b_quirk = 1 reproduces what the
+ * reference computes at dg.cpp:110 (row destination): `b * dq.row(i)` has
+ * mismatched inner dimensions, which -DNDEBUG leaves unchecked, and Eigen
+ * evaluates it as a depth-1 outer product, b(0,0) dq[v].  b_quirk = 2 is the
+ * column-destination form of fv.cpp:234, b(v,0) dq[0]. */
+static void b_times(const double *b, const double *dqd, int V, int quirk,
+                    double *out) {
+  for (int v = 0; v < V; v++) {
+    if (quirk == 1) {
+      out[v] = b[0] * dqd[v];
+    } else if (quirk == 2) {
+      out[v] = b[v * V] * dqd[0];
+    } else {
+      double acc = b[v * V] * dqd[0];
+      for (int j = 1; j < V; j++) acc = fma(b[v * V + j], dqd[j], acc);
+      out[v] = acc;
+    }
+  }
+}
+
+/* rhs(q) of the space-time system (dg.cpp:50-126):
+ * r = W_t prod W_x (S(q) - sum_d (B_d(q) dq_d + d_d F_d(q, dq))), then
+ * rhs = r dt + Ww with Ww[t][x] = psi_t(0) prod W_x w[x] (dg.cpp:156-171). */
+static void dg_rhs(const Ctx *C, const double *w, const double *q, double *rhs) {
+  const Tab *T = C->T;
+  int N = T->N, D = T->D, V = T->V, Nd = T->Nd;
+  double *dq = C->dq, *f = C->f;
+  for (int t = 0; t < N; t++) {
+    const double *qt = q + (long)t * Nd * V;
+    for (int idx = 0; idx < Nd; idx++)
+      for (int d = 0; d < D; d++)
+        node_deriv(T, qt, idx, d, C->dX[d], dq + (idx * MAXD + d) * V);
+    if (C->F)
+      for (int idx = 0; idx < Nd; idx++) {
+        double qq[MAXV], dqq[MAXD * MAXV] = {0};
+        memcpy(qq, qt + idx * V, V * sizeof(double));
+        memcpy(dqq, dq + idx * MAXD * V, D * V * sizeof(double));
+        for (int d = 0; d < D; d++) C->F(f + (d * MAXND + idx) * V, qq, dqq, d);
+      }
+    for (int idx = 0; idx < Nd; idx++) {
+      double r[MAXV], qq[MAXV];
+      memcpy(qq, qt + idx * V, V * sizeof(double));
+      if (C->S)
+        C->S(r, qq);
+      else
+        for (int v = 0; v < V; v++) r[v] = 0.0;
+      for (int d = 0; d < D; d++) {
+        const double *dqd = dq + (idx * MAXD + d) * V;
+        if (C->B) {
+          double b[MAXV * MAXV], bd[MAXV];
+          C->B(b, qq, d);
+          b_times(b, dqd, V, C->b_quirk ? 1 : 0, bd);
+          for (int v = 0; v < V; v++) r[v] = r[v] - bd[v];
+        }
+        if (C->F) {
+          double df[MAXV];
+          node_deriv(T, f + d * MAXND * V, idx, d, C->dX[d], df);
+          for (int v = 0; v < V; v++) r[v] = r[v] - df[v];
+        }
+      }
+      double c = T->w[t];
+      for (int d = 0; d < D; d++) c = c * T->w[T->kk[idx][d]];
+      double c0 = T->end0[t];
+      for (int d = 0; d < D; d++) c0 = c0 * T->w[T->kk[idx][d]];
+      for (int v = 0; v < V; v++) {
+        double rr = r[v] * c;
+        rhs[((long)t * Nd + idx) * V + v] = rr * C->dt + c0 * w[idx * V + v];
+      }
+    }
+  }
+}
+
+/* Picard iteration q1 = DG_U^-1 rhs(q0) (dg.cpp:187-202), DG_U = DG_MAT (x) diag W
+ * (x) ... (dg.cpp:42-47) applied as DG_MAT^-1 along time and 1/prod W per node.
+ * Continues while any |q1 - q0| > DG_TOL (1 + |q0|), at most DG_IT times. */
+static int picard_cell(const Ctx *C, const double *w, double *q, double *rhs,
+                       double *q1) {
+  const Tab *T = C->T;
+  int N = T->N, V = T->V, Nd = T->Nd;
+  long NTV = (long)T->NT * V;
+  int it;
+  for (it = 0; it < DG_IT; it++) {
+    dg_rhs(C, w, q, rhs);
+    int more = 0;
+    for (int t = 0; t < N; t++)
+      for (int idx = 0; idx < Nd; idx++)
+        for (int v = 0; v < V; v++) {
+          double acc = T->dginv[t][0] * rhs[(long)idx * V + v];
+          for (int s = 1; s < N; s++)
+            acc = fma(T->dginv[t][s], rhs[((long)s * Nd + idx) * V + v], acc);
+          long e = ((long)t * Nd + idx) * V + v;
+          q1[e] = acc * T->IWP[idx];
+          if (fabs(q1[e] - q[e]) > DG_TOL * (1.0 + fabs(q[e]))) more = 1;
+        }
+    if (!more) {
+      memcpy(q, q1, NTV * sizeof(double));
+      return it + 1;
+    }
+    memcpy(q, q1, NTV * sizeof(double));
+  }
+  return it;
+}
+
+/* LU with partial pivoting of a V x V matrix, then solve for n right-hand sides
+ * x_t = -P^-1 h_t. */
+static void lu_solve_neg(int V, double *P, int n, double h[MAXN][MAXV]) {
+  int piv[MAXV];
+  for (int k = 0; k < V; k++) {
+    int p = k;
+    for (int i = k + 1; i < V; i++)
+      if (fabs(P[i * V + k]) > fabs(P[p * V + k])) p = i;
+    piv[k] = p;
+    if (p != k)
+      for (int j = 0; j < V; j++) {
+        double t = P[k * V + j];
+        P[k * V + j] = P[p * V + j];
+        P[p * V + j] = t;
+      }
+    for (int i = k + 1; i < V; i++) {
+      double l = P[i * V + k] / P[k * V + k];
+      P[i * V + k] = l;
+      for (int j = k + 1; j < V; j++) P[i * V + j] = fma(-l, P[k * V + j], P[i * V + j]);
+    }
+  }
+  for (int t = 0; t < n; t++) {
+    double *b = h[t];
+    for (int k = 0; k < V; k++) {
+      int p = piv[k];
+      if (p != k) {
+        double tmp = b[k];
+        b[k] = b[p];
+        b[p] = tmp;
+      }
+    }
+    for (int i = 1; i < V; i++) {
+      double acc = b[i];
+      for (int j = 0; j < i; j++) acc = fma(-P[i * V + j], b[j], acc);
+      b[i] = acc;
+    }
+    for (int i = V - 1; i >= 0; i--) {
+      double acc = b[i];
+      for (int j = i + 1; j < V; j++) acc = fma(-P[i * V + j], b[j], acc);
+      b[i] = -(acc / P[i * V + i]);
+    }
+  }
+}
+
+/* Stiff predictor: Newton iteration on obj(q) = rhs(q) - (DG_MAT (x) W) q = 0
+ * (dg.cpp:128-154, whose root the reference finds by Newton-Krylov,
+ * dg.cpp:173-185, newton_krylov.cpp:144-183).
+ *
+ * This build's Jacobian keeps the exact time coupling and the stiff source,
+ * frozen per spatial node at the time-weighted mean state (flux and B terms
+ * lagged):  J = W_x diag(W_t) (x) I (dt JS - Tw), Tw = diag(1/W) DG_MAT, solved
+ * through (Tw - A)^-1 = -adj(xI - Tw)(A) p(A)^-1 with A = dt JS (the entries of
+ * the polynomial matrix commute).  The fixed point does not depend on J.
+ *
+ * Learned initial guess (guess != NULL): if the cell residual max|obj| after
+ * the first step is larger than at the guess itself, the guess is abandoned
+ * and the iteration restarts from the plain guess (w broadcast in time).
+ * Returns the number of residual evaluations; *fellback reports a restart. */
+static void init_guess(const Tab *T, const double *w, double *q);
+
+static int newton_cell(const Ctx *C, const double *w, double *q, double *rhs,
+                       int learned, double tol, int maxit, int *fellback) {
+  const Tab *T = C->T;
+  int N = T->N, V = T->V, Nd = T->Nd;
+  double r0 = 0.0;
+  int it;
+  *fellback = 0;
+  for (it = 0; it < maxit; it++) {
+    dg_rhs(C, w, q, rhs);
+    /* residual G (kept in rhs) and its cell max-norm */
+    double rmax = 0.0;
+    for (int idx = 0; idx < Nd; idx++) {
+      double wp = T->WP[idx];
+      for (int t = 0; t < N; t++)
+        for (int v = 0; v < V; v++) {
+          double acc = (T->dgmat[t][0] * wp) * q[(long)idx * V + v];
+          for (int s = 1; s < N; s++)
+            acc = fma(T->dgmat[t][s] * wp, q[((long)s * Nd + idx) * V + v], acc);
+          long e = ((long)t * Nd + idx) * V + v;
+          rhs[e] = rhs[e] - acc;
+          rmax = fmax(rmax, fabs(rhs[e]));
+        }
+    }
+    if (learned) {
+      if (it == 0) {
+        r0 = rmax;
+      } else {
+        learned = 0;
+        if (!(rmax <= r0)) {
+          init_guess(T, w, q);
+          *fellback = 1;
+          continue;
+        }
+      }
+    }
+    int more = 0;
+    for (int idx = 0; idx < Nd; idx++) {
+      double g[MAXN][MAXV], h[MAXN][MAXV], qbar[MAXV], JS[MAXV * MAXV], A[MAXV * MAXV],
+          P[MAXV * MAXV];
+      double wp = T->WP[idx];
+      for (int t = 0; t < N; t++)
+        for (int v = 0; v < V; v++)
+          g[t][v] = rhs[((long)t * Nd + idx) * V + v] * (1.0 / (T->w[t] * wp));
+      for (int v = 0; v < V; v++) {
+        double acc = T->w[0] * q[(long)idx * V + v];
+        for (int t = 1; t < N; t++) acc = fma(T->w[t], q[((long)t * Nd + idx) * V + v], acc);
+        qbar[v] = acc;
+      }
+      if (C->M) {
+        C->M(JS, qbar);
+      } else if (C->S) {
+        double s0[MAXV], s1[MAXV], qq[MAXV];
+        memcpy(qq, qbar, V * sizeof(double));
+        C->S(s0, qq);
+        for (int j = 0; j < V; j++) {
+          double q0 = qq[j];
+          double hh = fmax(FD_EPS * fabs(q0), FD_EPS);
+          qq[j] = q0 + hh;
+          C->S(s1, qq);
+          qq[j] = q0;
+          for (int i = 0; i < V; i++) JS[i * V + j] = (s1[i] - s0[i]) / hh;
+        }
+      } else {
+        for (int i = 0; i < V * V; i++) JS[i] = 0.0;
+      }
+      for (int i = 0; i < V * V; i++) A[i] = C->dt * JS[i];
+      /* h_t = sum_j A^j u_{t,j},  u_{t,j} = sum_s M_{N-j}[t][s] g_s  (Horner in A) */
+      for (int t = 0; t < N; t++) {
+        for (int j = N - 1; j >= 0; j--) {
+          double u[MAXV];
+          const double(*Mk)[MAXN] = T->flm[N - j - 1];
+          for (int v = 0; v < V; v++) {
+            double acc = Mk[t][0] * g[0][v];
+            for (int s = 1; s < N; s++) acc = fma(Mk[t][s], g[s][v], acc);
+            u[v] = acc;
+          }
+          if (j == N - 1) {
+            for (int v = 0; v < V; v++) h[t][v] = u[v];
+          } else {
+            double Ah[MAXV];
+            for (int i = 0; i < V; i++) {
+              double acc = A[i * V] * h[t][0];
+              for (int k = 1; k < V; k++) acc = fma(A[i * V + k], h[t][k], acc);
+              Ah[i] = acc;
+            }
+            for (int v = 0; v < V; v++) h[t][v] = Ah[v] + u[v];
+          }
+        }
+      }
+      /* P = p(A) = A^N + c_{N-1} A^{N-1} + ... + c_0 I  (Horner) */
+      for (int i = 0; i < V * V; i++) P[i] = A[i];
+      for (int i = 0; i < V; i++) P[i * V + i] = P[i * V + i] + T->flc[N - 1];
+      for (int j = N - 2; j >= 0; j--) {
+        double PA[MAXV * MAXV];
+        for (int i = 0; i < V; i++)
+          for (int k = 0; k < V; k++) {
+            double acc = P[i * V] * A[k];
+            for (int m = 1; m < V; m++) acc = fma(P[i * V + m], A[m * V + k], acc);
+            PA[i * V + k] = acc;
+          }
+        for (int i = 0; i < V * V; i++) P[i] = PA[i];
+        for (int i = 0; i < V; i++) P[i * V + i] = P[i * V + i] + T->flc[j];
+      }
+      lu_solve_neg(V, P, N, h);
+      for (int t = 0; t < N; t++)
+        for (int v = 0; v < V; v++) {
+          long e = ((long)t * Nd + idx) * V + v;
+          if (!(fabs(h[t][v]) <= tol * (1.0 + fabs(q[e])))) more = 1;
+          q[e] = q[e] + h[t][v];
+        }
+    }
+    if (!more) return it + 1;
+  }
+  return it;
+}
+
+static void init_guess(const Tab *T, const double *w, double *q) {
+  for (int t = 0; t < T->N; t++)
+    memcpy(q + (long)t * T->Nd * T->V, w, sizeof(double) * T->Nd * T->V);
+}
+
+typedef struct {
+  long iters, cells, maxit, capped, fallbacks;
+} PredStats;
+
+static int predictor_impl(const Tab *T, oracle_F F, oracle_B B, oracle_S S,
+                          oracle_M M, const double *wh, long nw,
+                          const double *dX, double dt, int stiff,
+                          const oracle_opts *opts, double *qh, int *iters,
+                          PredStats *ps) {
+  long NTV = (long)T->NT * T->V, NdV = (long)T->Nd * T->V;
+  double tol = (opts && opts->newton_tol > 0) ? opts->newton_tol : 1e-12;
+  int maxit = (opts && opts->newton_maxit > 0) ? opts->newton_maxit : 100;
+  const double *guess = opts ? opts->guess : NULL;
+  int gcheck = opts ? opts->guess_check : 0;
+  long s_it = 0, s_max = 0, s_cap = 0, s_fb = 0;
+#pragma omp parallel reduction(+ : s_it, s_cap, s_fb) reduction(max : s_max)
+  {
+    double *ws = (double *)malloc(sizeof(double) * (2 * MAXND * MAXD * MAXV + 2 * NTV));
+    Ctx C = {T, F, B, S, M, dX, dt, ws, ws + MAXND * MAXD * MAXV, opts ? opts->b_quirk : 0};
+    double *rhs = ws + 2 * MAXND * MAXD * MAXV, *q1 = rhs + NTV;
+#pragma omp for schedule(dynamic, 16)
+    for (long c = 0; c < nw; c++) {
+      double *q = qh + c * NTV;
+      const double *w = wh + c * NdV;
+      int it;
+      if (stiff) {
+        int fb = 0;
+        if (guess) {
+          memcpy(q, guess + c * NTV, NTV * sizeof(double));
+          it = newton_cell(&C, w, q, rhs, gcheck, tol, maxit, &fb);
+        } else {
+          init_guess(T, w, q);
+          it = newton_cell(&C, w, q, rhs, 0, tol, maxit, &fb);
+        }
+        s_fb += fb;
+        if (it >= maxit) s_cap++;
+      } else {
+        init_guess(T, w, q);
+        it = picard_cell(&C, w, q, rhs, q1);
+        if (it >= DG_IT) s_cap++;
+      }
+      if (iters) iters[c] = it;
+      s_it += it;
+      if (it > s_max) s_max = it;
+    }
+    free(ws);
+  }
+  if (ps) {
+    ps->iters += s_it;
+    ps->cells += nw;
+    if (s_max > ps->maxit) ps->maxit = s_max;
+    ps->capped += s_cap;
+    ps->fallbacks += s_fb;
+  }
+  return 0;
+}
+
+int oracle_predictor(oracle_F F, oracle_B B, oracle_S S, oracle_M M,
+                     const double *wh, long nw, const double *dX, int ndim,
+                     int N, int V, double dt, int stiff, const oracle_opts *opts,
+                     double *qh, int *iters) {
+  Tab T;
+  if (tab_init(&T, N, ndim, V)) return 1;
+  return predictor_impl(&T, F, B, S, M, wh, nw, dX, dt, stiff, opts, qh, iters,
+                        NULL);
+}
+
+int oracle_dg_obj(oracle_F F, oracle_B B, oracle_S S, const double *w,
+                  const double *q, const double *dX, int ndim, int N, int V,
+                  double dt, int b_quirk, double *res) {
+  Tab T;
+  if (tab_init(&T, N, ndim, V)) return 1;
+  double *ws = (double *)malloc(sizeof(double) * 2 * MAXND * MAXD * MAXV);
+  Ctx C = {&T, F, B, S, NULL, dX, dt, ws, ws + MAXND * MAXD * MAXV, b_quirk};
+  dg_rhs(&C, w, q, res);
+  free(ws);
+  for (int t = 0; t < N; t++)
+    for (int idx = 0; idx < T.Nd; idx++)
+      for (int v = 0; v < V; v++) {
+        double acc = 0.0;
+        for (int s = 0; s < N; s++)
+          acc += T.dgmat[t][s] * T.WP[idx] * q[((long)s * T.Nd + idx) * V + v];
+        res[((long)t * T.Nd + idx) * V + v] -= acc;
+      }
+  return 0;
+}
+
+/* ------------------------------------------------------------------- FV */
+
+/* Face trace of a cell's space-time tensor: line along axis d through the
+ * transverse node j at end e (evaluations.cpp:49-70). */
+static void face_trace(const Tab *T, const double *qc, int d, int e, int t,
+                       int j, double *out) {
+  int N = T->N, V = T->V, D = T->D;
+  int stride = ipow(N, D - 1 - d);
+  /* node index of (transverse j, k_d = 0): split j into the dims before/after d */
+  int hi = j / stride, lo = j % stride;
+  int base = hi * stride * N + lo;
+  const double *ev = e ? T->end1 : T->end0;
+  const double *qt = qc + (long)t * T->Nd * V;
+  for (int v = 0; v < V; v++) {
+    double acc = ev[0] * qt[base * V + v];
+    for (int m = 1; m < N; m++) acc = fma(ev[m], qt[(base + m * stride) * V + v], acc);
+    out[v] = acc;
+  }
+}
+
+/* Rusanov: f = max(lam(qL), lam(qR)) (qL - qR) + (F(qL) + F(qR))
+ * (fluxes.cpp:72-101); Bint = (sum_i W_i B(qL + x_i (qR - qL))) (qR - qL)
+ * (fluxes.cpp:103-116). */
+static void face_flux(const Tab *T, oracle_F F, oracle_B B, int flux,
+                      const double *qL, const double *qR, int d, double *f,
+                      double *bint) {
+  int V = T->V, N = T->N;
+  double dq0[MAXD * MAXV] = {0};
+  if (F) {
+    double fl[MAXV], fr[MAXV];
+    double l1 = max_abs_eigs(F, B, qL, dq0, d, V, fl);
+    double l2 = max_abs_eigs(F, B, qR, dq0, d, V, fr);
+    double s = fmax(l1, l2);
+    (void)flux;
+    for (int v = 0; v < V; v++) f[v] = s * (qL[v] - qR[v]) + (fl[v] + fr[v]);
+  } else {
+    for (int v = 0; v < V; v++) f[v] = 0.0;
+  }
+  if (B) {
+    double Dq[MAXV], Mx[MAXV * MAXV], b[MAXV * MAXV], q[MAXV];
+    for (int v = 0; v < V; v++) Dq[v] = qR[v] - qL[v];
+    for (int i = 0; i < N; i++) {
+      for (int v = 0; v < V; v++) q[v] = fma(T->x[i], Dq[v], qL[v]);
+      B(b, q, d);
+      for (int k = 0; k < V * V; k++) Mx[k] = (i == 0) ? T->w[0] * b[k] : fma(T->w[i], b[k], Mx[k]);
+    }
+    for (int v = 0; v < V; v++) {
+      double acc = Mx[v * V] * Dq[0];
+      for (int k = 1; k < V; k++) acc = fma(Mx[v * V + k], Dq[k], acc);
+      bint[v] = acc;
+    }
+  } else {
+    for (int v = 0; v < V; v++) bint[v] = 0.0;
+  }
+}
+
+/* Volume term (fv.cpp:33-85): sum over nodes of dt W_t prod W_x (S(qh) - sum_d
+ * B_d(qh) d_d qh).  Summed per spatial node over t, then over nodes. */
+static void cell_volume(const Tab *T, oracle_F F, oracle_B B, oracle_S S,
+                        const double *qc, const double *dX, double dt, int quirk,
+                        double *vol) {
+  int N = T->N, V = T->V, D = T->D, Nd = T->Nd;
+  (void)F;
+  for (int v = 0; v < V; v++) vol[v] = 0.0;
+  for (int idx = 0; idx < Nd; idx++) {
+    double part[MAXV];
+    for (int t = 0; t < N; t++) {
+      const double *qt = qc + (long)t * Nd * V;
+      double s[MAXV], qq[MAXV];
+      memcpy(qq, qt + idx * V, V * sizeof(double));
+      if (S)
+        S(s, qq);
+      else
+        for (int v = 0; v < V; v++) s[v] = 0.0;
+      if (B)
+        for (int d = 0; d < D; d++) {
+          double dqd[MAXV], b[MAXV * MAXV], bd[MAXV];
+          node_deriv(T, qt, idx, d, dX[d], dqd);
+          B(b, qq, d);
+          b_times(b, dqd, V, quirk ? 2 : 0, bd);
+          for (int v = 0; v < V; v++) s[v] = s[v] - bd[v];
+        }
+      double tmp = dt * T->w[t];
+      for (int d = 0; d < D; d++) tmp = tmp * T->w[T->kk[idx][d]];
+      for (int v = 0; v < V; v++) part[v] = (t == 0) ? tmp * s[v] : fma(tmp, s[v], part[v]);
+    }
+    for (int v = 0; v < V; v++) vol[v] = (idx == 0) ? part[v] : vol[v] + part[v];
+  }
+}
+
+/* Corrector (fv.cpp:200-207 = centers then interfaces, fv.cpp:124-198).  For
+ * an interior cell c the reference's face-cell loop visits, in order, the
+ * faces c+1-e_d (c on the right, d = 0..D-1, each over t then transverse
+ * node) and then the face-cell c+1 (c on the left, over t, then d, then
+ * transverse node).  Face fluxes are evaluated once per face point and then
+ * gathered per cell in exactly that order. */
+int oracle_fv(oracle_F F, oracle_B B, oracle_S S, double *u, const double *qh,
+              const int *nX, int ndim, const double *dX, int flux, int N, int V,
+              double dt, int b_quirk) {
+  Tab T;
+  if (tab_init(&T, N, ndim, V)) return 1;
+  int D = ndim, Nd1 = T.Nd1;
+  long NTV = (long)T.NT * V;
+  int n2[MAXD];
+  long ncell = 1;
+  for (int d = 0; d < D; d++) {
+    n2[d] = nX[d] + 2;
+    ncell *= nX[d];
+  }
+  /* face arrays per d: shape (nX_d + 1) along d, nX_d' across; entries
+   * [face][t][j][v] */
+  double *ff[MAXD], *fb[MAXD];
+  long nface[MAXD];
+  for (int d = 0; d < D; d++) {
+    long nf = 1;
+    for (int e = 0; e < D; e++) nf *= (e == d) ? nX[e] + 1 : nX[e];
+    nface[d] = nf;
+    ff[d] = (double *)malloc(sizeof(double) * nf * N * Nd1 * V);
+    fb[d] = (double *)malloc(sizeof(double) * nf * N * Nd1 * V);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long f = 0; f < nf; f++) {
+      int fc[MAXD];
+      long r = f;
+      for (int e = D - 1; e >= 0; e--) {
+        int ext = (e == d) ? nX[e] + 1 : nX[e];
+        fc[e] = (int)(r % ext);
+        r /= ext;
+      }
+      /* left w-cell (nX+2 coordinates) = face-cell F with F_e = fc_e + 1 across */
+      long wl = 0, wr = 0;
+      for (int e = 0; e < D; e++) {
+        int Fe = (e == d) ? fc[e] : fc[e] + 1;
+        wl = wl * n2[e] + Fe;
+        wr = wr * n2[e] + Fe + (e == d);
+      }
+      for (int t = 0; t < N; t++)
+        for (int j = 0; j < Nd1; j++) {
+          double qL[MAXV], qR[MAXV];
+          face_trace(&T, qh + wl * NTV, d, 1, t, j, qL);
+          face_trace(&T, qh + wr * NTV, d, 0, t, j, qR);
+          long o = ((f * N + t) * Nd1 + j) * V;
+          face_flux(&T, F, B, flux, qL, qR, d, ff[d] + o, fb[d] + o);
+        }
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < ncell; c++) {
+    int cc[MAXD];
+    long r = c;
+    for (int e = D - 1; e >= 0; e--) {
+      cc[e] = (int)(r % nX[e]);
+      r /= nX[e];
+    }
+    double *uc = u + c * V;
+    if (B || S) {
+      long wc = 0;
+      for (int e = 0; e < D; e++) wc = wc * n2[e] + cc[e] + 1;
+      double vol[MAXV];
+      cell_volume(&T, F, B, S, qh + wc * NTV, dX, dt, b_quirk, vol);
+      for (int v = 0; v < V; v++) uc[v] = uc[v] + vol[v];
+    }
+    if (!(F || B)) continue;
+    long fidx[2][MAXD];
+    for (int d = 0; d < D; d++)
+      for (int side = 0; side < 2; side++) {
+        long f = 0;
+        for (int e = 0; e < D; e++) {
+          int ext = (e == d) ? nX[e] + 1 : nX[e];
+          int fe = cc[e] + ((e == d) ? side : 0);
+          f = f * ext + fe;
+        }
+        fidx[side][d] = f;
+      }
+    /* c is the right cell of faces c+1-e_d */
+    for (int d = 0; d < D; d++)
+      for (int t = 0; t < N; t++) {
+        double c0 = (dt * T.w[t]) / (2.0 * dX[d]);
+        for (int j = 0; j < Nd1; j++) {
+          double tmp = c0;
+          int rr = j;
+          int jj[MAXD];
+          for (int i = D - 2; i >= 0; i--) {
+            jj[i] = rr % N;
+            rr /= N;
+          }
+          for (int i = 0; i < D - 1; i++) tmp = tmp * T.w[jj[i]];
+          long o = ((fidx[0][d] * N + t) * Nd1 + j) * V;
+          for (int v = 0; v < V; v++) uc[v] = uc[v] - tmp * (fb[d][o + v] - ff[d][o + v]);
+        }
+      }
+    /* c is the left cell of face-cell c+1 */
+    for (int t = 0; t < N; t++)
+      for (int d = 0; d < D; d++) {
+        double c0 = (dt * T.w[t]) / (2.0 * dX[d]);
+        for (int j = 0; j < Nd1; j++) {
+          double tmp = c0;
+          int rr = j;
+          int jj[MAXD];
+          for (int i = D - 2; i >= 0; i--) {
+            jj[i] = rr % N;
+            rr /= N;
+          }
+          for (int i = 0; i < D - 1; i++) tmp = tmp * T.w[jj[i]];
+          long o = ((fidx[1][d] * N + t) * Nd1 + j) * V;
+          for (int v = 0; v < V; v++) uc[v] = uc[v] - tmp * (fb[d][o + v] + ff[d][o + v]);
+        }
+      }
+  }
+  for (int d = 0; d < D; d++) {
+    free(ff[d]);
+    free(fb[d]);
+  }
+  return 0;
+}
+
+/* ---------------------------------------------------------------- driver */
+
+/* Time loop of iterator.cpp:38-151 (serial over the whole grid; the reference's
+ * slab split changes no arithmetic). */
+int oracle_pde_solver(oracle_F F, oracle_B B, oracle_S S, oracle_M M, int useF,
+                      int useB, int useS, double *u, double tf, const int *nX,
+                      int ndim, const double *dX, double CFL, const int *bc,
+                      int stiff, int flux, int N, int V, int ndt,
+                      int secondOrder, double *ret, const oracle_opts *opts,
+                      oracle_stats *stats) {
+  if (!useF) F = NULL;
+  if (!useB) B = NULL;
+  if (!useS) S = NULL;
+  if (!S) M = NULL;
+  if (secondOrder) return 10; /* viscous F(Q, dQ) path not restated */
+  if (flux != 0) return 11;   /* Rusanov only */
+  if (ndt < 1) return 12;
+  Tab T;
+  if (tab_init(&T, N, ndim, V)) return 13;
+  for (int d = 0; d < ndim; d++)
+    if (bc[d] < 0 || bc[d] > 2) return 14;
+  long ncell = 1, nub = 1, nw = 1;
+  int nb[MAXD];
+  for (int d = 0; d < ndim; d++) {
+    ncell *= nX[d];
+    nb[d] = nX[d] + 2 * N;
+    nub *= nb[d];
+    nw *= nX[d] + 2;
+  }
+  long NTV = (long)T.NT * V;
+  double *ub = (double *)malloc(sizeof(double) * nub * V);
+  double *qh = (double *)malloc(sizeof(double) * nw * NTV);
+  const double *reflect = opts ? opts->reflect : NULL;
+  long max_steps = opts ? opts->max_steps : 0;
+  double t = 0.0;
+  long count = 0;
+  int push = 0, rc = 0;
+  PredStats ps = {0, 0, 0, 0, 0};
+  while (t < tf && (max_steps <= 0 || count < max_steps)) {
+    oracle_boundaries(ub, u, nX, ndim, bc, reflect, N, V);
+    double *wh = weno_sweeps(&T, ub, nb);
+    double dt = oracle_timestep(F, B, wh, nw, dX, ndim, N, V, CFL, tf, t, count);
+    predictor_impl(&T, F, B, S, M, wh, nw, dX, dt, stiff, opts, qh, NULL, &ps);
+    free(wh);
+    oracle_fv(F, B, S, u, qh, nX, ndim, dX, flux, N, V, dt, opts ? opts->b_quirk : 0);
+    t += dt;
+    count += 1;
+    if (push < ndt && t >= (double)(push + 1) / (double)ndt * tf) {
+      memcpy(ret + (long)push * ncell * V, u, sizeof(double) * ncell * V);
+      push += 1;
+    }
+    int bad = 0;
+    for (long i = 0; i < ncell * V; i++)
+      if (u[i] != u[i]) bad = 1;
+    if (bad) {
+      rc = -(int)count;
+      break;
+    }
+  }
+  memcpy(ret + (long)(ndt - 1) * ncell * V, u, sizeof(double) * ncell * V);
+  if (stats) {
+    stats->steps = count;
+    stats->t = t;
+    stats->pred_iters = ps.iters;
+    stats->pred_cells = ps.cells;
+    stats->pred_maxit = ps.maxit;
+    stats->pred_capped = ps.capped;
+    stats->guess_fallbacks = ps.fallbacks;
+  }
+  free(ub);
+  free(qh);
+  return rc;
+}

@@ -1,0 +1,59 @@
+"""Writes the golden vectors under tests/golden/ from the reference's own C++
+core (oracle/_ref/libpypde_ref.so, compiled from the reference sources by
+oracle/Makefile).  Run from the repository root:  python tests/golden/make_golden.py
+
+tables.npz  basis/WENO/DG tables for N = 1..5 (poly/basis.cpp, weno_matrices.cpp,
+            dg_matrices.cpp)
+weno.npz    weno_solver (api.cpp:32-48) on seeded random data
+solves.npz  pde_solver end states: C0 Sod (order 3) plus the reference's own
+            1-ulp sensitivity on it, and 24x24 periodic 2D Euler, orders 2-4
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+import cpu_checkers as cc  # noqa: E402
+import test_oracle as to  # noqa: E402
+
+
+def main():
+    assert cc.ref_available(), "build oracle/_ref first (make -C oracle ref)"
+    t = {}
+    for n in [1, 2, 3, 4, 5]:
+        for k, v in cc.ref_tables(n).items():
+            t[f"{k}_{n}"] = v
+    np.savez_compressed(os.path.join(HERE, "tables.npz"), **t)
+
+    w = {}
+    for ndim, N, V in [(1, 2, 3), (1, 3, 4), (2, 2, 3), (2, 3, 5), (2, 4, 4)]:
+        rng = np.random.default_rng(100 + 10 * ndim + N)
+        nX = np.array([12, 9][:ndim], dtype=np.int32)
+        u = np.ascontiguousarray(rng.uniform(0.5, 2, tuple(nX) + (V,)))
+        nout = int(np.prod(nX - 2 * (N - 1)))
+        wh = np.zeros(nout * N ** ndim * V)
+        cc.ref().weno_solver(cc.dp(wh), cc.dp(u.copy()), cc.ip(nX), ndim, N, V)
+        w[f"u_{ndim}_{N}_{V}"] = u
+        w[f"wh_{ndim}_{N}_{V}"] = wh
+    np.savez_compressed(os.path.join(HERE, "weno.npz"), **w)
+
+    s = {}
+    u = to.sod1d()
+    ur, _ = cc.ref_solve("euler1d", u, 0.2, [1.0], 0, 3, nthreads=4)
+    u2 = u.copy()
+    u2[37, 0] = np.nextafter(u2[37, 0], 2.0)
+    ur2, _ = cc.ref_solve("euler1d", u2, 0.2, [1.0], 0, 3, nthreads=4)
+    s["sod_final"] = ur
+    s["sod_noise"] = np.array(to.relerr(ur2, ur))
+    for N in [2, 3, 4]:
+        ur, _ = cc.ref_solve("euler2d", to.euler2d_field(24, 24, noise=0.0), 0.05, [1, 1], 1, N,
+                             nthreads=8)
+        s[f"e2d_final_{N}"] = ur
+    np.savez_compressed(os.path.join(HERE, "solves.npz"), **s)
+    print("golden vectors written:", sorted(os.listdir(HERE)))
+
+
+if __name__ == "__main__":
+    main()
