@@ -1,0 +1,107 @@
+/* pypde_hip: MI355X-native ADER-WENO solver, C ABI.
+ *
+ * Drop-in for the compute path of haranjackson/PyPDE (libpypde.so).  The
+ * physics arrive as HIP device-function source (F, B, S and optionally
+ * M = dS/dQ; signatures in pypde_amd/systems/pde_prelude.h) instead of C
+ * function pointers, and are compiled with hipRTC together with kernels
+ * specialised for (ndim, N, V).
+ *
+ * Return codes: 0 success, >0 argument / compile / runtime errors (text via
+ * pypde_hip_last_error), <0 NaN detected at step -rc (the reference prints
+ * "NaNs found" and continues, iterator.cpp:141-145).
+ */
+#ifndef PYPDE_HIP_H
+#define PYPDE_HIP_H
+
+#include <stdbool.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYPDE_OK 0
+#define PYPDE_ERR_ARG 1
+#define PYPDE_ERR_COMPILE 2
+#define PYPDE_ERR_HIP 3
+#define PYPDE_ERR_UNSUPPORTED 4
+
+/* Options beyond the reference's argument list (defaults in brackets). */
+typedef struct {
+  int struct_size;          /* sizeof(pypde_options) */
+  double newton_tol;        /* stiff predictor: |dq| <= tol (1 + |q|) elementwise [1e-12] */
+  int newton_maxit;         /* stiff predictor iteration cap [100] */
+  long max_steps;           /* stop after this many steps even if t < tf; <= 0: none [0] */
+  const double *reflect;    /* ndim*V multipliers for reflective walls (bc 2) [all +1] */
+  int b_quirk;              /* 1: B.dQ evaluated as the reference does (dg.cpp:110,
+                               fv.cpp:234) instead of the matrix-vector product [0] */
+  int use_M;                /* the source defines M(out, Q) = dS/dQ [0: finite differences] */
+  int device;               /* HIP device ordinal [0] */
+  int profile;              /* time every kernel with HIP events [0] */
+  const double *guess;      /* host array: predictor initial guesses for the first step,
+                               nwc x N^(ndim+1) x V (testing the learned-guess path) */
+} pypde_options;
+
+typedef struct {
+  int struct_size;          /* sizeof(pypde_stats) */
+  long steps;
+  double t;
+  long pred_iters;          /* predictor iterations summed over cells and steps */
+  long pred_cells;
+  long pred_maxit;
+  long pred_capped;         /* cell solves that hit the iteration cap */
+  long guess_fallbacks;     /* learned guesses abandoned for the plain guess */
+  int nan_step;             /* first step with NaNs (0: none) */
+  double ms_loop;           /* wall time of the step loop */
+  double ms_kernel[8];      /* profile: summed time per kernel (weno, dt, predictor,
+                               flux0, flux1, update, bookkeeping, other) */
+} pypde_stats;
+
+void pypde_default_options(pypde_options *opts);
+
+/* One full solve.  Argument list of the reference's pde_solver (api.h:4-10)
+ * with the three callbacks replaced by `src`, followed by options/statistics.
+ * u (prod(nX) x V, row-major, variables fastest) is overwritten with the final
+ * state, as in the reference (api.cpp:18, iterator.cpp:129); ret receives ndt
+ * snapshot rows (iterator.cpp:136-150).  FLUX: 0 Rusanov (only).
+ * secondOrder (viscous F(Q, dQ)) is not supported. */
+int pde_solver_hip(const char *src, bool useF, bool useB, bool useS, double *u, double tf,
+                   int *nX, int ndim, double *dX, double CFL, int *boundaryTypes, bool STIFF,
+                   int FLUX, int N, int V, int ndt, bool secondOrder, double *ret,
+                   const pypde_options *opts, pypde_stats *stats);
+
+/* GPU replacement of the reference's weno_solver (api.h:12-13, api.cpp:32-48):
+ * same signature and semantics (no ghost cells; ret has shape
+ * (nX - 2(N-1)) x N^ndim x V).  Returns 0 on success. */
+int weno_solver(double *ret, double *u, int *nX, int ndim, int N, int V);
+
+/* Persistent solver context: compile once, keep the state resident in HBM. */
+typedef struct pypde_ctx pypde_ctx;
+pypde_ctx *pypde_create(const char *src, bool useF, bool useB, bool useS, int *nX, int ndim,
+                        double *dX, double CFL, int *boundaryTypes, bool STIFF, int FLUX, int N,
+                        int V, bool secondOrder, const pypde_options *opts, int *rc);
+int pypde_set_state(pypde_ctx *c, const double *u);          /* host, row-major */
+int pypde_get_state(pypde_ctx *c, double *u);                /* host, row-major */
+/* Advance from t = 0 until tf or max_steps; ndt snapshot rows into ret (may be
+ * NULL when ndt == 0).  Device state stays resident between calls. */
+int pypde_run(pypde_ctx *c, double tf, long max_steps, int ndt, double *ret, pypde_stats *stats);
+/* Advance `steps` more steps from the current (t, step count). */
+int pypde_step(pypde_ctx *c, long steps, double tf, pypde_stats *stats);
+/* Device pointers of the resident state (structure of arrays, [V][cell]) and
+ * of the last reconstruction/predictor outputs; stream used for all launches. */
+void *pypde_device_state(pypde_ctx *c);
+void *pypde_stream(pypde_ctx *c);
+void pypde_destroy(pypde_ctx *c);
+
+/* Compile (or fetch from the code-object cache) without touching a GPU. */
+int pypde_hip_compile(const char *src, bool useF, bool useB, bool useS, int ndim, int N, int V,
+                      bool STIFF, const pypde_options *opts);
+/* Tables exactly as the kernels see them (for checks). */
+int pypde_hip_tables(int N, double *nodes, double *wghts, double *endvals, double *dervals,
+                     double *weno_inv, double *sig, double *dgmat, double *dginv, double *fl_c,
+                     double *fl_m);
+const char *pypde_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

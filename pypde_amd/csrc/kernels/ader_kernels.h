@@ -1,0 +1,225 @@
+// ADER-WENO step kernels for MI355X (gfx950).  Compiled at run time by hipRTC
+// after ader_common.h, with the problem's D, N, V, tables and system source.
+//
+// Device data layout (structure of arrays, cell index fastest):
+//   u   [V][ncell]                          interior state
+//   wh  [N^D * V][nwc]                      reconstruction, plane = node*V + v
+//   tr  [2*D*N*N^(D-1)*V][nwc]              face traces of the predictor,
+//                                           plane = (((d*2+e)*N+t)*N^(D-1)+j)*V+v
+//   vol [V][nwc]                            volume (source / B) integral
+//   ff, fb [N*N^(D-1)*V][nf_d] per d        face flux / B path integral
+// nwc = prod(nx_d + 2) reconstructed cells (one ghost layer, iterator.cpp:76).
+#pragma once
+
+struct Geom {
+  int nx[2];     // interior cells per axis (nx[1] = 1 when D = 1)
+  int nw[2];     // reconstructed cells per axis (nx + 2; nw[1] = 1 when D = 1)
+  int nb[2];     // ghost-extended cells per axis (nx + 2N)
+  int bc[2];     // 0 transmissive, 1 periodic, 2 reflective
+  long ncell;    // prod nx
+  long nwc;      // prod nw
+  long nf[2];    // faces per direction: (nx_d + 1) * prod_{e != d} nx_e
+  double dX[2];
+  double refl[2][8];  // reflective-wall multipliers per axis (V <= 8)
+  int ndt;
+};
+
+struct StepState {
+  double t, dt, tf, cfl;
+  long long count, max_steps;
+  int push, done, nan_step, pad;
+  unsigned long long maxbits;   // max wave-speed sum as IEEE bits (values >= 0)
+  unsigned int nan_flag, pad2;
+  unsigned long long pred_iters, pred_cells, pred_capped, pred_fallbacks;
+  unsigned int pred_maxit, pad3;
+};
+
+DEV double wave_max(double x) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) x = dmax(x, __shfl_xor(x, o, 64));
+  return x;
+}
+DEV unsigned long long wave_sum_u64(unsigned long long x) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+DEV unsigned int wave_max_u32(unsigned int x) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    unsigned int y = __shfl_xor(x, o, 64);
+    x = x > y ? x : y;
+  }
+  return x;
+}
+
+// --------------------------------------------------------------- WENO
+// Boundary ghosts fused into the load, both sweeps staged in LDS
+// (boundaries.cpp:25-53, weno.cpp:61-121).  ghost = false gives the
+// reference's weno_solver semantics (no ghost extension).
+#ifndef WENO_TX
+#define WENO_TX 8
+#endif
+#ifndef WENO_TY
+#define WENO_TY 32
+#endif
+
+extern "C" __global__ void __launch_bounds__(256)
+k_weno(const double* __restrict__ u, double* __restrict__ wh, const StepState* __restrict__ st,
+       Geom g, int ghost, int nin0, int nin1, long out_stride) {
+  if (st && st->done) return;
+  constexpr int H = 2 * N - 2;
+  const int tid = threadIdx.x;
+  if constexpr (D == 1) {
+    constexpr int TX = 256;
+    constexpr int RX = TX + H;
+    __shared__ double s0[V][RX];
+    const int i0 = blockIdx.x * TX;
+    const int nb0 = ghost ? g.nb[0] : nin0;
+    const int no0 = nb0 - H;
+    for (int e = tid; e < V * RX; e += 256) {
+      int v = e / RX, r = e % RX;
+      int I = i0 + r;
+      double val = 0.0;
+      if (I < nb0) {
+        bool f0 = false;
+        int s = ghost ? ghost_src(I, g.bc[0], g.nx[0], &f0) : I;
+        double sg = 1.0;
+        if (f0) sg = sg * g.refl[0][v];
+        val = u[(long)v * (ghost ? g.ncell : nin0) + s] * sg;
+      }
+      s0[v][r] = val;
+    }
+    __syncthreads();
+    for (int e = tid; e < V * TX; e += 256) {
+      int v = e / TX, r = e % TX;
+      int i = i0 + r;
+      if (i < no0) {
+        double out[N];
+        weno_line([&](int m) { return s0[v][r + m]; }, out);
+#pragma unroll
+        for (int k = 0; k < N; k++) wh[(long)(k * V + v) * out_stride + i] = out[k];
+      }
+    }
+  } else {
+    constexpr int TX = WENO_TX, TY = WENO_TY;
+    constexpr int RX = TX + H, RY = TY + H, RYP = RY + 1;
+    __shared__ double s0[V][RX][RYP];
+    __shared__ double s1[V][N][TX][RYP];
+    const int i0 = blockIdx.y * TX, j0 = blockIdx.x * TY;
+    const int nb0 = ghost ? g.nb[0] : nin0, nb1 = ghost ? g.nb[1] : nin1;
+    const int no0 = nb0 - H, no1 = nb1 - H;
+    const long in_stride = ghost ? g.ncell : (long)nin0 * nin1;
+    const int in_row = ghost ? g.nx[1] : nin1;
+    for (int e = tid; e < V * RX * RY; e += 256) {
+      int v = e / (RX * RY), r = (e / RY) % RX, c = e % RY;
+      int I = i0 + r, J = j0 + c;
+      double val = 0.0;
+      if (I < nb0 && J < nb1) {
+        bool f0 = false, f1 = false;
+        int s0i = ghost ? ghost_src(I, g.bc[0], g.nx[0], &f0) : I;
+        int s1i = ghost ? ghost_src(J, g.bc[1], g.nx[1], &f1) : J;
+        double sg = 1.0;
+        if (f0) sg = sg * g.refl[0][v];
+        if (f1) sg = sg * g.refl[1][v];
+        val = u[(long)v * in_stride + (long)s0i * in_row + s1i] * sg;
+      }
+      s0[v][r][c] = val;
+    }
+    __syncthreads();
+    for (int e = tid; e < V * TX * RY; e += 256) {
+      int v = e / (TX * RY), r = (e / RY) % TX, c = e % RY;
+      if (i0 + r < no0) {
+        double out[N];
+        weno_line([&](int m) { return s0[v][r + m][c]; }, out);
+#pragma unroll
+        for (int k = 0; k < N; k++) s1[v][k][r][c] = out[k];
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < V * N * TX * TY; e += 256) {
+      int c = e % TY, r = (e / TY) % TX, k0 = (e / (TY * TX)) % N, v = e / (TY * TX * N);
+      int i = i0 + r, j = j0 + c;
+      if (i < no0 && j < no1) {
+        double out[N];
+        weno_line([&](int m) { return s1[v][k0][r][c + m]; }, out);
+        long cell = (long)i * no1 + j;
+#pragma unroll
+        for (int k1 = 0; k1 < N; k1++) wh[(long)((k0 * N + k1) * V + v) * out_stride + cell] = out[k1];
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ timestep
+// Largest sum_d lambda_d / dX_d over every reconstructed cell
+// (stepper.cpp:42-65), reduced as IEEE bit patterns (all values >= 0).
+template <int d>
+DEV void cell_deriv(const double (&w)[ND][V], int idx, double dx, double* out) {
+  constexpr int stride = ipow_c(N, D - 1 - d);
+  // idx is a compile-time constant at every call site after unrolling
+  const int kd = (idx / stride) % N;
+  const int base = idx - kd * stride;
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    double acc = T_DER[kd][0] * w[base][v];
+#pragma unroll
+    for (int j = 1; j < N; j++) acc = dfma(T_DER[kd][j], w[base + j * stride][v], acc);
+    out[v] = acc / dx;
+  }
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+k_dt(const double* __restrict__ wh, StepState* st, Geom g) {
+  if (st->done) return;
+  const long c = (long)blockIdx.x * 256 + threadIdx.x;
+  double sum = 0.0;
+  if (c < g.nwc) {
+    double w[ND][V];
+#pragma unroll
+    for (int idx = 0; idx < ND; idx++)
+#pragma unroll
+      for (int v = 0; v < V; v++) w[idx][v] = wh[(long)(idx * V + v) * g.nwc + c];
+    double q[V], dq[D * V];
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      double acc = T_WP[0] * w[0][v];
+#pragma unroll
+      for (int idx = 1; idx < ND; idx++) acc = dfma(T_WP[idx], w[idx][v], acc);
+      q[v] = acc;
+    }
+    static_for<0, D>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      double acc[V], tmp[V];
+#pragma unroll
+      for (int idx = 0; idx < ND; idx++) {
+        cell_deriv<d>(w, idx, g.dX[d], tmp);
+#pragma unroll
+        for (int v = 0; v < V; v++) acc[v] = (idx == 0) ? T_WP[0] * tmp[v] : dfma(T_WP[idx], tmp[v], acc[v]);
+      }
+#pragma unroll
+      for (int v = 0; v < V; v++) dq[d * V + v] = acc[v];
+    });
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      double f0[V];
+      double lam = max_abs_eigs(q, dq, d, f0);
+      sum = sum + lam / g.dX[d];
+    }
+    if (!(sum > 0.0)) sum = 0.0;
+  }
+  double m = wave_max(sum);
+  if ((threadIdx.x & 63) == 0 && m > 0.0)
+    atomicMax(&st->maxbits, (unsigned long long)__double_as_longlong(m));
+}
+
+// dt = CFL / MAX, x0.2 while count <= 5, clamped to tf - t (stepper.cpp:67-75).
+extern "C" __global__ void k_dt_finalize(StepState* st) {
+  if (threadIdx.x != 0 || st->done) return;
+  double MAX = __longlong_as_double((long long)st->maxbits);
+  double dt = st->cfl / MAX;
+  if (st->count <= 5) dt = dt * 0.2;
+  if (st->t + dt > st->tf) dt = st->tf - st->t;
+  st->dt = dt;
+  st->maxbits = 0ull;
+}

@@ -1,0 +1,757 @@
+// Predictor, face flux, corrector and step bookkeeping kernels (see
+// ader_kernels.h for layouts).
+#pragma once
+
+// ----------------------------------------------------------- predictor
+// One 64-lane wavefront holds CPW cells; lane = slot * ND + spatial node.
+// Each lane keeps its node's values at all N time nodes; spatial derivatives
+// exchange through LDS.  Cells converge independently (their lanes go idle).
+constexpr int CPW = 64 / ND;
+static_assert(CPW >= 1, "N^D must not exceed 64");
+
+#ifndef PDE_GUESS
+#define PDE_GUESS 0
+#endif
+
+DEV double tw(int i) {
+  double r = 0.0;
+#pragma unroll
+  for (int k = 0; k < N; k++)
+    if (k == i) r = T_W[k];
+  return r;
+}
+DEV double tder(int i, int j) {
+  double r = 0.0;
+#pragma unroll
+  for (int k = 0; k < N; k++)
+    if (k == i) r = T_DER[k][j];
+  return r;
+}
+
+struct NodeCtx {
+  int slot, idx;
+  int kd[D];
+  int base[D];
+  double der[D][N];   // DERVALS row k_d
+  double wk[D];       // W[k_d]
+  double WP, IWP;     // prod_d W[k_d], 1 / WP
+  double ct[N];       // W_t prod W (dg.cpp:105-116)
+  double c0[N];       // psi_t(0) prod W (dg.cpp:156-171)
+};
+
+DEV void node_ctx(NodeCtx& c, int lane) {
+  c.slot = lane / ND;
+  c.idx = lane % ND;
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    int stride = ipow_c(N, D - 1 - d);
+    c.kd[d] = (c.idx / stride) % N;
+    c.base[d] = c.idx - c.kd[d] * stride;
+    c.wk[d] = tw(c.kd[d]);
+#pragma unroll
+    for (int j = 0; j < N; j++) c.der[d][j] = tder(c.kd[d], j);
+  }
+  double p = c.wk[0];
+#pragma unroll
+  for (int d = 1; d < D; d++) p = p * c.wk[d];
+  c.WP = p;
+  c.IWP = 1.0 / p;
+#pragma unroll
+  for (int t = 0; t < N; t++) {
+    double a = T_W[t], b = T_END0[t];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      a = a * c.wk[d];
+      b = b * c.wk[d];
+    }
+    c.ct[t] = a;
+    c.c0[t] = b;
+  }
+}
+
+// derivative along d at this lane's node of buffer buf[slot][t][node][v]
+template <int d>
+DEV void lds_deriv(const NodeCtx& c, const double (*buf)[N][ND][V], int t, double dx, double* out) {
+  constexpr int stride = ipow_c(N, D - 1 - d);
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    double acc = c.der[d][0] * buf[c.slot][t][c.base[d]][v];
+#pragma unroll
+    for (int j = 1; j < N; j++) acc = dfma(c.der[d][j], buf[c.slot][t][c.base[d] + j * stride][v], acc);
+    out[v] = acc / dx;
+  }
+}
+
+DEV void callS(double* out, const double* q) {
+#if PDE_USE_S
+  S(out, q);
+#else
+#pragma unroll
+  for (int v = 0; v < V; v++) out[v] = 0.0;
+#endif
+}
+
+struct PredLDS {
+  double q[CPW][N][ND][V];
+#if PDE_USE_F
+  double f[D][CPW][N][ND][V];
+#endif
+  double red[64];
+};
+
+// rhs of the space-time system at this lane's node for all t (dg.cpp:50-126):
+// r = W_t prod W (S(q) - sum_d (B_d dq_d + d_d F_d)),  rhs = r dt + Ww.
+// Assumes sm.q holds the current iterate; leaves the lane's dq in dqs.
+DEV void node_rhs(const NodeCtx& c, PredLDS& sm, const double (&q)[N][V], const double* w,
+                  double dt, const Geom& g, bool active, double (&rhs)[N][V],
+                  double (&dqs)[N][D][V]) {
+  if (active) {
+#pragma unroll
+    for (int t = 0; t < N; t++) {
+      static_for<0, D>([&](auto dc) {
+        constexpr int d = decltype(dc)::value;
+        lds_deriv<d>(c, sm.q, t, g.dX[d], dqs[t][d]);
+      });
+#if PDE_USE_F
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        double f[V];
+        F(f, q[t], &dqs[t][0][0], d);
+#pragma unroll
+        for (int v = 0; v < V; v++) sm.f[d][c.slot][t][c.idx][v] = f[v];
+      }
+#endif
+    }
+  }
+#if PDE_USE_F
+  __syncthreads();
+#endif
+  if (active) {
+#pragma unroll
+    for (int t = 0; t < N; t++) {
+      double r[V];
+      callS(r, q[t]);
+      static_for<0, D>([&](auto dc) {
+        constexpr int d = decltype(dc)::value;
+#if PDE_USE_B
+        double b[V * V], bd[V];
+        B(b, q[t], d);
+        b_times<PDE_B_QUIRK ? 1 : 0>(b, dqs[t][d], bd);
+#pragma unroll
+        for (int v = 0; v < V; v++) r[v] = r[v] - bd[v];
+#endif
+#if PDE_USE_F
+        double df[V];
+        lds_deriv<d>(c, sm.f[d], t, g.dX[d], df);
+#pragma unroll
+        for (int v = 0; v < V; v++) r[v] = r[v] - df[v];
+#endif
+      });
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        double rr = r[v] * c.ct[t];
+        rhs[t][v] = rr * dt + c.c0[t] * w[v];
+      }
+    }
+  }
+}
+
+#if PDE_STIFF
+// Newton correction at one spatial node (see the CPU restatement for the
+// derivation): g_t = G_t / (W_t WP), A = dt dS/dQ at the time-weighted mean,
+// h_t = sum_j A^j sum_s M_{N-j}[t][s] g_s, P = p(A), dq = -P^-1 h.
+DEV void newton_node(const NodeCtx& c, const double (&q)[N][V], double (&G)[N][V], double dt) {
+  double g[N][V];
+#pragma unroll
+  for (int t = 0; t < N; t++) {
+    double sc = 1.0 / (T_W[t] * c.WP);
+#pragma unroll
+    for (int v = 0; v < V; v++) g[t][v] = G[t][v] * sc;
+  }
+  double qbar[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    double acc = T_W[0] * q[0][v];
+#pragma unroll
+    for (int t = 1; t < N; t++) acc = dfma(T_W[t], q[t][v], acc);
+    qbar[v] = acc;
+  }
+  double A[V * V];
+#if PDE_USE_M
+  M(A, qbar);
+#elif PDE_USE_S
+  {
+    double s0[V], s1[V], qq[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) qq[v] = qbar[v];
+    S(s0, qq);
+#pragma unroll
+    for (int j = 0; j < V; j++) {
+      double q0 = qq[j];
+      double hh = dmax(FD_EPS * dabs(q0), FD_EPS);
+      qq[j] = q0 + hh;
+      S(s1, qq);
+      qq[j] = q0;
+#pragma unroll
+      for (int i = 0; i < V; i++) A[i * V + j] = (s1[i] - s0[i]) / hh;
+    }
+  }
+#else
+#pragma unroll
+  for (int i = 0; i < V * V; i++) A[i] = 0.0;
+#endif
+#pragma unroll
+  for (int i = 0; i < V * V; i++) A[i] = dt * A[i];
+  // h_t (Horner in A); result left in G
+#pragma unroll
+  for (int t = 0; t < N; t++) {
+    double h[V];
+#pragma unroll
+    for (int j = N - 1; j >= 0; j--) {
+      double u[V];
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        double acc = T_FLM[N - j - 1][t][0] * g[0][v];
+#pragma unroll
+        for (int s = 1; s < N; s++) acc = dfma(T_FLM[N - j - 1][t][s], g[s][v], acc);
+        u[v] = acc;
+      }
+      if (j == N - 1) {
+#pragma unroll
+        for (int v = 0; v < V; v++) h[v] = u[v];
+      } else {
+        double Ah[V];
+#pragma unroll
+        for (int i = 0; i < V; i++) {
+          double acc = A[i * V] * h[0];
+#pragma unroll
+          for (int k = 1; k < V; k++) acc = dfma(A[i * V + k], h[k], acc);
+          Ah[i] = acc;
+        }
+#pragma unroll
+        for (int v = 0; v < V; v++) h[v] = Ah[v] + u[v];
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < V; v++) G[t][v] = h[v];
+  }
+  // P = p(A) = A^N + c_{N-1} A^{N-1} + ... + c_0 I
+  double P[V * V];
+#pragma unroll
+  for (int i = 0; i < V * V; i++) P[i] = A[i];
+#pragma unroll
+  for (int i = 0; i < V; i++) P[i * V + i] = P[i * V + i] + T_FLC[N - 1];
+#pragma unroll
+  for (int j = N - 2; j >= 0; j--) {
+    double PA[V * V];
+#pragma unroll
+    for (int i = 0; i < V; i++)
+#pragma unroll
+      for (int k = 0; k < V; k++) {
+        double acc = P[i * V] * A[k];
+#pragma unroll
+        for (int m = 1; m < V; m++) acc = dfma(P[i * V + m], A[m * V + k], acc);
+        PA[i * V + k] = acc;
+      }
+#pragma unroll
+    for (int i = 0; i < V * V; i++) P[i] = PA[i];
+#pragma unroll
+    for (int i = 0; i < V; i++) P[i * V + i] = P[i * V + i] + T_FLC[j];
+  }
+  // LU with partial pivoting (first largest pivot), static indices only
+  int piv[V];
+#pragma unroll
+  for (int k = 0; k < V; k++) {
+    int p = k;
+    double best = dabs(P[k * V + k]);
+#pragma unroll
+    for (int i = k + 1; i < V; i++) {
+      double a = dabs(P[i * V + k]);
+      if (a > best) {
+        best = a;
+        p = i;
+      }
+    }
+    piv[k] = p;
+#pragma unroll
+    for (int i = k + 1; i < V; i++) {
+      if (p == i) {
+#pragma unroll
+        for (int j = 0; j < V; j++) {
+          double tmp = P[k * V + j];
+          P[k * V + j] = P[i * V + j];
+          P[i * V + j] = tmp;
+        }
+      }
+    }
+#pragma unroll
+    for (int i = k + 1; i < V; i++) {
+      double l = P[i * V + k] / P[k * V + k];
+      P[i * V + k] = l;
+#pragma unroll
+      for (int j = k + 1; j < V; j++) P[i * V + j] = dfma(-l, P[k * V + j], P[i * V + j]);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < N; t++) {
+    double* b = G[t];
+#pragma unroll
+    for (int k = 0; k < V; k++) {
+#pragma unroll
+      for (int i = k + 1; i < V; i++) {
+        if (piv[k] == i) {
+          double tmp = b[k];
+          b[k] = b[i];
+          b[i] = tmp;
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 1; i < V; i++) {
+      double acc = b[i];
+#pragma unroll
+      for (int j = 0; j < i; j++) acc = dfma(-P[i * V + j], b[j], acc);
+      b[i] = acc;
+    }
+#pragma unroll
+    for (int i = V - 1; i >= 0; i--) {
+      double acc = b[i];
+#pragma unroll
+      for (int j = i + 1; j < V; j++) acc = dfma(-P[i * V + j], b[j], acc);
+      b[i] = -(acc / P[i * V + i]);
+    }
+  }
+}
+#endif
+
+extern "C" __global__ void __launch_bounds__(64)
+k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
+            double* __restrict__ tr, double* __restrict__ vol, double* __restrict__ qh,
+            StepState* st, Geom g, double newton_tol, int newton_maxit) {
+  if (st->done) return;
+  __shared__ PredLDS sm;
+  const double dt = st->dt;
+  const int lane = threadIdx.x;
+  NodeCtx c;
+  node_ctx(c, lane);
+  const long cell = (long)blockIdx.x * CPW + c.slot;
+  const bool valid = (c.slot < CPW) && (cell < g.nwc);
+  double w[V], q[N][V], rhs[N][V], dqs[N][D][V];
+  if (valid) {
+#pragma unroll
+    for (int v = 0; v < V; v++) w[v] = wh[(long)(c.idx * V + v) * g.nwc + cell];
+#pragma unroll
+    for (int t = 0; t < N; t++)
+#pragma unroll
+      for (int v = 0; v < V; v++)
+        q[t][v] = (PDE_GUESS && guess) ? guess[(long)((t * ND + c.idx) * V + v) * g.nwc + cell] : w[v];
+  }
+  const unsigned long long cellmask =
+      (ND == 64 ? ~0ull : ((1ull << ND) - 1ull)) << (c.slot < CPW ? c.slot * ND : 0);
+#if PDE_STIFF
+  const int maxit = newton_maxit;
+#else
+  const int maxit = DG_IT;
+#endif
+  bool active = valid;
+  int it = 0;
+  int learned = (PDE_GUESS && guess) ? 1 : 0;
+  int fellback = 0;
+  double r0 = 0.0;
+  while (__ballot(active) != 0ull) {
+    if (active) {
+#pragma unroll
+      for (int t = 0; t < N; t++)
+#pragma unroll
+        for (int v = 0; v < V; v++) sm.q[c.slot][t][c.idx][v] = q[t][v];
+    }
+    __syncthreads();
+    node_rhs(c, sm, q, w, dt, g, active, rhs, dqs);
+    bool more = false;
+    bool skip = false;
+#if PDE_STIFF
+    // residual G = rhs - (DG_MAT (x) W) q   (dg.cpp:128-154)
+    double rmax = 0.0;
+    if (active) {
+#pragma unroll
+      for (int t = 0; t < N; t++)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          double acc = (T_DGMAT[t][0] * c.WP) * q[0][v];
+#pragma unroll
+          for (int s = 1; s < N; s++) acc = dfma(T_DGMAT[t][s] * c.WP, q[s][v], acc);
+          rhs[t][v] = rhs[t][v] - acc;
+          rmax = dmax(rmax, dabs(rhs[t][v]));
+        }
+    }
+#if PDE_GUESS
+    if (learned) {  // wave-uniform: every cell of the launch started from a guess
+      __syncthreads();
+      sm.red[lane] = rmax;
+      __syncthreads();
+      double cm = 0.0;
+#pragma unroll
+      for (int i = 0; i < ND; i++) cm = dmax(cm, sm.red[(c.slot < CPW ? c.slot : 0) * ND + i]);
+      if (active) {
+        if (it == 0) {
+          r0 = cm;
+        } else {
+          if (!(cm <= r0)) {
+#pragma unroll
+            for (int t = 0; t < N; t++)
+#pragma unroll
+              for (int v = 0; v < V; v++) q[t][v] = w[v];
+            fellback = 1;
+            skip = true;
+          }
+        }
+      }
+      if (it >= 1) learned = 0;
+    }
+#endif
+    if (active && !skip) {
+      newton_node(c, q, rhs, dt);
+#pragma unroll
+      for (int t = 0; t < N; t++)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          if (!(dabs(rhs[t][v]) <= newton_tol * (1.0 + dabs(q[t][v])))) more = true;
+          q[t][v] = q[t][v] + rhs[t][v];
+        }
+    }
+    if (skip) more = true;
+#else
+    (void)learned;
+    (void)r0;
+    (void)newton_tol;
+    if (active) {
+      // Picard: q1 = DG_MAT^-1 rhs / prod W  (dg.cpp:187-202)
+#pragma unroll
+      for (int t = 0; t < N; t++)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          double acc = T_DGINV[t][0] * rhs[0][v];
+#pragma unroll
+          for (int s = 1; s < N; s++) acc = dfma(T_DGINV[t][s], rhs[s][v], acc);
+          double q1 = acc * c.IWP;
+          if (dabs(q1 - q[t][v]) > DG_TOL * (1.0 + dabs(q[t][v]))) more = true;
+          q[t][v] = q1;
+        }
+    }
+#endif
+    const unsigned long long mb = __ballot(more);
+    if (active) {
+      it++;
+      if ((mb & cellmask) == 0ull || it >= maxit) active = false;
+    }
+    __syncthreads();
+  }
+  // ---- epilogue: traces, volume term, optional full qh
+  if (valid) {
+#pragma unroll
+    for (int t = 0; t < N; t++)
+#pragma unroll
+      for (int v = 0; v < V; v++) sm.q[c.slot][t][c.idx][v] = q[t][v];
+  }
+  __syncthreads();
+  if (valid) {
+    static_for<0, D>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      constexpr int stride = ipow_c(N, D - 1 - d);
+      if (c.kd[d] == 0) {
+        const int j = (c.idx / (stride * N)) * stride + c.idx % stride;
+#pragma unroll
+        for (int e = 0; e < 2; e++)
+#pragma unroll
+          for (int t = 0; t < N; t++)
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+              double acc = (e ? T_END1[0] : T_END0[0]) * sm.q[c.slot][t][c.idx][v];
+#pragma unroll
+              for (int m = 1; m < N; m++)
+                acc = dfma(e ? T_END1[m] : T_END0[m], sm.q[c.slot][t][c.idx + m * stride][v], acc);
+              const int plane = (((d * 2 + e) * N + t) * ND1 + j) * V + v;
+              tr[(long)plane * g.nwc + cell] = acc;
+            }
+      }
+    });
+    if (qh) {
+#pragma unroll
+      for (int t = 0; t < N; t++)
+#pragma unroll
+        for (int v = 0; v < V; v++) qh[(long)((t * ND + c.idx) * V + v) * g.nwc + cell] = q[t][v];
+    }
+  }
+#if PDE_USE_S || PDE_USE_B
+  // volume term (fv.cpp:33-85): per node sum over t, then over nodes in order
+  double part[V];
+  if (valid) {
+#pragma unroll
+    for (int t = 0; t < N; t++) {
+      double s[V];
+      callS(s, q[t]);
+#if PDE_USE_B
+      static_for<0, D>([&](auto dc) {
+        constexpr int d = decltype(dc)::value;
+        double dqd[V], b[V * V], bd[V];
+        lds_deriv<d>(c, sm.q, t, g.dX[d], dqd);
+        B(b, q[t], d);
+        b_times<PDE_B_QUIRK ? 2 : 0>(b, dqd, bd);
+#pragma unroll
+        for (int v = 0; v < V; v++) s[v] = s[v] - bd[v];
+      });
+#endif
+      double tmp = dt * T_W[t];
+#pragma unroll
+      for (int d = 0; d < D; d++) tmp = tmp * c.wk[d];
+#pragma unroll
+      for (int v = 0; v < V; v++) part[v] = (t == 0) ? tmp * s[v] : dfma(tmp, s[v], part[v]);
+    }
+  }
+  __syncthreads();
+  if (valid) {
+#pragma unroll
+    for (int v = 0; v < V; v++) sm.q[c.slot][0][c.idx][v] = part[v];
+  }
+  __syncthreads();
+  if (valid && c.idx == 0) {
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      double acc = sm.q[c.slot][0][0][v];
+#pragma unroll
+      for (int i = 1; i < ND; i++) acc = acc + sm.q[c.slot][0][i][v];
+      vol[(long)v * g.nwc + cell] = acc;
+    }
+  }
+#endif
+  // ---- statistics (one contribution per cell)
+  unsigned long long mine = (valid && c.idx == 0) ? (unsigned long long)it : 0ull;
+  unsigned long long cnt = (valid && c.idx == 0) ? 1ull : 0ull;
+  unsigned long long cap = (valid && c.idx == 0 && it >= maxit) ? 1ull : 0ull;
+  unsigned long long fb = (valid && c.idx == 0) ? (unsigned long long)fellback : 0ull;
+  unsigned int mx = (valid && c.idx == 0) ? (unsigned int)it : 0u;
+  mine = wave_sum_u64(mine);
+  cnt = wave_sum_u64(cnt);
+  cap = wave_sum_u64(cap);
+  fb = wave_sum_u64(fb);
+  mx = wave_max_u32(mx);
+  if (lane == 0) {
+    atomicAdd(&st->pred_iters, mine);
+    atomicAdd(&st->pred_cells, cnt);
+    if (cap) atomicAdd(&st->pred_capped, cap);
+    if (fb) atomicAdd(&st->pred_fallbacks, fb);
+    atomicMax(&st->pred_maxit, mx);
+  }
+}
+
+// ------------------------------------------------------------ face flux
+// Rusanov flux and B path integral at every face point (fluxes.cpp:72-116).
+// Faces along d: (nx_d + 1) x prod_{e != d} nx_e, all touching an interior cell.
+template <int d>
+DEV void face_cells(const Geom& g, long f, long* wl, long* wr) {
+  if constexpr (D == 1) {
+    *wl = f;
+    *wr = f + 1;
+  } else if constexpr (d == 0) {
+    long f0 = f / g.nx[1], f1 = f % g.nx[1];
+    *wl = f0 * g.nw[1] + (f1 + 1);
+    *wr = *wl + g.nw[1];
+  } else {
+    long f0 = f / (g.nx[1] + 1), f1 = f % (g.nx[1] + 1);
+    *wl = (f0 + 1) * g.nw[1] + f1;
+    *wr = *wl + 1;
+  }
+}
+
+template <int d>
+DEV void flux_dir(const double* __restrict__ tr, double* __restrict__ ff, double* __restrict__ fb,
+                  const StepState* st, const Geom& g) {
+  const long nf = g.nf[d];
+  const long item = (long)blockIdx.x * 256 + threadIdx.x;
+  if (item >= nf * N * ND1) return;
+  const long f = item % nf;
+  const int j = (int)((item / nf) % ND1);
+  const int t = (int)(item / (nf * ND1));
+  long wl, wr;
+  face_cells<d>(g, f, &wl, &wr);
+  double qL[V], qR[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    qL[v] = tr[(long)((((d * 2 + 1) * N + t) * ND1 + j) * V + v) * g.nwc + wl];
+    qR[v] = tr[(long)((((d * 2 + 0) * N + t) * ND1 + j) * V + v) * g.nwc + wr];
+  }
+  const long o = (long)((t * ND1 + j) * V) * nf + f;
+#if PDE_USE_F
+  {
+    double dq0[D * V];
+#pragma unroll
+    for (int i = 0; i < D * V; i++) dq0[i] = 0.0;
+    double fl[V], fr[V];
+    double l1 = max_abs_eigs(qL, dq0, d, fl);
+    double l2 = max_abs_eigs(qR, dq0, d, fr);
+    double s = dmax(l1, l2);
+#pragma unroll
+    for (int v = 0; v < V; v++) ff[o + (long)v * nf] = s * (qL[v] - qR[v]) + (fl[v] + fr[v]);
+  }
+#endif
+#if PDE_USE_B
+  {
+    double Dq[V], Mx[V * V], b[V * V], q[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) Dq[v] = qR[v] - qL[v];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+#pragma unroll
+      for (int v = 0; v < V; v++) q[v] = dfma(T_X[i], Dq[v], qL[v]);
+      B(b, q, d);
+#pragma unroll
+      for (int k = 0; k < V * V; k++) Mx[k] = (i == 0) ? T_W[0] * b[k] : dfma(T_W[i], b[k], Mx[k]);
+    }
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      double acc = Mx[v * V] * Dq[0];
+#pragma unroll
+      for (int k = 1; k < V; k++) acc = dfma(Mx[v * V + k], Dq[k], acc);
+      fb[o + (long)v * nf] = acc;
+    }
+  }
+#endif
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+k_flux0(const double* __restrict__ tr, double* __restrict__ ff, double* __restrict__ fb,
+        const StepState* st, Geom g) {
+  if (st->done) return;
+  flux_dir<0>(tr, ff, fb, st, g);
+}
+#if PDE_D >= 2
+extern "C" __global__ void __launch_bounds__(256)
+k_flux1(const double* __restrict__ tr, double* __restrict__ ff, double* __restrict__ fb,
+        const StepState* st, Geom g) {
+  if (st->done) return;
+  flux_dir<1>(tr, ff, fb, st, g);
+}
+#endif
+
+// ------------------------------------------------------------- corrector
+// u += volume term, then the face contributions in the reference's
+// face-cell order (fv.cpp:148-197): faces c+1-e_d (c on the right, d
+// ascending, each over t then transverse node), then face-cell c+1 (c on the
+// left, over t, then d, then transverse node).  Also writes the snapshot row
+// and raises the NaN flag.
+struct FaceBufs {
+  const double* ff[2];
+  const double* fb[2];
+};
+
+extern "C" __global__ void __launch_bounds__(256)
+k_update(double* __restrict__ u, const double* __restrict__ vol, FaceBufs fbufs,
+         double* __restrict__ ret, StepState* st, Geom g) {
+  if (st->done) return;
+  const long c = (long)blockIdx.x * 256 + threadIdx.x;
+  if (c >= g.ncell) return;
+  const double dt = st->dt;
+  int cc[2];
+  cc[0] = (D == 1) ? (int)c : (int)(c / g.nx[1]);
+  cc[1] = (D == 1) ? 0 : (int)(c % g.nx[1]);
+  double uc[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) uc[v] = u[(long)v * g.ncell + c];
+#if PDE_USE_S || PDE_USE_B
+  {
+    long wc = (D == 1) ? (long)(cc[0] + 1) : (long)(cc[0] + 1) * g.nw[1] + (cc[1] + 1);
+#pragma unroll
+    for (int v = 0; v < V; v++) uc[v] = uc[v] + vol[(long)v * g.nwc + wc];
+  }
+#endif
+#if PDE_USE_F || PDE_USE_B
+  long fidx[2][2];  // [side][d]
+#pragma unroll
+  for (int d = 0; d < D; d++)
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+      if (D == 1) {
+        fidx[side][d] = cc[0] + side;
+      } else if (d == 0) {
+        fidx[side][d] = (long)(cc[0] + side) * g.nx[1] + cc[1];
+      } else {
+        fidx[side][d] = (long)cc[0] * (g.nx[1] + 1) + cc[1] + side;
+      }
+    }
+  auto contrib = [&](int d, int side, int t) {
+    const double c0 = (dt * T_W[t]) / (2.0 * g.dX[d]);
+    const long nf = g.nf[d];
+#pragma unroll
+    for (int j = 0; j < ND1; j++) {
+      double tmp = c0;
+      if (D == 2) tmp = tmp * T_W[j];
+      const long o = (long)((t * ND1 + j) * V) * nf + fidx[side][d];
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+#if PDE_USE_F
+        double f = fbufs.ff[d][o + (long)v * nf];
+#else
+        double f = 0.0;
+#endif
+#if PDE_USE_B
+        double b = fbufs.fb[d][o + (long)v * nf];
+#else
+        double b = 0.0;
+#endif
+        uc[v] = side == 0 ? uc[v] - tmp * (b - f) : uc[v] - tmp * (b + f);
+      }
+    }
+  };
+#pragma unroll
+  for (int d = 0; d < D; d++)
+#pragma unroll
+    for (int t = 0; t < N; t++) contrib(d, 0, t);
+#pragma unroll
+  for (int t = 0; t < N; t++)
+#pragma unroll
+    for (int d = 0; d < D; d++) contrib(d, 1, t);
+#endif
+  bool bad = false;
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    u[(long)v * g.ncell + c] = uc[v];
+    if (uc[v] != uc[v]) bad = true;
+  }
+  if (bad) atomicOr(&st->nan_flag, 1u);
+  const double tn = st->t + dt;
+  const int push = st->push;
+  if (ret && push < g.ndt && tn >= (double)(push + 1) / (double)g.ndt * st->tf) {
+#pragma unroll
+    for (int v = 0; v < V; v++) ret[((long)push * g.ncell + c) * V + v] = uc[v];
+  }
+}
+
+// t += dt; count++; snapshot row; NaN and end-of-run checks (iterator.cpp:129-147).
+extern "C" __global__ void k_advance(StepState* st, int ndt) {
+  if (threadIdx.x != 0 || st->done) return;
+  st->t = st->t + st->dt;
+  st->count += 1;
+  if (st->push < ndt && st->t >= (double)(st->push + 1) / (double)ndt * st->tf) st->push += 1;
+  if (st->nan_flag) {
+    st->done = 1;
+    st->nan_step = (int)st->count;
+  }
+  if (!(st->dt > 0.0)) st->done = 1;  // no progress possible (infinite wave speed)
+  if (!(st->t < st->tf) || (st->max_steps > 0 && st->count >= st->max_steps)) st->done = 1;
+}
+
+// Interleaved [cell][v] <-> structure-of-arrays [v][cell].
+extern "C" __global__ void k_aos_to_soa(const double* __restrict__ a, double* __restrict__ s, long ncell) {
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= ncell * V) return;
+  long c = i / V;
+  int v = (int)(i % V);
+  s[(long)v * ncell + c] = a[i];
+}
+extern "C" __global__ void k_soa_to_aos(const double* __restrict__ s, double* __restrict__ a, long ncell) {
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= ncell * V) return;
+  long c = i / V;
+  int v = (int)(i % V);
+  a[i] = s[(long)v * ncell + c];
+}

@@ -1,0 +1,756 @@
+// pypde_hip host library: hipRTC specialisation of the ADER-WENO kernels,
+// resident device state, step loop and the C ABI of include/pypde_hip.h.
+//
+// Reference counterparts: src/api.cpp:5-48 (entry points),
+// src/solvers/iterator.cpp:38-151 (time loop), pypde/cfuncs.py:30-75 (the JIT
+// step that turned Python physics into callable code; here hipRTC).
+#include "pypde_hip.h"
+
+#include <dlfcn.h>
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "tables.h"
+
+namespace {
+
+#include "kernels_embed.inc"  // PRELUDE_SRC, COMMON_SRC, KERNELS_SRC, STEP_SRC
+
+constexpr int MAXV = 8;
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[2048];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess)                                                          \
+      return fail(PYPDE_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_));       \
+  } while (0)
+
+// ------------------------------------------------------------ device structs
+// Must match Geom / StepState / FaceBufs in kernels/ader_kernels.h (refl sized
+// for the largest V the build accepts).
+struct Geom {
+  int nx[2];
+  int nw[2];
+  int nb[2];
+  int bc[2];
+  long ncell;
+  long nwc;
+  long nf[2];
+  double dX[2];
+  double refl[2][MAXV];
+  int ndt;
+};
+
+struct StepState {
+  double t, dt, tf, cfl;
+  long long count, max_steps;
+  int push, done, nan_step, pad;
+  unsigned long long maxbits;
+  unsigned int nan_flag, pad2;
+  unsigned long long pred_iters, pred_cells, pred_capped, pred_fallbacks;
+  unsigned int pred_maxit, pad3;
+};
+
+struct FaceBufs {
+  const double* ff[2];
+  const double* fb[2];
+};
+
+// ------------------------------------------------------------ compilation
+uint64_t fnv1a(const std::string& s, uint64_t h = 1469598103934665603ull) {
+  for (unsigned char c : s) {
+    h ^= c;
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+std::string cache_dir() {
+  const char* e = getenv("PYPDE_HIP_CACHE");
+  if (e && *e) return e;
+  Dl_info info;
+  if (dladdr((void*)&cache_dir, &info) && info.dli_fname) {
+    std::string p = info.dli_fname;
+    size_t k = p.rfind('/');
+    if (k != std::string::npos) return p.substr(0, k) + "/_jitcache";
+  }
+  return "/tmp/pypde_hip_cache";
+}
+
+struct Spec {
+  std::string src;
+  bool useF, useB, useS, useM, stiff, guess;
+  int D, N, V, b_quirk;
+};
+
+std::string full_source(const Spec& s, const pypde::Tables& T) {
+  std::ostringstream o;
+  o << "// generated: ndim=" << s.D << " N=" << s.N << " V=" << s.V << "\n";
+  o << "#define PDE_D " << s.D << "\n#define PDE_N " << s.N << "\n#define PDE_V " << s.V << "\n";
+  o << "#define PDE_USE_F " << (int)s.useF << "\n#define PDE_USE_B " << (int)s.useB << "\n";
+  o << "#define PDE_USE_S " << (int)s.useS << "\n#define PDE_USE_M " << (int)(s.useS && s.useM)
+    << "\n";
+  o << "#define PDE_STIFF " << (int)s.stiff << "\n#define PDE_B_QUIRK " << s.b_quirk << "\n";
+  o << "#define PDE_GUESS " << (int)s.guess << "\n";
+  o << "constexpr int ipow_c(int b, int e) { return e <= 0 ? 1 : b * ipow_c(b, e - 1); }\n";
+  o << pypde::tables_source(T, s.D);
+  o << PRELUDE_SRC << "\n";
+  o << "#line 1 \"system\"\n" << s.src << "\n";
+  o << COMMON_SRC << "\n" << KERNELS_SRC << "\n" << STEP_SRC << "\n";
+  return o.str();
+}
+
+const std::vector<std::string>& compile_options() {
+  static const std::vector<std::string> o = {
+      "--offload-arch=gfx950", "-O3", "-std=c++20", "-ffp-contract=off",
+      "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  return o;
+}
+
+std::mutex g_mu;
+std::map<uint64_t, std::vector<char>> g_codes;
+
+int compile_code(const Spec& spec, std::vector<char>* code) {
+  pypde::Tables T;
+  if (pypde::make_tables(spec.N, &T)) return fail(PYPDE_ERR_ARG, "bad order N=%d", spec.N);
+  std::string src = full_source(spec, T);
+  if (const char* dump = getenv("PYPDE_HIP_DUMP_SRC")) {
+    std::ofstream f(dump);
+    f << src;
+  }
+  std::string key = src;
+  for (auto& x : compile_options()) key += "\n" + x;
+  int ver = 0;
+  hiprtcVersion(&ver, &ver);
+  key += "\nhiprtc" + std::to_string(ver);
+  uint64_t h = fnv1a(key);
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_codes.find(h);
+    if (it != g_codes.end()) {
+      *code = it->second;
+      return 0;
+    }
+  }
+  char name[64];
+  snprintf(name, sizeof name, "%016llx", (unsigned long long)h);
+  std::string dir = cache_dir();
+  std::string path = dir + "/" + name + ".hsaco";
+  {
+    std::ifstream f(path, std::ios::binary);
+    if (f) {
+      std::vector<char> c((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+      if (!c.empty()) {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_codes[h] = c;
+        *code = c;
+        return 0;
+      }
+    }
+  }
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "pypde_ader.hip", 0, nullptr, nullptr) !=
+      HIPRTC_SUCCESS)
+    return fail(PYPDE_ERR_COMPILE, "hiprtcCreateProgram failed");
+  std::vector<const char*> opts;
+  for (auto& x : compile_options()) opts.push_back(x.c_str());
+  hiprtcResult r = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
+  size_t ls = 0;
+  hiprtcGetProgramLogSize(prog, &ls);
+  std::string log(ls, '\0');
+  if (ls) hiprtcGetProgramLog(prog, &log[0]);
+  if (r != HIPRTC_SUCCESS) {
+    hiprtcDestroyProgram(&prog);
+    return fail(PYPDE_ERR_COMPILE, "hipRTC compile failed:\n%s", log.c_str());
+  }
+  size_t cs = 0;
+  hiprtcGetCodeSize(prog, &cs);
+  std::vector<char> c(cs);
+  hiprtcGetCode(prog, c.data());
+  hiprtcDestroyProgram(&prog);
+  mkdir(dir.c_str(), 0755);
+  {
+    std::string tmp = path + ".tmp" + std::to_string(getpid());
+    std::ofstream f(tmp, std::ios::binary);
+    if (f) {
+      f.write(c.data(), (std::streamsize)c.size());
+      f.close();
+      rename(tmp.c_str(), path.c_str());
+    }
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_codes[h] = c;
+  *code = c;
+  return 0;
+}
+
+int check_spec(const Spec& s, bool secondOrder, int FLUX) {
+  if (s.D < 1 || s.D > 2)
+    return fail(PYPDE_ERR_UNSUPPORTED,
+                "ndim=%d: only 1 and 2 are supported (the reference's node indexing, "
+                "poly/evaluations.cpp:4-21, is only correct for ndim <= 2)", s.D);
+  if (s.N < 1 || s.N > 6) return fail(PYPDE_ERR_ARG, "order N=%d outside 1..6", s.N);
+  int nd = 1;
+  for (int d = 0; d < s.D; d++) nd *= s.N;
+  if (nd > 64) return fail(PYPDE_ERR_UNSUPPORTED, "N^ndim=%d > 64 nodes per cell", nd);
+  if (s.V < 1 || s.V > MAXV) return fail(PYPDE_ERR_ARG, "V=%d outside 1..%d", s.V, MAXV);
+  if (secondOrder) return fail(PYPDE_ERR_UNSUPPORTED, "second-order (viscous) F not supported");
+  if (FLUX != 0) return fail(PYPDE_ERR_UNSUPPORTED, "FLUX=%d: only Rusanov (0) is supported", FLUX);
+  if (!s.src.size() && (s.useF || s.useB || s.useS))
+    return fail(PYPDE_ERR_ARG, "empty system source");
+  return 0;
+}
+
+enum { K_WENO, K_DT, K_DTFIN, K_PRED, K_FLUX0, K_FLUX1, K_UPDATE, K_ADVANCE, K_A2S, K_S2A, K_COUNT };
+const char* kernel_names[K_COUNT] = {"k_weno",   "k_dt",   "k_dt_finalize", "k_predictor",
+                                     "k_flux0",  "k_flux1", "k_update",     "k_advance",
+                                     "k_aos_to_soa", "k_soa_to_aos"};
+
+}  // namespace
+
+// ------------------------------------------------------------------ context
+struct pypde_ctx {
+  Spec spec;
+  pypde_options opts;
+  Geom g;
+  int D, N, V, ND, NT, CPW;
+  hipModule_t mod = nullptr;
+  hipFunction_t fn[K_COUNT] = {};
+  hipStream_t stream = nullptr;
+  double *u = nullptr, *wh = nullptr, *tr = nullptr, *vol = nullptr, *ff[2] = {}, *fb[2] = {},
+         *ret = nullptr, *guess = nullptr, *tmp = nullptr;
+  StepState* st = nullptr;
+  StepState host_st;
+  size_t ret_rows = 0;
+  int device = 0;
+  std::vector<hipEvent_t> ev;
+  double ms_kernel[8] = {0};
+};
+
+namespace {
+
+int launch(pypde_ctx* c, int k, dim3 grid, dim3 block, std::vector<void*>& args) {
+  hipError_t e = hipModuleLaunchKernel(c->fn[k], grid.x, grid.y, grid.z, block.x, block.y,
+                                       block.z, 0, c->stream, args.data(), nullptr);
+  if (e != hipSuccess)
+    return fail(PYPDE_ERR_HIP, "launch %s: %s", kernel_names[k], hipGetErrorString(e));
+  return 0;
+}
+
+long cdiv(long a, long b) { return (a + b - 1) / b; }
+
+// One timestep: boundaries + WENO, CFL reduction, predictor, face fluxes,
+// corrector, bookkeeping (iterator.cpp:99-148).  Every kernel returns at once
+// when the device flag `done` is set, so whole chunks can be queued blind.
+int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
+  Geom g = c->g;
+  int rc;
+  auto mark = [&]() {
+    if (evs) {
+      hipEvent_t e;
+      hipEventCreate(&e);
+      hipEventRecord(e, c->stream);
+      evs->push_back(e);
+    }
+  };
+  mark();
+  {
+    const double* uu = c->u;
+    double* w = c->wh;
+    const StepState* st = c->st;
+    int ghost = 1, z = 0;
+    long os = g.nwc;
+    std::vector<void*> a = {&uu, &w, &st, &g, &ghost, &z, &z, &os};
+    dim3 grid = c->D == 1 ? dim3((unsigned)cdiv(g.nw[0], 256))
+                          : dim3((unsigned)cdiv(g.nw[1], 32), (unsigned)cdiv(g.nw[0], 8));
+    if ((rc = launch(c, K_WENO, grid, dim3(256), a))) return rc;
+  }
+  mark();
+  {
+    const double* w = c->wh;
+    StepState* st = c->st;
+    std::vector<void*> a = {&w, &st, &g};
+    if ((rc = launch(c, K_DT, dim3((unsigned)cdiv(g.nwc, 256)), dim3(256), a))) return rc;
+    std::vector<void*> b = {&st};
+    if ((rc = launch(c, K_DTFIN, dim3(1), dim3(64), b))) return rc;
+  }
+  mark();
+  {
+    const double* w = c->wh;
+    const double* gs = guess_now ? c->guess : nullptr;
+    double *t = c->tr, *v = c->vol, *qh = nullptr;
+    StepState* st = c->st;
+    double tol = c->opts.newton_tol;
+    int maxit = c->opts.newton_maxit;
+    std::vector<void*> a = {&w, &gs, &t, &v, &qh, &st, &g, &tol, &maxit};
+    if ((rc = launch(c, K_PRED, dim3((unsigned)cdiv(g.nwc, c->CPW)), dim3(64), a))) return rc;
+  }
+  mark();
+  for (int d = 0; d < c->D; d++) {
+    const double* t = c->tr;
+    double *f = c->ff[d], *b = c->fb[d];
+    const StepState* st = c->st;
+    std::vector<void*> a = {&t, &f, &b, &st, &g};
+    long items = g.nf[d] * c->N * (c->ND / c->N);
+    if ((rc = launch(c, d == 0 ? K_FLUX0 : K_FLUX1, dim3((unsigned)cdiv(items, 256)), dim3(256), a)))
+      return rc;
+    mark();
+  }
+  if (c->D == 1) mark();
+  {
+    double* uu = c->u;
+    const double* v = c->vol;
+    FaceBufs fbufs = {{c->ff[0], c->ff[1]}, {c->fb[0], c->fb[1]}};
+    double* r = c->ret;
+    StepState* st = c->st;
+    std::vector<void*> a = {&uu, &v, &fbufs, &r, &st, &g};
+    if ((rc = launch(c, K_UPDATE, dim3((unsigned)cdiv(g.ncell, 256)), dim3(256), a))) return rc;
+  }
+  mark();
+  {
+    StepState* st = c->st;
+    int ndt = g.ndt;
+    std::vector<void*> a = {&st, &ndt};
+    if ((rc = launch(c, K_ADVANCE, dim3(1), dim3(64), a))) return rc;
+  }
+  mark();
+  return 0;
+}
+
+int make_spec(Spec* s, const char* src, bool useF, bool useB, bool useS, int ndim, int N, int V,
+              bool STIFF, const pypde_options* o) {
+  s->src = src ? src : "";
+  s->useF = useF;
+  s->useB = useB;
+  s->useS = useS;
+  s->useM = o && o->use_M;
+  s->stiff = STIFF;
+  s->guess = false;
+  s->D = ndim;
+  s->N = N;
+  s->V = V;
+  s->b_quirk = o ? o->b_quirk : 0;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void pypde_default_options(pypde_options* o) {
+  memset(o, 0, sizeof *o);
+  o->struct_size = sizeof *o;
+  o->newton_tol = 1e-12;
+  o->newton_maxit = 100;
+}
+
+const char* pypde_hip_last_error(void) { return g_err.c_str(); }
+
+int pypde_hip_tables(int N, double* nodes, double* wghts, double* endvals, double* dervals,
+                     double* weno_inv, double* sig, double* dgmat, double* dginv, double* fl_c,
+                     double* fl_m) {
+  pypde::Tables T;
+  if (pypde::make_tables(N, &T)) return fail(PYPDE_ERR_ARG, "bad order N=%d", N);
+  for (int i = 0; i < N; i++) {
+    if (nodes) nodes[i] = T.x[i];
+    if (wghts) wghts[i] = T.w[i];
+    if (endvals) {
+      endvals[i] = T.end0[i];
+      endvals[N + i] = T.end1[i];
+    }
+    if (fl_c) fl_c[i] = T.flc[i];
+    for (int j = 0; j < N; j++) {
+      if (dervals) dervals[i * N + j] = T.der[i][j];
+      if (sig) sig[i * N + j] = T.sig[i][j];
+      if (dgmat) dgmat[i * N + j] = T.dgmat[i][j];
+      if (dginv) dginv[i * N + j] = T.dginv[i][j];
+      for (int s = 0; s < 4; s++)
+        if (weno_inv) weno_inv[(s * N + i) * N + j] = T.winv[s][i][j];
+      for (int k = 0; k < N; k++)
+        if (fl_m) fl_m[(k * N + i) * N + j] = T.flm[k][i][j];
+    }
+  }
+  return 0;
+}
+
+int pypde_hip_compile(const char* src, bool useF, bool useB, bool useS, int ndim, int N, int V,
+                      bool STIFF, const pypde_options* opts) {
+  Spec s;
+  make_spec(&s, src, useF, useB, useS, ndim, N, V, STIFF, opts);
+  int rc = check_spec(s, false, 0);
+  if (rc) return rc;
+  std::vector<char> code;
+  if ((rc = compile_code(s, &code))) return rc;
+  if (STIFF) {  // the learned-guess variant of the stiff predictor
+    s.guess = true;
+    if ((rc = compile_code(s, &code))) return rc;
+  }
+  return 0;
+}
+
+void pypde_destroy(pypde_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (double* p : {c->u, c->wh, c->tr, c->vol, c->ff[0], c->ff[1], c->fb[0], c->fb[1], c->ret,
+                    c->guess, c->tmp})
+    if (p) hipFree(p);
+  if (c->st) hipFree(c->st);
+  if (c->mod) hipModuleUnload(c->mod);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+pypde_ctx* pypde_create(const char* src, bool useF, bool useB, bool useS, int* nX, int ndim,
+                        double* dX, double CFL, int* boundaryTypes, bool STIFF, int FLUX, int N,
+                        int V, bool secondOrder, const pypde_options* opts_in, int* rc_out) {
+  int dummy;
+  int& rc = rc_out ? *rc_out : dummy;
+  pypde_options opts;
+  pypde_default_options(&opts);
+  if (opts_in) {
+    size_t n = opts_in->struct_size > 0 && (size_t)opts_in->struct_size < sizeof opts
+                   ? (size_t)opts_in->struct_size
+                   : sizeof opts;
+    memcpy(&opts, opts_in, n);
+    opts.struct_size = sizeof opts;
+    if (!(opts.newton_tol > 0)) opts.newton_tol = 1e-12;
+    if (opts.newton_maxit <= 0) opts.newton_maxit = 100;
+  }
+  Spec s;
+  make_spec(&s, src, useF, useB, useS, ndim, N, V, STIFF, &opts);
+  s.guess = opts.guess != nullptr;
+  if ((rc = check_spec(s, secondOrder, FLUX))) return nullptr;
+  if (!nX || !dX || !boundaryTypes) {
+    rc = fail(PYPDE_ERR_ARG, "null grid argument");
+    return nullptr;
+  }
+  for (int d = 0; d < ndim; d++) {
+    if (nX[d] < 1) {
+      rc = fail(PYPDE_ERR_ARG, "nX[%d]=%d", d, nX[d]);
+      return nullptr;
+    }
+    if (boundaryTypes[d] < 0 || boundaryTypes[d] > 2) {
+      rc = fail(PYPDE_ERR_ARG, "boundaryTypes[%d]=%d (0 transmissive, 1 periodic, 2 reflective)",
+                d, boundaryTypes[d]);
+      return nullptr;
+    }
+    if (!(dX[d] > 0)) {
+      rc = fail(PYPDE_ERR_ARG, "dX[%d] must be positive", d);
+      return nullptr;
+    }
+    if (boundaryTypes[d] != 0 && nX[d] < N) {
+      rc = fail(PYPDE_ERR_ARG, "periodic/reflective axis %d needs nX >= N (%d < %d)", d, nX[d], N);
+      return nullptr;
+    }
+  }
+  std::vector<char> code;
+  if ((rc = compile_code(s, &code))) return nullptr;
+  pypde_ctx* c = new pypde_ctx();
+  c->spec = s;
+  c->opts = opts;
+  c->D = ndim;
+  c->N = N;
+  c->V = V;
+  c->ND = 1;
+  for (int d = 0; d < ndim; d++) c->ND *= N;
+  c->NT = c->ND * N;
+  c->CPW = 64 / c->ND;
+  c->device = opts.device;
+  Geom& g = c->g;
+  memset(&g, 0, sizeof g);
+  g.ncell = 1;
+  g.nwc = 1;
+  for (int d = 0; d < 2; d++) {
+    bool on = d < ndim;
+    g.nx[d] = on ? nX[d] : 1;
+    g.nw[d] = on ? nX[d] + 2 : 1;
+    g.nb[d] = on ? nX[d] + 2 * N : 1;
+    g.bc[d] = on ? boundaryTypes[d] : 0;
+    g.dX[d] = on ? dX[d] : 1.0;
+    for (int v = 0; v < MAXV; v++) g.refl[d][v] = (on && opts.reflect && v < V) ? opts.reflect[d * V + v] : 1.0;
+    g.ncell *= g.nx[d];
+    g.nwc *= g.nw[d];
+  }
+  for (int d = 0; d < ndim; d++) {
+    g.nf[d] = 1;
+    for (int e = 0; e < ndim; e++) g.nf[d] *= (e == d) ? nX[e] + 1 : nX[e];
+  }
+  auto bail = [&](int code_) {
+    rc = code_;
+    pypde_destroy(c);
+    return (pypde_ctx*)nullptr;
+  };
+  if (hipSetDevice(c->device) != hipSuccess)
+    return bail(fail(PYPDE_ERR_HIP, "hipSetDevice(%d) failed: no usable GPU", c->device));
+  if (hipModuleLoadData(&c->mod, code.data()) != hipSuccess)
+    return bail(fail(PYPDE_ERR_HIP, "hipModuleLoadData failed"));
+  for (int k = 0; k < K_COUNT; k++) {
+    if (k == K_FLUX1 && ndim < 2) continue;
+    if (hipModuleGetFunction(&c->fn[k], c->mod, kernel_names[k]) != hipSuccess)
+      return bail(fail(PYPDE_ERR_HIP, "kernel %s missing", kernel_names[k]));
+  }
+  if (hipStreamCreate(&c->stream) != hipSuccess)
+    return bail(fail(PYPDE_ERR_HIP, "hipStreamCreate failed"));
+  size_t NTR = 2 * (size_t)ndim * N * (c->ND / N) * V;
+  auto alloc = [&](double** p, size_t n) {
+    return hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(double)) == hipSuccess;
+  };
+  bool ok = alloc(&c->u, g.ncell * V) && alloc(&c->wh, g.nwc * c->ND * V) &&
+            alloc(&c->tr, g.nwc * NTR) && alloc(&c->tmp, g.ncell * V) &&
+            ((!useS && !useB) || alloc(&c->vol, g.nwc * V));
+  for (int d = 0; d < ndim && ok; d++) {
+    size_t fs = (size_t)g.nf[d] * N * (c->ND / N) * V;
+    ok = (!useF || alloc(&c->ff[d], fs)) && (!useB || alloc(&c->fb[d], fs));
+  }
+  if (ok && opts.guess) {
+    ok = alloc(&c->guess, g.nwc * c->NT * V);
+    if (ok) {
+      // host layout [cell][t][node][v] -> device planes [(t*ND+node)*V+v][cell]
+      std::vector<double> h(g.nwc * c->NT * V);
+      for (long cc = 0; cc < g.nwc; cc++)
+        for (int p = 0; p < c->NT * V; p++) h[(size_t)p * g.nwc + cc] = opts.guess[cc * c->NT * V + p];
+      ok = hipMemcpy(c->guess, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+    }
+  }
+  ok = ok && hipMalloc((void**)&c->st, sizeof(StepState)) == hipSuccess;
+  if (!ok) return bail(fail(PYPDE_ERR_HIP, "device allocation failed"));
+  memset(&c->host_st, 0, sizeof c->host_st);
+  c->host_st.cfl = CFL;
+  c->host_st.done = 1;
+  if (hipMemcpy(c->st, &c->host_st, sizeof(StepState), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(PYPDE_ERR_HIP, "state upload failed"));
+  rc = 0;
+  return c;
+}
+
+int pypde_set_state(pypde_ctx* c, const double* u) {
+  HIPCHK(hipSetDevice(c->device));
+  size_t n = c->g.ncell * c->V;
+  HIPCHK(hipMemcpyAsync(c->tmp, u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  std::vector<void*> a = {&c->tmp, &c->u, &c->g.ncell};
+  int rc = launch(c, K_A2S, dim3((unsigned)cdiv(n, 256)), dim3(256), a);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int pypde_get_state(pypde_ctx* c, double* u) {
+  HIPCHK(hipSetDevice(c->device));
+  size_t n = c->g.ncell * c->V;
+  std::vector<void*> a = {&c->u, &c->tmp, &c->g.ncell};
+  int rc = launch(c, K_S2A, dim3((unsigned)cdiv(n, 256)), dim3(256), a);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(u, c->tmp, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+void* pypde_device_state(pypde_ctx* c) { return c->u; }
+void* pypde_stream(pypde_ctx* c) { return c->stream; }
+
+static int run_loop(pypde_ctx* c, long max_steps, pypde_stats* stats, bool first_guess) {
+  auto t0 = std::chrono::steady_clock::now();
+  int chunk = 4;
+  bool guess_now = first_guess;
+  std::vector<hipEvent_t> evs;
+  while (true) {
+    for (int i = 0; i < chunk; i++) {
+      std::vector<hipEvent_t>* ep = c->opts.profile ? &evs : nullptr;
+      int rc = enqueue_step(c, guess_now, ep);
+      if (rc) return rc;
+      guess_now = false;
+    }
+    HIPCHK(hipMemcpyAsync(&c->host_st, c->st, sizeof(StepState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->host_st.done) break;
+    if (chunk < 64) chunk *= 2;
+    (void)max_steps;
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  if (c->opts.profile) {
+    // 8 marks per step bracket: weno | dt | predictor | flux0 | flux1 | update | advance
+    for (size_t s0 = 0; s0 + 8 <= evs.size(); s0 += 8)
+      for (int k = 0; k < 7; k++) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, evs[s0 + k], evs[s0 + k + 1]);
+        c->ms_kernel[k] += ms;
+      }
+    for (auto e : evs) hipEventDestroy(e);
+  }
+  if (stats) {
+    stats->steps = c->host_st.count;
+    stats->t = c->host_st.t;
+    stats->pred_iters = (long)c->host_st.pred_iters;
+    stats->pred_cells = (long)c->host_st.pred_cells;
+    stats->pred_maxit = (long)c->host_st.pred_maxit;
+    stats->pred_capped = (long)c->host_st.pred_capped;
+    stats->guess_fallbacks = (long)c->host_st.pred_fallbacks;
+    stats->nan_step = c->host_st.nan_step;
+    stats->ms_loop = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    for (int k = 0; k < 8; k++) stats->ms_kernel[k] = c->ms_kernel[k];
+  }
+  return c->host_st.nan_step ? -c->host_st.nan_step : 0;
+}
+
+int pypde_run(pypde_ctx* c, double tf, long max_steps, int ndt, double* ret, pypde_stats* stats) {
+  HIPCHK(hipSetDevice(c->device));
+  if (ndt < 0) return fail(PYPDE_ERR_ARG, "ndt < 0");
+  size_t rows = (size_t)ndt;
+  size_t row = c->g.ncell * c->V;
+  if (rows > c->ret_rows) {
+    if (c->ret) hipFree(c->ret);
+    c->ret = nullptr;
+    HIPCHK(hipMalloc((void**)&c->ret, rows * row * sizeof(double)));
+    c->ret_rows = rows;
+  }
+  if (rows) HIPCHK(hipMemsetAsync(c->ret, 0, rows * row * sizeof(double), c->stream));
+  c->g.ndt = ndt;
+  StepState s;
+  memset(&s, 0, sizeof s);
+  s.tf = tf;
+  s.cfl = c->host_st.cfl;
+  s.max_steps = max_steps;
+  s.done = (0.0 < tf) ? 0 : 1;
+  HIPCHK(hipMemcpyAsync(c->st, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
+  for (double& m : c->ms_kernel) m = 0;
+  double* keep_ret = c->ret;
+  if (!rows) c->ret = nullptr;
+  int rc = run_loop(c, max_steps, stats, c->guess != nullptr);
+  c->ret = keep_ret;
+  if (rc > 0) return rc;
+  if (rows) {
+    // final state into the last row (iterator.cpp:150)
+    std::vector<void*> a = {&c->u, &c->tmp, &c->g.ncell};
+    int r2 = launch(c, K_S2A, dim3((unsigned)cdiv(row, 256)), dim3(256), a);
+    if (r2) return r2;
+    HIPCHK(hipMemcpyAsync(c->ret + (rows - 1) * row, c->tmp, row * sizeof(double),
+                          hipMemcpyDeviceToDevice, c->stream));
+    if (ret) HIPCHK(hipMemcpyAsync(ret, c->ret, rows * row * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return rc;
+}
+
+int pypde_step(pypde_ctx* c, long steps, double tf, pypde_stats* stats) {
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync(&c->host_st, c->st, sizeof(StepState), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  StepState s = c->host_st;
+  s.tf = tf;
+  s.max_steps = s.count + steps;
+  s.done = (s.t < tf && steps > 0) ? 0 : 1;
+  c->g.ndt = 0;
+  HIPCHK(hipMemcpyAsync(c->st, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
+  for (double& m : c->ms_kernel) m = 0;
+  double* keep_ret = c->ret;
+  c->ret = nullptr;
+  int rc = run_loop(c, s.max_steps, stats, false);
+  c->ret = keep_ret;
+  return rc;
+}
+
+int pde_solver_hip(const char* src, bool useF, bool useB, bool useS, double* u, double tf,
+                   int* nX, int ndim, double* dX, double CFL, int* boundaryTypes, bool STIFF,
+                   int FLUX, int N, int V, int ndt, bool secondOrder, double* ret,
+                   const pypde_options* opts, pypde_stats* stats) {
+  if (!u || !ret || ndt < 1) return fail(PYPDE_ERR_ARG, "u, ret and ndt >= 1 are required");
+  int rc = 0;
+  pypde_ctx* c = pypde_create(src, useF, useB, useS, nX, ndim, dX, CFL, boundaryTypes, STIFF,
+                              FLUX, N, V, secondOrder, opts, &rc);
+  if (!c) return rc ? rc : PYPDE_ERR_HIP;
+  if ((rc = pypde_set_state(c, u))) {
+    pypde_destroy(c);
+    return rc;
+  }
+  long ms = opts ? opts->max_steps : 0;
+  rc = pypde_run(c, tf, ms, ndt, ret, stats);
+  if (rc <= 0) {
+    int r2 = pypde_get_state(c, u);
+    if (r2) rc = r2;
+  }
+  pypde_destroy(c);
+  return rc;
+}
+
+int weno_solver(double* ret, double* u, int* nX, int ndim, int N, int V) {
+  Spec s;
+  make_spec(&s, "", false, false, false, ndim, N, V, false, nullptr);
+  int rc = check_spec(s, false, 0);
+  if (rc) return rc;
+  long nin = 1, nout = 1;
+  int no[2] = {1, 1};
+  for (int d = 0; d < ndim; d++) {
+    no[d] = nX[d] - 2 * (N - 1);
+    if (no[d] < 1) return fail(PYPDE_ERR_ARG, "nX[%d]=%d too small for N=%d", d, nX[d], N);
+    nin *= nX[d];
+    nout *= no[d];
+  }
+  std::vector<char> code;
+  if ((rc = compile_code(s, &code))) return rc;
+  HIPCHK(hipSetDevice(0));
+  hipModule_t mod;
+  HIPCHK(hipModuleLoadData(&mod, code.data()));
+  hipFunction_t fw, fa;
+  HIPCHK(hipModuleGetFunction(&fw, mod, "k_weno"));
+  HIPCHK(hipModuleGetFunction(&fa, mod, "k_aos_to_soa"));
+  int ND = 1;
+  for (int d = 0; d < ndim; d++) ND *= N;
+  double *du = nullptr, *dus = nullptr, *dw = nullptr;
+  HIPCHK(hipMalloc((void**)&du, nin * V * sizeof(double)));
+  HIPCHK(hipMalloc((void**)&dus, nin * V * sizeof(double)));
+  HIPCHK(hipMalloc((void**)&dw, nout * ND * V * sizeof(double)));
+  HIPCHK(hipMemcpy(du, u, nin * V * sizeof(double), hipMemcpyHostToDevice));
+  {
+    void* a[] = {&du, &dus, &nin};
+    HIPCHK(hipModuleLaunchKernel(fa, (unsigned)cdiv(nin * V, 256), 1, 1, 256, 1, 1, 0, nullptr, a, nullptr));
+  }
+  Geom g;
+  memset(&g, 0, sizeof g);
+  const StepState* st = nullptr;
+  int ghost = 0, n0 = nX[0], n1 = ndim > 1 ? nX[1] : 1;
+  {
+    void* a[] = {&dus, &dw, &st, &g, &ghost, &n0, &n1, &nout};
+    unsigned gx = ndim == 1 ? (unsigned)cdiv(no[0], 256) : (unsigned)cdiv(no[1], 32);
+    unsigned gy = ndim == 1 ? 1 : (unsigned)cdiv(no[0], 8);
+    HIPCHK(hipModuleLaunchKernel(fw, gx, gy, 1, 256, 1, 1, 0, nullptr, a, nullptr));
+  }
+  std::vector<double> h(nout * ND * V);
+  HIPCHK(hipMemcpy(h.data(), dw, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  // planes [node*V+v][cell] -> reference layout [cell][node][v]
+  for (long cc = 0; cc < nout; cc++)
+    for (int p = 0; p < ND * V; p++) ret[cc * ND * V + p] = h[(size_t)p * nout + cc];
+  hipFree(du);
+  hipFree(dus);
+  hipFree(dw);
+  hipModuleUnload(mod);
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,177 @@
+"""Python front end with the reference's call surface (pypde/solvers.py:13-244).
+
+``pde_solver(Q0, tf, L, F, B, S, ...)`` keeps the reference's arguments,
+defaults, output shape ``(ndt,) + Q0.shape`` and in-place update of a
+C-contiguous float64 ``Q0``.  The physics are HIP device source instead of
+numba-compiled Python callables (the reference JIT-compiles them with numba,
+pypde/cfuncs.py:30-75; here hipRTC compiles them into the kernels):
+
+    F = '''PDE_FN void F(double *out, const double *Q, const double *dQ, int d) {...}'''
+    B = '''PDE_FN void B(double *out, const double *Q, int d) {...}'''   # V x V row-major
+    S = '''PDE_FN void S(double *out, const double *Q) {...}'''
+    M = '''PDE_FN void M(double *out, const double *Q) {...}'''          # dS/dQ (optional)
+
+or ``system=`` one source defining several of them (see pypde_amd/systems).
+"""
+import ctypes as C
+
+import numpy as np
+
+from pypde_amd import _capi
+
+FLUXES = {"rusanov": 0, "roe": 1, "osher": 2}
+BOUNDARIES = {"transitive": 0, "periodic": 1, "reflective": 2}
+
+
+def parse_boundary_types(boundaryTypes, ndim):
+    """As pypde/utils.py:37-66, plus 'reflective'; raises instead of sys.exit."""
+    err = ("boundaryTypes must be a string from {'transitive', 'periodic', 'reflective'} "
+           "or a list of such strings, of length equal to the number of dimensions")
+    if isinstance(boundaryTypes, str):
+        if boundaryTypes not in BOUNDARIES:
+            raise ValueError(err)
+        ret = [BOUNDARIES[boundaryTypes]] * ndim
+    elif isinstance(boundaryTypes, (list, tuple)):
+        if len(boundaryTypes) != ndim or any(b not in BOUNDARIES for b in boundaryTypes):
+            raise ValueError(err)
+        ret = [BOUNDARIES[b] for b in boundaryTypes]
+    else:
+        raise ValueError(err)
+    return np.array(ret, dtype=np.int32)
+
+
+def _source(F, B, S, M, system):
+    if system is not None:
+        has = {k: f"#define PDE_HAS_{k}" in system for k in "FBSM"}
+        return system, has["F"], has["B"], has["S"], has["M"]
+    parts = [x for x in (F, B, S, M) if x]
+    return "\n".join(parts), F is not None, B is not None, S is not None, M is not None
+
+
+def pde_solver(Q0, tf, L, F=None, B=None, S=None, boundaryTypes="transitive", cfl=0.9,
+               order=2, ndt=100, flux="rusanov", stiff=True, nThreads=-1, M=None,
+               system=None, reflect=None, newton_tol=1e-12, newton_maxit=100, max_steps=0,
+               b_quirk=False, device=0, stats=None):
+    """Solve dQ/dt + div F(Q) + B(Q).grad Q = S(Q) on [0, L_1] x ... x [0, L_n]
+    with the ADER-WENO method on one MI355X.  Returns ``(ndt,) + Q0.shape``.
+
+    Extra keywords (not in the reference): ``M`` / ``system`` (above),
+    ``reflect`` (ndim x V multipliers for 'reflective' walls), ``newton_tol`` /
+    ``newton_maxit`` (stiff predictor), ``max_steps`` (fixed step window),
+    ``b_quirk`` (evaluate B.dQ as the reference's build does), ``stats`` (dict
+    filled with run statistics).  ``nThreads`` is accepted and ignored."""
+    Q0 = np.asarray(Q0)
+    nX = np.array(Q0.shape[:-1], dtype=np.int32)
+    ndim = len(nX)
+    V = Q0.shape[-1]
+    dX = np.array([L[i] / nX[i] for i in range(len(L))], dtype=np.float64)
+    bcs = parse_boundary_types(boundaryTypes, ndim)
+    if flux not in FLUXES:
+        raise ValueError(f"flux must be one of {sorted(FLUXES)}")
+    src, useF, useB, useS, useM = _source(F, B, S, M, system)
+    secondOrder = False
+    if useF and "dQ[" in src and "PDE_SECOND_ORDER" in src:
+        secondOrder = True
+    inplace = Q0.dtype == np.float64 and Q0.flags.c_contiguous
+    u = Q0.ravel() if inplace else np.ascontiguousarray(Q0, dtype=np.float64).ravel().copy()
+    ret = np.zeros(ndt * u.size)
+    if reflect is None and 2 in bcs:
+        reflect = np.ones((ndim, V))
+    opts = _capi.make_options(newton_tol=newton_tol, newton_maxit=newton_maxit,
+                              max_steps=max_steps, reflect=reflect, b_quirk=int(b_quirk),
+                              use_M=useM, device=device)
+    st = _capi.Stats()
+    st.struct_size = C.sizeof(_capi.Stats)
+    rc = _capi.lib().pde_solver_hip(src.encode(), useF, useB, useS, _capi.dp(u), float(tf),
+                                    _capi.ip(nX), ndim, _capi.dp(dX), float(cfl), _capi.ip(bcs),
+                                    bool(stiff), FLUXES[flux], int(order), V, int(ndt),
+                                    secondOrder, _capi.dp(ret), C.byref(opts), C.byref(st))
+    _capi.check(rc, "pde_solver_hip")
+    if stats is not None:
+        stats.update(st.as_dict())
+        stats["rc"] = rc
+    if rc < 0:
+        print("NaNs found")
+    return ret.reshape((ndt,) + Q0.shape)
+
+
+def weno_solver(u, order=2):
+    """WENO reconstruction without ghost cells (pypde/solvers.py:217-244)."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    nX = np.array(u.shape[:-1], dtype=np.int32)
+    ndim = len(nX)
+    V = u.shape[-1]
+    nXret = nX - 2 * (order - 1)
+    ret = np.zeros(int(np.prod(nXret)) * order ** ndim * V)
+    rc = _capi.lib().weno_solver(_capi.dp(ret), _capi.dp(u), _capi.ip(nX), ndim, order, V)
+    _capi.check(rc, "weno_solver")
+    return ret.reshape(tuple(nXret) + (order,) * ndim + (V,))
+
+
+class Solver:
+    """Resident solver: compile once, keep the grid in HBM, step repeatedly."""
+
+    def __init__(self, u0, L, system, ndim=None, order=3, boundaryTypes="transitive", cfl=0.9,
+                 stiff=False, reflect=None, newton_tol=1e-12, newton_maxit=100, b_quirk=False,
+                 profile=False, device=0, guess=None):
+        u0 = np.ascontiguousarray(u0, dtype=np.float64)
+        self.shape = u0.shape
+        self.nX = np.array(u0.shape[:-1], dtype=np.int32)
+        nd = len(self.nX)
+        V = u0.shape[-1]
+        self.dX = np.array([L[i] / self.nX[i] for i in range(nd)], dtype=np.float64)
+        self.bcs = parse_boundary_types(boundaryTypes, nd)
+        src, useF, useB, useS, useM = _source(None, None, None, None, system)
+        if reflect is None and 2 in self.bcs:
+            reflect = np.ones((nd, V))
+        self.opts = _capi.make_options(newton_tol=newton_tol, newton_maxit=newton_maxit,
+                                       reflect=reflect, b_quirk=int(b_quirk), use_M=useM,
+                                       profile=profile, device=device, guess=guess)
+        rc = C.c_int(0)
+        self.h = _capi.lib().pypde_create(src.encode(), useF, useB, useS, _capi.ip(self.nX), nd,
+                                          _capi.dp(self.dX), float(cfl), _capi.ip(self.bcs),
+                                          bool(stiff), 0, int(order), V, False,
+                                          C.byref(self.opts), C.byref(rc))
+        if not self.h:
+            _capi.check(rc.value or 3, "pypde_create")
+        self.set_state(u0)
+
+    def set_state(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        _capi.check(_capi.lib().pypde_set_state(self.h, _capi.dp(u)), "set_state")
+
+    def state(self):
+        out = np.zeros(self.shape)
+        _capi.check(_capi.lib().pypde_get_state(self.h, _capi.dp(out)), "get_state")
+        return out
+
+    def run(self, tf, max_steps=0, ndt=0):
+        st = _capi.Stats()
+        st.struct_size = C.sizeof(_capi.Stats)
+        ret = np.zeros((max(ndt, 1),) + self.shape) if ndt else None
+        rc = _capi.lib().pypde_run(self.h, float(tf), int(max_steps), int(ndt),
+                                   _capi.dp(ret) if ret is not None else None, C.byref(st))
+        _capi.check(rc, "run")
+        d = st.as_dict()
+        d["rc"] = rc
+        return (ret, d) if ndt else d
+
+    def step(self, steps, tf=1e300):
+        st = _capi.Stats()
+        st.struct_size = C.sizeof(_capi.Stats)
+        rc = _capi.lib().pypde_step(self.h, int(steps), float(tf), C.byref(st))
+        _capi.check(rc, "step")
+        d = st.as_dict()
+        d["rc"] = rc
+        return d
+
+    def close(self):
+        if getattr(self, "h", None):
+            _capi.lib().pypde_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

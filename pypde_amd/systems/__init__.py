@@ -1,0 +1,115 @@
+"""PDE systems shipped as single-source HIP/C text (*.pde), plus the initial
+conditions of the workloads this package is measured on.
+
+A system source defines F, B, S and optionally M (see pde_prelude.h).  The
+same text is what a user hands to ``pde_solver`` for their own physics.
+"""
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+# name: (V, ndim it is written for, has F, has B, has S, has M)
+SYSTEMS = {
+    "euler1d": dict(V=3, F=True, B=False, S=False, M=False),
+    "euler2d": dict(V=4, F=True, B=False, S=False, M=False),
+    "reactive2d": dict(V=5, F=True, B=False, S=True, M=True),
+    "advect": dict(V=1, F=True, B=False, S=False, M=False),
+    "ncadvect": dict(V=2, F=False, B=True, S=True, M=False),
+}
+
+
+def source(name):
+    with open(os.path.join(HERE, name + ".pde")) as f:
+        return f.read()
+
+
+def info(name):
+    return dict(SYSTEMS[name])
+
+
+def standard_configs(full=False):
+    """(system, ndim, N, stiff) combinations used by the tests, smoke and bench."""
+    combos = [("euler1d", 1, 3, False), ("euler2d", 2, 2, False), ("euler2d", 2, 3, False),
+              ("euler2d", 2, 4, False), ("reactive2d", 2, 3, True)]
+    if full:
+        combos += [("euler1d", 1, 2, False), ("advect", 1, 3, False), ("advect", 2, 3, False),
+                   ("ncadvect", 2, 3, False), ("reactive2d", 2, 3, False),
+                   ("euler2d", 2, 3, True), ("ncadvect", 2, 3, True)]
+    out = []
+    for name, ndim, N, stiff in combos:
+        s = SYSTEMS[name]
+        out.append(dict(src=source(name), useF=s["F"], useB=s["B"], useS=s["S"], ndim=ndim, N=N,
+                        V=s["V"], stiff=stiff, use_M=s["M"]))
+    return out
+
+
+# ------------------------------------------------------------ workloads
+
+def sod1d(n=200, gamma=1.4):
+    """Reference Sod initial state (pypde/tests/euler/__init__.py:25-44) in 1D:
+    (rho, u, p) = (1, 0, 1) left of x = 0.5, (0.125, 0, 0.1) right."""
+    u = np.zeros((n, 3))
+    for i in range(n):
+        r, p = (1.0, 1.0) if i / n < 0.5 else (0.125, 0.1)
+        u[i] = [r, 0.0, p / (gamma - 1)]
+    return u
+
+
+def splitmix64(seed, count):
+    """`count` uniforms in [0, 1) from the SplitMix64 stream of `seed`:
+    x_i = seed + (i + 1) * 0x9E3779B97F4A7C15, mixed, top 53 bits scaled."""
+    with np.errstate(over="ignore"):
+        i = np.arange(1, count + 1, dtype=np.uint64)
+        z = np.uint64(seed) + i * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def reactive_channel(nx=1024, ny=256, seed=0, gamma=1.2, Qc=25.0):
+    """2D reactive Euler channel: burnt high-pressure slab (p=20, rho=1.8,
+    lambda=0) for x < 0.1 over quiescent unburnt gas (rho=1, p=1, lambda=1),
+    density scaled per cell by 1 + 0.05 U(-1, 1) (SplitMix64 stream `seed`).
+    Domain [0,1] x [0, ny/nx]; conserved (rho, rho u, rho v, E, rho lambda)."""
+    noise = (2.0 * splitmix64(seed, nx * ny) - 1.0).reshape(nx, ny)
+    x = (np.arange(nx) + 0.5) / nx
+    X = np.broadcast_to(x[:, None], (nx, ny))
+    burnt = X < 0.1
+    rho = np.where(burnt, 1.8, 1.0) * (1.0 + 0.05 * noise)
+    p = np.where(burnt, 20.0, 1.0)
+    lam = np.where(burnt, 0.0, 1.0)
+    E = p / (gamma - 1.0) + Qc * rho * lam
+    z = np.zeros_like(rho)
+    return np.ascontiguousarray(np.stack([rho, z, z, E, rho * lam], -1))
+
+
+REACTIVE_REFLECT = np.array([[1, -1, 1, 1, 1], [1, 1, -1, 1, 1]], dtype=np.float64)
+
+
+def euler2d_modes(n=512, seed=0, gamma=1.4, amp=0.2, vamp=0.1):
+    """2D periodic Euler: rho = 1 + amp * P_rho, u, v = 0.5 + vamp * P_{u,v}, p = 1,
+    each P a sum of 8 seeded Fourier modes (wavenumbers 1..4, amplitudes
+    U(0,1), phases U(0, 2 pi)) normalised to max |P| = 1."""
+    r = iter(splitmix64(seed, 3 * 8 * 4))
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x, indexing="ij")
+
+    def field():
+        P = np.zeros_like(X)
+        for _ in range(8):
+            kx = 1 + int(next(r) * 4)
+            ky = 1 + int(next(r) * 4)
+            a = next(r)
+            ph = 2 * np.pi * next(r)
+            P += a * np.sin(2 * np.pi * (kx * X + ky * Y) + ph)
+        return P / np.max(np.abs(P))
+
+    rho = 1 + amp * field()
+    u = 0.5 + vamp * field()
+    v = 0.5 + vamp * field()
+    p = np.ones_like(X)
+    E = p / (gamma - 1) + 0.5 * rho * (u * u + v * v)
+    return np.ascontiguousarray(np.stack([rho, rho * u, rho * v, E], -1))

@@ -1,0 +1,206 @@
+"""HIP path vs the CPU oracle.
+
+The kernels issue the oracle's floating-point operations in the same order, so
+on identical inputs the expected difference is zero; every test states the
+tolerance it enforces (north star: max relative L-inf <= 1e-10 for FP64).
+CPU-only tests here check the C ABI itself (exports, tables, errors)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cpu_checkers as cc
+from pypde_amd import _capi, systems
+from pypde_amd.solvers import Solver, pde_solver, weno_solver
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TOL = 1e-10   # max relative L-inf, per variable
+
+
+def relerr(a, b):
+    V = b.shape[-1]
+    sc = np.maximum(np.max(np.abs(b.reshape(-1, V)), axis=0), 1e-300)
+    return float(np.max(np.abs((a - b).reshape(-1, V)) / sc))
+
+
+# ------------------------------------------------------------ CPU checks
+
+def test_library_exports_header_symbols():
+    hdr = open(os.path.join(ROOT, "include", "pypde_hip.h")).read()
+    names = re.findall(r"^\s*(?:int|void|const char|pypde_ctx)\s*\*?\s*(\w+)\s*\(", hdr, re.M)
+    assert len(names) >= 12
+    lib = _capi.lib()
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
+def test_tables_bitwise_equal_to_oracle(N):
+    p, o = _capi.tables(N), cc.tables(N)
+    for k in p:
+        assert np.array_equal(p[k], o[k]), k
+
+
+def test_argument_errors_are_reported():
+    with pytest.raises(ValueError):
+        pde_solver(np.ones((8, 3)), 0.1, [1.0], system=systems.source("euler1d"),
+                   boundaryTypes="bogus")
+    o = _capi.make_options()
+    rc = _capi.lib().pypde_hip_compile(systems.source("euler2d").encode(), True, False, False,
+                                       3, 3, 4, False, C.byref(o))
+    assert rc == 4 and "ndim=3" in _capi.last_error()
+    rc = _capi.lib().pypde_hip_compile(b"this is not C", True, False, False, 1, 3, 3, False,
+                                       C.byref(o))
+    assert rc == 2 and "error" in _capi.last_error()
+
+
+def test_no_cpu_fallback_without_gpu():
+    """The product path has no CPU fallback: without a GPU it fails loudly."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(_capi.PypdeError, match="GPU|hip"):
+        pde_solver(systems.sod1d(32), 0.01, [1.0], system=systems.source("euler1d"), order=3,
+                   ndt=1, stiff=False)
+
+
+# ------------------------------------------------------------ GPU parity
+
+def oracle_run(name, u0, tf, L, bc, N, stiff, **kw):
+    return cc.oracle_solve(name, u0, tf, L, bc, N, stiff=stiff, **kw)
+
+
+def gpu_run(name, u0, tf, L, bc, N, stiff, ndt=1, **kw):
+    st = {}
+    u = np.ascontiguousarray(u0, dtype=np.float64).copy()
+    ret = pde_solver(u, tf, L, system=systems.source(name), boundaryTypes=bc, order=N, ndt=ndt,
+                     stiff=stiff, stats=st, **kw)
+    return u, ret, st
+
+
+BCNAME = {0: "transitive", 1: "periodic", 2: "reflective"}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ndim,N,V", [(1, 2, 3), (1, 3, 4), (2, 2, 3), (2, 3, 5), (2, 4, 4)])
+def test_weno_solver_matches_oracle(ndim, N, V):
+    rng = np.random.default_rng(40 + N)
+    nX = [37, 29][:ndim]
+    u = rng.uniform(0.5, 2.0, tuple(nX) + (V,))
+    got = weno_solver(u, order=N)
+    nXa = np.array(nX, dtype=np.int32)
+    want = np.zeros(got.size)
+    cc.oracle().oracle_weno(cc.dp(want), cc.dp(np.ascontiguousarray(u)), cc.ip(nXa), ndim, N, V)
+    assert np.array_equal(got.ravel(), want)
+
+
+@pytest.mark.gpu
+def test_sod_order3_matches_oracle():
+    """C0: 1D Sod, 200 cells, order 3, Rusanov, CFL 0.9, tf 0.2, 4 snapshots."""
+    u0 = systems.sod1d()
+    uo, ro, st, rc = oracle_run("euler1d", u0, 0.2, [1.0], 0, 3, False, ndt=4)
+    ug, rg, sg = gpu_run("euler1d", u0, 0.2, [1.0], "transitive", 3, False, ndt=4)
+    print("sod: steps", sg["steps"], "oracle", st.steps, "err", relerr(ug, uo))
+    assert rc == 0 and sg["rc"] == 0
+    assert sg["steps"] == st.steps == 103
+    assert sg["pred_iters"] == st.pred_iters
+    assert relerr(ug, uo) <= TOL
+    assert relerr(rg, ro) <= TOL
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N,bc", [(2, 1), (3, 1), (4, 1), (3, 0)])
+def test_euler2d_matches_oracle(N, bc):
+    u0 = systems.euler2d_modes(40, seed=3)
+    if bc == 0:
+        u0 = u0[:, :24].copy()
+    L = [1.0, u0.shape[1] / u0.shape[0]]
+    uo, ro, st, rc = oracle_run("euler2d", u0, 0.04, L, bc, N, False)
+    ug, rg, sg = gpu_run("euler2d", u0, 0.04, L, BCNAME[bc], N, False)
+    print(f"euler2d N={N} bc={bc}: steps {sg['steps']} err {relerr(ug, uo):.3e}")
+    assert sg["steps"] == st.steps and sg["pred_iters"] == st.pred_iters
+    assert relerr(ug, uo) <= TOL
+
+
+def reactive_small(nx=64, ny=16, seed=0):
+    return systems.reactive_channel(nx, ny, seed=seed)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("stiff", [True, False])
+def test_reactive_channel_matches_oracle(stiff):
+    """C2 physics on a 64 x 16 window of the 1024 x 256 grid spacing (dx = 1/1024),
+    reflective walls, fixed 12-step window; stiff = per-cell Newton with the
+    analytic dS/dQ."""
+    u0 = reactive_small()
+    L = [64 / 1024, 16 / 1024]
+    refl = systems.REACTIVE_REFLECT
+    uo, ro, st, rc = oracle_run("reactive2d", u0, 1.0, L, 2, 3, stiff, max_steps=12,
+                                reflect=refl.ravel())
+    ug, rg, sg = gpu_run("reactive2d", u0, 1.0, L, "reflective", 3, stiff, max_steps=12,
+                         reflect=refl)
+    print("reactive stiff", stiff, sg, "err", relerr(ug, uo))
+    assert sg["steps"] == st.steps == 12
+    assert sg["pred_iters"] == st.pred_iters and sg["pred_capped"] == st.pred_capped == 0
+    assert relerr(ug, uo) <= TOL
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("quirk", [0, 1])
+def test_nonconservative_b_matches_oracle(quirk):
+    x = (np.arange(24) + 0.5) / 24
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    u0 = np.stack([np.sin(2 * np.pi * X), np.cos(2 * np.pi * Y)], -1)
+    uo, ro, st, rc = oracle_run("ncadvect", u0, 0.05, [1, 1], 1, 3, False, b_quirk=quirk)
+    ug, rg, sg = gpu_run("ncadvect", u0, 0.05, [1, 1], "periodic", 3, False, b_quirk=quirk)
+    assert sg["steps"] == st.steps
+    assert relerr(ug, uo) <= TOL
+
+
+@pytest.mark.gpu
+def test_learned_guess_path_matches_oracle():
+    """Predictor started from supplied guesses (the learned-guess hook of
+    dg.cpp:216): GPU and oracle take the same iterations, fall back on the
+    same cells, and the converged state does not depend on the guess."""
+    u0 = reactive_small(48, 12)
+    L = [48 / 1024, 12 / 1024]
+    refl = systems.REACTIVE_REFLECT
+    nw = 50 * 14
+    NT, V = 27, 5
+    rng = np.random.default_rng(9)
+    # garbage guesses: the residual grows after the first step on many cells,
+    # which must trigger the fallback to the plain guess identically
+    guess = rng.uniform(-1, 1, (nw, NT, V)) * 1e3
+    opts = cc.OracleOpts()
+    opts.newton_tol = 1e-12
+    opts.newton_maxit = 100
+    opts.max_steps = 1
+    r = np.ascontiguousarray(refl.ravel())
+    opts.reflect = cc.dp(r)
+    gg = np.ascontiguousarray(guess)
+    opts.guess = cc.dp(gg)
+    opts.guess_check = 1
+    uo = np.ascontiguousarray(u0).copy()
+    nX = np.array(u0.shape[:2], dtype=np.int32)
+    ret = np.zeros(uo.size)
+    sto = cc.OracleStats()
+    s = cc.system("reactive2d")
+    dX = np.array([1 / 1024, 1 / 1024])
+    rc = cc.oracle().oracle_pde_solver(s.F, None, s.S, s.M, 1, 0, 1, cc.dp(uo), 1.0, cc.ip(nX), 2,
+                                       cc.dp(dX), 0.9, cc.ip(np.array([2, 2], dtype=np.int32)), 1,
+                                       0, 3, V, 1, 0, cc.dp(ret), C.byref(opts), C.byref(sto))
+    assert rc == 0 and sto.guess_fallbacks > 0
+    sol = Solver(u0, L, systems.source("reactive2d"), order=3, boundaryTypes="reflective",
+                 stiff=True, reflect=refl, guess=guess)
+    sg = sol.run(1.0, max_steps=1)
+    ug = sol.state()
+    sol.close()
+    assert sg["guess_fallbacks"] == sto.guess_fallbacks
+    assert sg["pred_iters"] == sto.pred_iters
+    assert relerr(ug, uo) <= TOL
+    # same converged answer as the plain guess, to the Newton tolerance scale
+    up, _, _, _ = oracle_run("reactive2d", u0, 1.0, L, 2, 3, True, max_steps=1,
+                             reflect=refl.ravel())
+    assert relerr(uo, up) <= 1e-9
