@@ -399,11 +399,16 @@ static void house_right(double *H, int n, const double *v, int m, double beta,
 }
 
 /* max |lambda| of a real n x n matrix, the quantity the reference takes from
- * Eigen's EigenSolver (system.cpp:28-43; Eigen RealSchur scales by the largest
- * |entry|, RealSchur.h:258-276).  Method: scale, Householder reduction to
- * Hessenberg form, then Francis double-shift QR steps (Golub & Van Loan,
- * Alg. 7.5.1-7.5.2) with deflation on negligible subdiagonals and ad hoc
- * exceptional shifts after 10 and 20 iterations without deflation. */
+ * Eigen's EigenSolver (system.cpp:28-43).  Method (EISPACK hqr structure,
+ * eigenvalues only): scale by the largest |entry| (as Eigen's RealSchur,
+ * RealSchur.h:258-276), Householder reduction to Hessenberg form, then Francis
+ * double-shift QR steps (Golub & Van Loan Alg. 7.5.1-7.5.2) on the active
+ * block, deflating on subdiagonals below eps (|h_kk| + |h_k-1,k-1|).  After 10
+ * and 30 steps without deflation the origin is moved (exceptional shifts of
+ * Wilkinson and of Moler, accumulated in `ex`), which is what lets clusters of
+ * nearly equal small eigenvalues, as finite-difference Jacobians of
+ * multi-wave systems produce, deflate at all.  After 40 n steps the remaining
+ * diagonal entries stand in for the unresolved eigenvalues. */
 double oracle_max_abs_eig(const double *A, int n) {
   double H[MAXV * MAXV];
   double scale = 0.0;
@@ -423,7 +428,7 @@ double oracle_max_abs_eig(const double *A, int n) {
   }
   double anorm = 0.0;
   for (int i = 0; i < n * n; i++) anorm = anorm + fabs(H[i]);
-  double best = 0.0;
+  double best = 0.0, ex = 0.0;
   int hi = n - 1, iter = 0, total = 0;
   while (hi >= 0) {
     int lo = hi;
@@ -437,14 +442,14 @@ double oracle_max_abs_eig(const double *A, int n) {
       lo--;
     }
     if (lo == hi) {
-      best = fmax(best, fabs(H[hi * n + hi]));
+      best = fmax(best, fabs(H[hi * n + hi] + ex));
       hi -= 1;
       iter = 0;
       continue;
     }
     if (lo == hi - 1) {
-      double a = H[(hi - 1) * n + hi - 1], b = H[(hi - 1) * n + hi];
-      double c = H[hi * n + hi - 1], dd = H[hi * n + hi];
+      double a = H[(hi - 1) * n + hi - 1] + ex, b = H[(hi - 1) * n + hi];
+      double c = H[hi * n + hi - 1], dd = H[hi * n + hi] + ex;
       double p = 0.5 * (a - dd);
       double bc = b * c;
       double disc = p * p + bc;
@@ -462,38 +467,68 @@ double oracle_max_abs_eig(const double *A, int n) {
       iter = 0;
       continue;
     }
-    if (total >= 30 * n) return NAN;
+    if (total >= 40 * n) {
+      for (int i = 0; i <= hi; i++) best = fmax(best, fabs(H[i * n + i] + ex));
+      break;
+    }
+    /* shift data from the trailing 2x2 block */
+    double sx = H[hi * n + hi], sy = H[(hi - 1) * n + hi - 1];
+    double sw = H[hi * n + hi - 1] * H[(hi - 1) * n + hi];
+    if (iter == 10) {
+      ex = ex + sx;
+      for (int i = 0; i <= hi; i++) H[i * n + i] = H[i * n + i] - sx;
+      double s = fabs(H[hi * n + hi - 1]) + fabs(H[(hi - 1) * n + hi - 2]);
+      sx = 0.75 * s;
+      sy = sx;
+      sw = -0.4375 * s * s;
+    } else if (iter == 30) {
+      double s = (sy - sx) / 2.0;
+      s = s * s + sw;
+      if (s > 0.0) {
+        s = sqrt(s);
+        if (sy < sx) s = -s;
+        s = s + (sy - sx) / 2.0;
+        s = sx - sw / s;
+        ex = ex + s;
+        for (int i = 0; i <= hi; i++) H[i * n + i] = H[i * n + i] - s;
+        sx = 0.964;
+        sy = 0.964;
+        sw = 0.964;
+      }
+    }
     iter++;
     total++;
-    double s, t;
-    if (iter == 10 || iter == 20) {
-      double ww = fabs(H[hi * n + hi - 1]) + fabs(H[(hi - 1) * n + hi - 2]);
-      s = 1.5 * ww;
-      t = ww * ww;
-    } else {
-      s = H[(hi - 1) * n + hi - 1] + H[hi * n + hi];
-      t = H[(hi - 1) * n + hi - 1] * H[hi * n + hi] - H[(hi - 1) * n + hi] * H[hi * n + hi - 1];
-    }
-    double h00 = H[lo * n + lo], h01 = H[lo * n + lo + 1], h10 = H[(lo + 1) * n + lo];
-    double h11 = H[(lo + 1) * n + lo + 1], h21 = H[(lo + 2) * n + lo + 1];
-    x[0] = h00 * h00 + h01 * h10 - s * h00 + t;
-    x[1] = h10 * (h00 + h11 - s);
-    x[2] = h10 * h21;
+    /* first column of (H - s1)(H - s2), scaled by 1/h(lo+1,lo) */
+    double h00 = H[lo * n + lo];
+    double r = sx - h00, q = sy - h00;
+    x[0] = (r * q - sw) / H[(lo + 1) * n + lo] + H[lo * n + lo + 1];
+    x[1] = H[(lo + 1) * n + lo + 1] - h00 - r - q;
+    x[2] = H[(lo + 2) * n + lo + 1];
     for (int k = lo; k <= hi - 2; k++) {
+      if (k > lo) {
+        x[0] = H[k * n + k - 1];
+        x[1] = H[(k + 1) * n + k - 1];
+        x[2] = H[(k + 2) * n + k - 1];
+      }
       double beta = house(x, 3, v);
       if (beta != 0.0) {
-        int q = (k - 1 > lo) ? k - 1 : lo;
-        house_left(H, n, v, 3, beta, k, q, hi);
-        int r = (k + 3 < hi) ? k + 3 : hi;
-        house_right(H, n, v, 3, beta, lo, r, k);
+        if (k > lo) {
+          H[k * n + k - 1] = sqrt(x[0] * x[0] + (x[1] * x[1] + x[2] * x[2]));
+          H[(k + 1) * n + k - 1] = 0.0;
+          H[(k + 2) * n + k - 1] = 0.0;
+        }
+        house_left(H, n, v, 3, beta, k, k, hi);
+        int rr = (k + 3 < hi) ? k + 3 : hi;
+        house_right(H, n, v, 3, beta, lo, rr, k);
       }
-      x[0] = H[(k + 1) * n + k];
-      x[1] = H[(k + 2) * n + k];
-      if (k < hi - 2) x[2] = H[(k + 3) * n + k];
     }
+    x[0] = H[(hi - 1) * n + hi - 2];
+    x[1] = H[hi * n + hi - 2];
     double beta = house(x, 2, v);
     if (beta != 0.0) {
-      house_left(H, n, v, 2, beta, hi - 1, hi - 2, hi);
+      H[(hi - 1) * n + hi - 2] = sqrt(x[0] * x[0] + x[1] * x[1]);
+      H[hi * n + hi - 2] = 0.0;
+      house_left(H, n, v, 2, beta, hi - 1, hi - 1, hi);
       house_right(H, n, v, 2, beta, lo, hi, hi - 1);
     }
   }

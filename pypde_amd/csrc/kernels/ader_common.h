@@ -166,11 +166,14 @@ DEV double hget(const double* H, int r, int c) {
   return out;
 }
 
-// max |lambda| of A (V x V, row-major; destroyed): scale by the largest |entry|
-// (as Eigen's RealSchur, RealSchur.h:258-276), Householder Hessenberg reduction,
-// then Francis double-shift QR steps (Golub & Van Loan Alg. 7.5.1-7.5.2) with
-// deflation and exceptional shifts after 10 and 20 steps without deflation.
-// All array indices are compile-time constants; run-time bounds only predicate.
+// max |lambda| of A (V x V, row-major; destroyed), as the reference takes it from
+// Eigen's EigenSolver (system.cpp:28-43): scale by the largest |entry| (Eigen's
+// RealSchur, RealSchur.h:258-276), Householder Hessenberg reduction, then
+// Francis double-shift QR steps (Golub & Van Loan Alg. 7.5.1-7.5.2, EISPACK
+// hqr structure) with deflation on eps-small subdiagonals and origin-moving
+// exceptional shifts after 10 and 30 steps without deflation; after 40 V steps
+// the remaining diagonal stands in for unresolved eigenvalues.  All array
+// indices are compile-time constants; run-time bounds only predicate.
 DEV double max_abs_eig(double* H) {
   double scale = 0.0;
 #pragma unroll
@@ -193,7 +196,7 @@ DEV double max_abs_eig(double* H) {
   double anorm = 0.0;
 #pragma unroll
   for (int i = 0; i < V * V; i++) anorm = anorm + dabs(H[i]);
-  double best = 0.0;
+  double best = 0.0, ex = 0.0;
   int hi = V - 1, iter = 0, total = 0;
   while (hi >= 0) {
     int lo = 0;
@@ -211,14 +214,14 @@ DEV double max_abs_eig(double* H) {
       }
     }
     if (lo == hi) {
-      best = dmax(best, dabs(hget(H, hi, hi)));
+      best = dmax(best, dabs(hget(H, hi, hi) + ex));
       hi -= 1;
       iter = 0;
       continue;
     }
     if (lo == hi - 1) {
-      double a = hget(H, hi - 1, hi - 1), b = hget(H, hi - 1, hi);
-      double c = hget(H, hi, hi - 1), dd = hget(H, hi, hi);
+      double a = hget(H, hi - 1, hi - 1) + ex, b = hget(H, hi - 1, hi);
+      double c = hget(H, hi, hi - 1), dd = hget(H, hi, hi) + ex;
       double p = 0.5 * (a - dd);
       double bc = b * c;
       double disc = p * p + bc;
@@ -237,48 +240,83 @@ DEV double max_abs_eig(double* H) {
       continue;
     }
     if constexpr (V >= 3) {
-      if (total >= 30 * V) return __builtin_nan("");
+      if (total >= 40 * V) {
+#pragma unroll
+        for (int i = 0; i < V; i++)
+          if (i <= hi) best = dmax(best, dabs(H[i * V + i] + ex));
+        break;
+      }
+      double sx = hget(H, hi, hi), sy = hget(H, hi - 1, hi - 1);
+      double sw = hget(H, hi, hi - 1) * hget(H, hi - 1, hi);
+      if (iter == 10) {
+        ex = ex + sx;
+#pragma unroll
+        for (int i = 0; i < V; i++)
+          if (i <= hi) H[i * V + i] = H[i * V + i] - sx;
+        double s = dabs(hget(H, hi, hi - 1)) + dabs(hget(H, hi - 1, hi - 2));
+        sx = 0.75 * s;
+        sy = sx;
+        sw = -0.4375 * s * s;
+      } else if (iter == 30) {
+        double s = (sy - sx) / 2.0;
+        s = s * s + sw;
+        if (s > 0.0) {
+          s = dsqrt(s);
+          if (sy < sx) s = -s;
+          s = s + (sy - sx) / 2.0;
+          s = sx - sw / s;
+          ex = ex + s;
+#pragma unroll
+          for (int i = 0; i < V; i++)
+            if (i <= hi) H[i * V + i] = H[i * V + i] - s;
+          sx = 0.964;
+          sy = 0.964;
+          sw = 0.964;
+        }
+      }
       iter++;
       total++;
-      double s, t;
-      if (iter == 10 || iter == 20) {
-        double ww = dabs(hget(H, hi, hi - 1)) + dabs(hget(H, hi - 1, hi - 2));
-        s = 1.5 * ww;
-        t = ww * ww;
-      } else {
-        double h1 = hget(H, hi - 1, hi - 1), h2 = hget(H, hi, hi);
-        s = h1 + h2;
-        t = h1 * h2 - hget(H, hi - 1, hi) * hget(H, hi, hi - 1);
-      }
-      double h00 = hget(H, lo, lo), h01 = hget(H, lo, lo + 1), h10 = hget(H, lo + 1, lo);
-      double h11 = hget(H, lo + 1, lo + 1), h21 = hget(H, lo + 2, lo + 1);
+      double h00 = hget(H, lo, lo);
+      double r = sx - h00, q = sy - h00;
       double x[3], v[3];
-      x[0] = h00 * h00 + h01 * h10 - s * h00 + t;
-      x[1] = h10 * (h00 + h11 - s);
-      x[2] = h10 * h21;
+      x[0] = (r * q - sw) / hget(H, lo + 1, lo) + hget(H, lo, lo + 1);
+      x[1] = hget(H, lo + 1, lo + 1) - h00 - r - q;
+      x[2] = hget(H, lo + 2, lo + 1);
       static_for<0, (V >= 3 ? V - 2 : 0)>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
         if (k >= lo && k <= hi - 2) {
+          if constexpr (k >= 1) {
+            if (k > lo) {
+              x[0] = H[k * V + k - 1];
+              x[1] = H[(k + 1) * V + k - 1];
+              x[2] = H[(k + 2) * V + k - 1];
+            }
+          }
           double beta = house<3>(x, v);
           if (beta != 0.0) {
-            int q = (k - 1 > lo) ? k - 1 : lo;
-            house_left<3, k>(H, v, beta, q, hi);
-            int r = (k + 3 < hi) ? k + 3 : hi;
-            house_right<3, k>(H, v, beta, lo, r);
-          }
-          x[0] = H[(k + 1) * V + k];
-          x[1] = H[(k + 2) * V + k];
-          if constexpr (k + 3 < V) {
-            if (k < hi - 2) x[2] = H[(k + 3) * V + k];
+            if constexpr (k >= 1) {
+              if (k > lo) {
+                H[k * V + k - 1] = dsqrt(x[0] * x[0] + (x[1] * x[1] + x[2] * x[2]));
+                H[(k + 1) * V + k - 1] = 0.0;
+                H[(k + 2) * V + k - 1] = 0.0;
+              }
+            }
+            house_left<3, k>(H, v, beta, k, hi);
+            int rr = (k + 3 < hi) ? k + 3 : hi;
+            house_right<3, k>(H, v, beta, lo, rr);
           }
         }
       });
       static_for<2, (V >= 3 ? V : 2)>([&](auto hc) {
         constexpr int hh = decltype(hc)::value;  // hi in 2..V-1
         if (hh == hi) {
+          x[0] = H[(hh - 1) * V + hh - 2];
+          x[1] = H[hh * V + hh - 2];
           double beta = house<2>(x, v);
           if (beta != 0.0) {
-            house_left<2, hh - 1>(H, v, beta, hh - 2, hh);
+            H[(hh - 1) * V + hh - 2] = dsqrt(x[0] * x[0] + x[1] * x[1]);
+            H[hh * V + hh - 2] = 0.0;
+            house_left<2, hh - 1>(H, v, beta, hh - 1, hh);
             house_right<2, hh - 1>(H, v, beta, lo, hh);
           }
         }

@@ -79,6 +79,11 @@ def oracle():
                                           [PD, C.c_double, PI, C.c_int, PD, C.c_double,
                                            PI, C.c_int, C.c_int, C.c_int, C.c_int,
                                            C.c_int, C.c_int, PD, C.c_void_p, C.c_void_p])
+        lib.oracle_system_matrix.argtypes = [C.c_void_p, C.c_void_p, PD, PD, C.c_int, C.c_int,
+                                             C.c_int, PD]
+        lib.oracle_system_matrix.restype = None
+        lib.oracle_boundaries.argtypes = [PD, PD, PI, C.c_int, PI, PD, C.c_int, C.c_int]
+        lib.oracle_weno.argtypes = [PD, PD, PI, C.c_int, C.c_int, C.c_int]
         lib.oracle_dg_obj.argtypes = [C.c_void_p] * 3 + [PD, PD, PD, C.c_int, C.c_int,
                                                         C.c_int, C.c_double, C.c_int, PD]
         _cache["oracle"] = lib

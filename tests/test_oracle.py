@@ -125,6 +125,17 @@ def test_max_abs_eig_numpy():
             got = cc.oracle().oracle_max_abs_eig(cc.dp(A), V)
             want = np.max(np.abs(np.linalg.eigvals(A)))
             assert abs(got - want) <= 1e-12 * want, (V, k)
+    # clustered spectra as finite-difference Jacobians of multi-wave systems
+    # produce (a near-triple eigenvalue split by rounding-level noise)
+    for k in range(300):
+        V = 3 + k % 6
+        lam = rng.uniform(-2, 2)
+        D = np.diag(np.r_[np.full(V - 2, lam), rng.uniform(-4, 4, 2)])
+        P = rng.standard_normal((V, V))
+        A = np.ascontiguousarray(P @ D @ np.linalg.inv(P) + 1e-12 * rng.standard_normal((V, V)))
+        got = cc.oracle().oracle_max_abs_eig(cc.dp(A), V)
+        want = np.max(np.abs(np.linalg.eigvals(A)))
+        assert abs(got - want) <= 1e-9 * want, (V, k)
     # special cases: zero matrix, Jordan block, pure rotation (complex pair)
     assert cc.oracle().oracle_max_abs_eig(cc.dp(np.zeros((3, 3))), 3) == 0.0
     J = np.ascontiguousarray(np.eye(4) * 2 + np.eye(4, k=1))
