@@ -963,9 +963,10 @@ static void lu_solve_neg(int V, double *P, int n, double h[MAXN][MAXV]) {
  * through (Tw - A)^-1 = -adj(xI - Tw)(A) p(A)^-1 with A = dt JS (the entries of
  * the polynomial matrix commute).  The fixed point does not depend on J.
  *
- * Learned initial guess (guess != NULL): if the cell residual max|obj| after
- * the first step is larger than at the guess itself, the guess is abandoned
- * and the iteration restarts from the plain guess (w broadcast in time).
+ * Learned initial guess (guess != NULL): while the cell is still on its
+ * guess, a cell residual max|obj| that is not finite or larger than at the
+ * previous iterate abandons the guess; the iteration restarts from the plain
+ * guess (w broadcast in time), counting the evaluations already spent.
  * Returns the number of residual evaluations; *fellback reports a restart. */
 static void init_guess(const Tab *T, const double *w, double *q);
 
@@ -989,20 +990,20 @@ static int newton_cell(const Ctx *C, const double *w, double *q, double *rhs,
             acc = fma(T->dgmat[t][s] * wp, q[((long)s * Nd + idx) * V + v], acc);
           long e = ((long)t * Nd + idx) * V + v;
           rhs[e] = rhs[e] - acc;
-          rmax = fmax(rmax, fabs(rhs[e]));
+          double a = fabs(rhs[e]);
+          rmax = (a > rmax || a != a) ? a : rmax; /* keeps a NaN */
         }
     }
     if (learned) {
-      if (it == 0) {
-        r0 = rmax;
-      } else {
+      /* still on the learned guess: a residual that is not finite, or larger
+       * than at the previous iterate, abandons the guess */
+      if (!(rmax <= (it > 0 ? r0 : DBL_MAX))) {
+        init_guess(T, w, q);
+        *fellback = 1;
         learned = 0;
-        if (!(rmax <= r0)) {
-          init_guess(T, w, q);
-          *fellback = 1;
-          continue;
-        }
+        continue;
       }
+      r0 = rmax;
     }
     int more = 0;
     for (int idx = 0; idx < Nd; idx++) {

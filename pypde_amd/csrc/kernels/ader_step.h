@@ -381,32 +381,36 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
 #pragma unroll
           for (int s = 1; s < N; s++) acc = dfma(T_DGMAT[t][s] * c.WP, q[s][v], acc);
           rhs[t][v] = rhs[t][v] - acc;
-          rmax = dmax(rmax, dabs(rhs[t][v]));
+          double a = dabs(rhs[t][v]);
+          rmax = (a > rmax || a != a) ? a : rmax;  // keeps a NaN
         }
     }
 #if PDE_GUESS
-    if (learned) {  // wave-uniform: every cell of the launch started from a guess
+    // while a cell is still on its learned guess, a residual that is not finite
+    // or larger than at the previous iterate sends it back to the plain guess
+    if (__ballot(active && learned) != 0ull) {
       __syncthreads();
       sm.red[lane] = rmax;
       __syncthreads();
       double cm = 0.0;
 #pragma unroll
-      for (int i = 0; i < ND; i++) cm = dmax(cm, sm.red[(c.slot < CPW ? c.slot : 0) * ND + i]);
-      if (active) {
-        if (it == 0) {
-          r0 = cm;
+      for (int i = 0; i < ND; i++) {
+        double e = sm.red[(c.slot < CPW ? c.slot : 0) * ND + i];
+        cm = (e > cm || e != e) ? e : cm;  // max that keeps a NaN
+      }
+      if (active && learned) {
+        if (!(cm <= (it > 0 ? r0 : 1.7976931348623157e308))) {
+#pragma unroll
+          for (int t = 0; t < N; t++)
+#pragma unroll
+            for (int v = 0; v < V; v++) q[t][v] = w[v];
+          fellback = 1;
+          learned = 0;
+          skip = true;
         } else {
-          if (!(cm <= r0)) {
-#pragma unroll
-            for (int t = 0; t < N; t++)
-#pragma unroll
-              for (int v = 0; v < V; v++) q[t][v] = w[v];
-            fellback = 1;
-            skip = true;
-          }
+          r0 = cm;
         }
       }
-      if (it >= 1) learned = 0;
     }
 #endif
     if (active && !skip) {

@@ -20,8 +20,12 @@ TOL = 1e-10   # max relative L-inf, per variable
 
 
 def relerr(a, b):
+    """max over variables of |a - b|_inf / |b_v|_inf; a variable whose scale is
+    below 1e-8 of the largest one (e.g. transverse momentum of a quasi-1D
+    flow, ~1e-10) is measured against that floor instead."""
     V = b.shape[-1]
-    sc = np.maximum(np.max(np.abs(b.reshape(-1, V)), axis=0), 1e-300)
+    sc = np.max(np.abs(b.reshape(-1, V)), axis=0)
+    sc = np.maximum(sc, 1e-8 * sc.max() + 1e-300)
     return float(np.max(np.abs((a - b).reshape(-1, V)) / sc))
 
 
@@ -159,20 +163,45 @@ def test_nonconservative_b_matches_oracle(quirk):
     assert relerr(ug, uo) <= TOL
 
 
+def first_step_predictor(u0, L, refl, N=3, V=5):
+    """Oracle stages of the first step: ghosts, WENO, dt, converged stiff
+    predictor; returns (qh, nw)."""
+    s = cc.system("reactive2d")
+    O = cc.oracle()
+    nX = np.array(u0.shape[:2], dtype=np.int32)
+    nb = nX + 2 * N
+    dX = np.array([L[0] / nX[0], L[1] / nX[1]])
+    ub = np.zeros(int(np.prod(nb)) * V)
+    r = np.ascontiguousarray(refl.ravel())
+    O.oracle_boundaries(cc.dp(ub), cc.dp(np.ascontiguousarray(u0)), cc.ip(nX), 2,
+                        cc.ip(np.array([2, 2], dtype=np.int32)), cc.dp(r), N, V)
+    nw = int(np.prod(nX + 2))
+    wh = np.zeros(nw * N * N * V)
+    O.oracle_weno(cc.dp(wh), cc.dp(ub), cc.ip(np.ascontiguousarray(nb, dtype=np.int32)), 2, N, V)
+    dt = O.oracle_timestep(s.F, None, cc.dp(wh), nw, cc.dp(dX), 2, N, V, 0.9, 1.0, 0.0, 0)
+    qh = np.zeros(nw * N ** 3 * V)
+    opts = cc.OracleOpts()
+    opts.newton_tol = 1e-12
+    opts.newton_maxit = 100
+    O.oracle_predictor(s.F, None, s.S, s.M, cc.dp(wh), nw, cc.dp(dX), 2, N, V, dt, 1,
+                       C.byref(opts), cc.dp(qh), None)
+    return qh.reshape(nw, N ** 3, V), nw
+
+
 @pytest.mark.gpu
 def test_learned_guess_path_matches_oracle():
-    """Predictor started from supplied guesses (the learned-guess hook of
-    dg.cpp:216): GPU and oracle take the same iterations, fall back on the
-    same cells, and the converged state does not depend on the guess."""
+    """Predictor started from supplied guesses (the initial-guess hook of
+    dg.cpp:216): near-solution guesses on most cells, garbage on every third.
+    GPU and oracle take the same iterations and fall back on the same cells,
+    and the state agrees with the plain-guess run to the Newton tolerance."""
     u0 = reactive_small(48, 12)
     L = [48 / 1024, 12 / 1024]
     refl = systems.REACTIVE_REFLECT
-    nw = 50 * 14
-    NT, V = 27, 5
+    qh, nw = first_step_predictor(u0, L, refl)
     rng = np.random.default_rng(9)
-    # garbage guesses: the residual grows after the first step on many cells,
-    # which must trigger the fallback to the plain guess identically
-    guess = rng.uniform(-1, 1, (nw, NT, V)) * 1e3
+    guess = qh * (1 + 1e-4 * rng.uniform(-1, 1, qh.shape))
+    guess[::3] = rng.uniform(-1, 1, guess[::3].shape) * 1e3
+    V = 5
     opts = cc.OracleOpts()
     opts.newton_tol = 1e-12
     opts.newton_maxit = 100
@@ -191,7 +220,7 @@ def test_learned_guess_path_matches_oracle():
     rc = cc.oracle().oracle_pde_solver(s.F, None, s.S, s.M, 1, 0, 1, cc.dp(uo), 1.0, cc.ip(nX), 2,
                                        cc.dp(dX), 0.9, cc.ip(np.array([2, 2], dtype=np.int32)), 1,
                                        0, 3, V, 1, 0, cc.dp(ret), C.byref(opts), C.byref(sto))
-    assert rc == 0 and sto.guess_fallbacks > 0
+    assert rc == 0 and sto.guess_fallbacks >= nw // 3
     sol = Solver(u0, L, systems.source("reactive2d"), order=3, boundaryTypes="reflective",
                  stiff=True, reflect=refl, guess=guess)
     sg = sol.run(1.0, max_steps=1)
@@ -200,7 +229,6 @@ def test_learned_guess_path_matches_oracle():
     assert sg["guess_fallbacks"] == sto.guess_fallbacks
     assert sg["pred_iters"] == sto.pred_iters
     assert relerr(ug, uo) <= TOL
-    # same converged answer as the plain guess, to the Newton tolerance scale
-    up, _, _, _ = oracle_run("reactive2d", u0, 1.0, L, 2, 3, True, max_steps=1,
-                             reflect=refl.ravel())
+    up, _, stp, _ = oracle_run("reactive2d", u0, 1.0, L, 2, 3, True, max_steps=1,
+                               reflect=refl.ravel())
     assert relerr(uo, up) <= 1e-9
