@@ -231,4 +231,5 @@ def test_learned_guess_path_matches_oracle():
     assert relerr(ug, uo) <= TOL
     up, _, stp, _ = oracle_run("reactive2d", u0, 1.0, L, 2, 3, True, max_steps=1,
                                reflect=refl.ravel())
-    assert relerr(uo, up) <= 1e-9
+    # both runs stop at |dq| <= 1e-12 (1 + |q|); the states agree to that scale
+    assert np.max(np.abs(uo - up)) <= 1e-9 * np.max(np.abs(up))
