@@ -92,6 +92,42 @@ void *pypde_device_state(pypde_ctx *c);
 void *pypde_stream(pypde_ctx *c);
 void pypde_destroy(pypde_ctx *c);
 
+/* Learned Newton initial guess (no counterpart in the reference; its hook is
+ * DGSolver::initial_guess, dg.cpp:12-20,216).  A 3-layer SiLU MLP, weights as
+ * device fp32 arrays (Wt: out x in, row-major), inputs (wh, dt) normalised by
+ * in_mean / in_rstd, outputs (qh - w) / dt de-normalised by out_std / out_mean.
+ * Once attached, every step infers guesses on the matrix cores (BF16, FP32
+ * accumulate) before the stiff predictor; Newton still runs to newton_tol and
+ * abandons a guess whose residual grows.  NULL detaches. */
+typedef struct {
+  const float *W1t, *b1, *W2t, *b2, *W3t, *b3;
+  const float *in_mean, *in_rstd, *out_mean, *out_std;
+} pypde_mlp;
+int pypde_set_guess_model(pypde_ctx *c, const pypde_mlp *m);
+/* One step that also writes training pairs as fp32 device columns col0..:
+ * X (V N^ndim + 1 rows: wh, dt) and Y (V N^(ndim+1) rows: (qh - w) / dt), leading
+ * dimension ldx. */
+int pypde_harvest_step(pypde_ctx *c, float *X, float *Y, long ldx, long col0, pypde_stats *stats);
+/* Host copy of the last inferred guesses, nwc x N^(ndim+1) x V. */
+int pypde_last_guess(pypde_ctx *c, double *out);
+int pypde_geometry(pypde_ctx *c, long *nwc, int *n_in, int *n_out);
+
+/* Training kernels (device fp32 pointers; stream = HIP stream or NULL):
+ * C = alpha op(A) op(B) + beta C on the matrix cores (BF16 operands, FP32
+ * accumulate); Z += b, H = SiLU(Z) (act 1) or Z (act 0); dZ = dH SiLU'(Z);
+ * db = row sums; MSE loss and gradient; Adam step. */
+int pypde_mlp_gemm(const float *A, int lda, int transA, const float *B, int ldb, int transB,
+                   float *C, int ldc, int M, int N, int K, float alpha, float beta, void *stream);
+int pypde_mlp_bias_act(float *Z, const float *b, float *H, int rows, int cols, int act,
+                       void *stream);
+int pypde_mlp_act_grad(float *dZ, const float *dH, const float *Z, long n, void *stream);
+int pypde_mlp_rowsum(const float *dZ, float *db, int rows, int cols, void *stream);
+int pypde_mlp_mse(const float *Y, const float *T, float *dY, float *loss, int rows, int cols,
+                  void *stream);
+int pypde_mlp_adam(float *w, const float *g, float *m, float *v, long n, float lr, float b1,
+                   float b2, float eps, int step, void *stream);
+int pypde_mlp_compile(void);
+
 /* Compile (or fetch from the code-object cache) without touching a GPU. */
 int pypde_hip_compile(const char *src, bool useF, bool useB, bool useS, int ndim, int N, int V,
                       bool STIFF, const pypde_options *opts);

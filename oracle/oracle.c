@@ -398,6 +398,22 @@ static void house_right(double *H, int n, const double *v, int m, double beta,
   }
 }
 
+/* True when no eigenvalue of the undeflated leading block H[0..hi][0..hi]
+ * (shifted back by ex) can exceed `best` in modulus: |lambda| <= the block's
+ * infinity norm + |ex|.  Only the largest modulus is wanted, so the iteration
+ * stops there with the same answer (typically after the two acoustic
+ * eigenvalues, leaving the cluster of the advected waves unresolved). */
+static int rest_bounded(const double *H, int n, int hi, double ex, double best) {
+  if (hi < 0) return 0;
+  double bound = 0.0;
+  for (int i = 0; i <= hi; i++) {
+    double rs = 0.0;
+    for (int j = 0; j <= hi; j++) rs = rs + fabs(H[i * n + j]);
+    bound = fmax(bound, rs);
+  }
+  return bound + fabs(ex) <= best;
+}
+
 /* max |lambda| of a real n x n matrix, the quantity the reference takes from
  * Eigen's EigenSolver (system.cpp:28-43).  Method (EISPACK hqr structure,
  * eigenvalues only): scale by the largest |entry| (as Eigen's RealSchur,
@@ -445,6 +461,7 @@ double oracle_max_abs_eig(const double *A, int n) {
       best = fmax(best, fabs(H[hi * n + hi] + ex));
       hi -= 1;
       iter = 0;
+      if (rest_bounded(H, n, hi, ex, best)) break;
       continue;
     }
     if (lo == hi - 1) {
@@ -465,6 +482,7 @@ double oracle_max_abs_eig(const double *A, int n) {
       }
       hi -= 2;
       iter = 0;
+      if (rest_bounded(H, n, hi, ex, best)) break;
       continue;
     }
     if (total >= 40 * n) {

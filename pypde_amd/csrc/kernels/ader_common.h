@@ -166,6 +166,25 @@ DEV double hget(const double* H, int r, int c) {
   return out;
 }
 
+// No eigenvalue of the undeflated leading block H[0..hi][0..hi] (shifted back
+// by ex) can exceed `best` in modulus when its infinity norm + |ex| <= best:
+// the largest modulus is then final.
+DEV bool rest_bounded(const double* H, int hi, double ex, double best) {
+  if (hi < 0) return false;
+  double bound = 0.0;
+#pragma unroll
+  for (int i = 0; i < V; i++) {
+    if (i <= hi) {
+      double rs = 0.0;
+#pragma unroll
+      for (int j = 0; j < V; j++)
+        if (j <= hi) rs = rs + dabs(H[i * V + j]);
+      bound = dmax(bound, rs);
+    }
+  }
+  return bound + dabs(ex) <= best;
+}
+
 // max |lambda| of A (V x V, row-major; destroyed), as the reference takes it from
 // Eigen's EigenSolver (system.cpp:28-43): scale by the largest |entry| (Eigen's
 // RealSchur, RealSchur.h:258-276), Householder Hessenberg reduction, then
@@ -217,6 +236,7 @@ DEV double max_abs_eig(double* H) {
       best = dmax(best, dabs(hget(H, hi, hi) + ex));
       hi -= 1;
       iter = 0;
+      if (rest_bounded(H, hi, ex, best)) break;
       continue;
     }
     if (lo == hi - 1) {
@@ -237,6 +257,7 @@ DEV double max_abs_eig(double* H) {
       }
       hi -= 2;
       iter = 0;
+      if (rest_bounded(H, hi, ex, best)) break;
       continue;
     }
     if constexpr (V >= 3) {

@@ -759,3 +759,143 @@ extern "C" __global__ void k_soa_to_aos(const double* __restrict__ s, double* __
   int v = (int)(i % V);
   a[i] = s[(long)v * ncell + c];
 }
+
+// ------------------------------------------------ learned Newton guess (MLP)
+// Training pairs: features x = (wh of the cell (N^D * V values), dt), targets
+// y = (qh - w broadcast in time) / dt (N^(D+1) * V values), written as fp32
+// columns col0 + cell of the caller's feature-major arrays.
+constexpr int MLP_IN = ND * V + 1;
+constexpr int MLP_OUT = NT * V;
+
+extern "C" __global__ void __launch_bounds__(256)
+k_harvest(const double* __restrict__ wh, const double* __restrict__ qh, const StepState* st,
+          float* __restrict__ X, float* __restrict__ Y, long ldx, long col0, long nwc) {
+  const long c = (long)blockIdx.x * 256 + threadIdx.x;
+  if (c >= nwc) return;
+  const double dt = st->dt;
+  for (int k = 0; k < ND * V; k++) X[(long)k * ldx + col0 + c] = (float)wh[(long)k * nwc + c];
+  X[(long)(ND * V) * ldx + col0 + c] = (float)dt;
+  for (int o = 0; o < MLP_OUT; o++) {
+    const int p = o % (ND * V);  // (node, v) of output o = (t, node, v)
+    Y[(long)o * ldx + col0 + c] = (float)((qh[(long)o * nwc + c] - wh[(long)p * nwc + c]) / dt);
+  }
+}
+
+#if PDE_GUESS
+#ifndef MLP_HID
+#define MLP_HID 128
+#endif
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+struct MLPW {
+  const float *W1t, *b1, *W2t, *b2, *W3t, *b3;  // Wt: out x in, row-major
+  const float *in_mean, *in_rstd, *out_mean, *out_std;
+};
+
+DEV short to_bf16(float x) {
+  unsigned int b = __float_as_uint(x);
+  b += 0x7FFFu + ((b >> 16) & 1u);
+  return (short)(b >> 16);
+}
+DEV float silu(float z) { return z / (1.0f + __expf(-z)); }
+
+constexpr int KB_IN = (MLP_IN + 7) / 8;          // k-blocks of the input layer
+constexpr int RB_HID = MLP_HID / 32;             // 32-row blocks of a hidden layer
+constexpr int RB_OUT = (MLP_OUT + 31) / 32;
+static_assert(MLP_HID % 32 == 0, "hidden width must be a multiple of 32");
+
+// A fragment of Wt (rows rb*32.., k-block kb) from fp32 memory
+DEV bf16x4 wfrag(const float* Wt, int ld, int rows, int cols, int rb, int kb, int lane) {
+  bf16x4 a;
+  const int r = rb * 32 + (lane & 31);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int k = kb * 8 + 4 * (lane >> 5) + i;
+    a[i] = to_bf16((r < rows && k < cols) ? Wt[(long)r * ld + k] : 0.0f);
+  }
+  return a;
+}
+
+// One wave infers 32 cells (lane % 32 = cell); the transposed form
+// H^T = W^T X^T makes each layer's accumulators the next layer's B operand.
+extern "C" __global__ void __launch_bounds__(256)
+k_mlp_guess(const double* __restrict__ wh, const StepState* st, MLPW m,
+            double* __restrict__ guess, long nwc) {
+  if (st->done) return;
+  const int lane = threadIdx.x & 63;
+  const long cell = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 32 + (lane & 31);
+  const bool ok = cell < nwc;
+  const double dt = st->dt;
+  // input fragments
+  bf16x4 xb[KB_IN];
+#pragma unroll
+  for (int kb = 0; kb < KB_IN; kb++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int k = kb * 8 + 4 * (lane >> 5) + i;
+      float x = 0.0f;
+      if (ok && k < MLP_IN) {
+        const double raw = (k < ND * V) ? wh[(long)k * nwc + cell] : dt;
+        x = ((float)raw - m.in_mean[k]) * m.in_rstd[k];
+      }
+      xb[kb][i] = to_bf16(x);
+    }
+  // layer 1
+  f32x16 h[RB_HID];
+#pragma unroll
+  for (int rb = 0; rb < RB_HID; rb++) {
+#pragma unroll
+    for (int j = 0; j < 16; j++) h[rb][j] = 0.0f;
+#pragma unroll
+    for (int kb = 0; kb < KB_IN; kb++)
+      h[rb] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
+          wfrag(m.W1t, MLP_IN, MLP_HID, MLP_IN, rb, kb, lane), xb[kb], h[rb], 0, 0, 0);
+  }
+  // layers 2 and 3 consume SiLU(h + b) as B fragments: k-block kb = rows
+  // 8 kb .. 8 kb + 7 = accumulator block kb / 4, group kb % 4
+  auto hidden_frag = [&](const f32x16* hh, const float* b, int kb) {
+    bf16x4 f;
+    const int rb = kb / 4, g = kb % 4;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int row = rb * 32 + 8 * g + 4 * (lane >> 5) + r;
+      f[r] = to_bf16(silu(hh[rb][4 * g + r] + b[row]));
+    }
+    return f;
+  };
+  f32x16 h2[RB_HID];
+#pragma unroll
+  for (int rb = 0; rb < RB_HID; rb++) {
+#pragma unroll
+    for (int j = 0; j < 16; j++) h2[rb][j] = 0.0f;
+#pragma unroll
+    for (int kb = 0; kb < MLP_HID / 8; kb++)
+      h2[rb] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
+          wfrag(m.W2t, MLP_HID, MLP_HID, MLP_HID, rb, kb, lane), hidden_frag(h, m.b1, kb), h2[rb],
+          0, 0, 0);
+  }
+#pragma unroll
+  for (int rb = 0; rb < RB_OUT; rb++) {
+    f32x16 y;
+#pragma unroll
+    for (int j = 0; j < 16; j++) y[j] = 0.0f;
+#pragma unroll
+    for (int kb = 0; kb < MLP_HID / 8; kb++)
+      y = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
+          wfrag(m.W3t, MLP_HID, MLP_OUT, MLP_HID, rb, kb, lane), hidden_frag(h2, m.b2, kb), y, 0,
+          0, 0);
+    if (ok) {
+#pragma unroll
+      for (int j = 0; j < 16; j++) {
+        const int o = rb * 32 + 8 * (j / 4) + 4 * (lane >> 5) + (j % 4);
+        if (o < MLP_OUT) {
+          const float val = (y[j] + m.b3[o]) * m.out_std[o] + m.out_mean[o];
+          const int p = o % (ND * V);
+          guess[(long)o * nwc + cell] = wh[(long)p * nwc + cell] + dt * (double)val;
+        }
+      }
+    }
+  }
+}
+#endif

@@ -227,10 +227,16 @@ int check_spec(const Spec& s, bool secondOrder, int FLUX) {
   return 0;
 }
 
-enum { K_WENO, K_DT, K_DTFIN, K_PRED, K_FLUX0, K_FLUX1, K_UPDATE, K_ADVANCE, K_A2S, K_S2A, K_COUNT };
-const char* kernel_names[K_COUNT] = {"k_weno",   "k_dt",   "k_dt_finalize", "k_predictor",
-                                     "k_flux0",  "k_flux1", "k_update",     "k_advance",
-                                     "k_aos_to_soa", "k_soa_to_aos"};
+enum { K_WENO, K_DT, K_DTFIN, K_PRED, K_FLUX0, K_FLUX1, K_UPDATE, K_ADVANCE, K_A2S, K_S2A,
+       K_HARVEST, K_MLPGUESS, K_COUNT };
+const char* kernel_names[K_COUNT] = {"k_weno",   "k_dt",    "k_dt_finalize", "k_predictor",
+                                     "k_flux0",  "k_flux1", "k_update",      "k_advance",
+                                     "k_aos_to_soa", "k_soa_to_aos", "k_harvest", "k_mlp_guess"};
+
+struct MLPW {
+  const float *W1t, *b1, *W2t, *b2, *W3t, *b3;
+  const float *in_mean, *in_rstd, *out_mean, *out_std;
+};
 
 }  // namespace
 
@@ -251,11 +257,31 @@ struct pypde_ctx {
   int device = 0;
   std::vector<hipEvent_t> ev;
   double ms_kernel[8] = {0};
+  hipModule_t mod_guess = nullptr;  // learned-guess variant of the predictor
+  hipFunction_t fn_plain[K_COUNT] = {}, fn_guess[K_COUNT] = {};
+  bool use_model = false;
+  MLPW model;
+  double* gbuf = nullptr;   // per-step MLP guesses
+  double* qh = nullptr;     // full predictor output (harvest)
 };
 
 namespace {
 
+int load_module(pypde_ctx* c, const std::vector<char>& code, hipModule_t* mod, hipFunction_t* fn) {
+  if (hipModuleLoadData(mod, code.data()) != hipSuccess) return fail(PYPDE_ERR_HIP, "hipModuleLoadData failed");
+  for (int k = 0; k < K_COUNT; k++) {
+    fn[k] = nullptr;
+    bool optional = (k == K_FLUX1 && c->D < 2) || k == K_MLPGUESS;
+    if (hipModuleGetFunction(&fn[k], *mod, kernel_names[k]) != hipSuccess) {
+      fn[k] = nullptr;
+      if (!optional) return fail(PYPDE_ERR_HIP, "kernel %s missing", kernel_names[k]);
+    }
+  }
+  return 0;
+}
+
 int launch(pypde_ctx* c, int k, dim3 grid, dim3 block, std::vector<void*>& args) {
+  if (!c->fn[k]) return fail(PYPDE_ERR_HIP, "kernel %s not loaded", kernel_names[k]);
   hipError_t e = hipModuleLaunchKernel(c->fn[k], grid.x, grid.y, grid.z, block.x, block.y,
                                        block.z, 0, c->stream, args.data(), nullptr);
   if (e != hipSuccess)
@@ -301,10 +327,19 @@ int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
     if ((rc = launch(c, K_DTFIN, dim3(1), dim3(64), b))) return rc;
   }
   mark();
+  if (c->use_model) {
+    const double* w = c->wh;
+    const StepState* st = c->st;
+    MLPW m = c->model;
+    double* gb = c->gbuf;
+    long nwc = g.nwc;
+    std::vector<void*> a = {&w, &st, &m, &gb, &nwc};
+    if ((rc = launch(c, K_MLPGUESS, dim3((unsigned)cdiv(g.nwc, 128)), dim3(256), a))) return rc;
+  }
   {
     const double* w = c->wh;
-    const double* gs = guess_now ? c->guess : nullptr;
-    double *t = c->tr, *v = c->vol, *qh = nullptr;
+    const double* gs = c->use_model ? c->gbuf : (guess_now ? c->guess : nullptr);
+    double *t = c->tr, *v = c->vol, *qh = c->qh;
     StepState* st = c->st;
     double tol = c->opts.newton_tol;
     int maxit = c->opts.newton_maxit;
@@ -422,7 +457,10 @@ void pypde_destroy(pypde_ctx* c) {
                     c->guess, c->tmp})
     if (p) hipFree(p);
   if (c->st) hipFree(c->st);
+  if (c->gbuf) hipFree(c->gbuf);
+  if (c->qh) hipFree(c->qh);
   if (c->mod) hipModuleUnload(c->mod);
+  if (c->mod_guess) hipModuleUnload(c->mod_guess);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -509,13 +547,8 @@ pypde_ctx* pypde_create(const char* src, bool useF, bool useB, bool useS, int* n
   };
   if (hipSetDevice(c->device) != hipSuccess)
     return bail(fail(PYPDE_ERR_HIP, "hipSetDevice(%d) failed: no usable GPU", c->device));
-  if (hipModuleLoadData(&c->mod, code.data()) != hipSuccess)
-    return bail(fail(PYPDE_ERR_HIP, "hipModuleLoadData failed"));
-  for (int k = 0; k < K_COUNT; k++) {
-    if (k == K_FLUX1 && ndim < 2) continue;
-    if (hipModuleGetFunction(&c->fn[k], c->mod, kernel_names[k]) != hipSuccess)
-      return bail(fail(PYPDE_ERR_HIP, "kernel %s missing", kernel_names[k]));
-  }
+  if (load_module(c, code, &c->mod, c->fn_plain)) return bail(PYPDE_ERR_HIP);
+  memcpy(c->fn, c->fn_plain, sizeof c->fn);
   if (hipStreamCreate(&c->stream) != hipSuccess)
     return bail(fail(PYPDE_ERR_HIP, "hipStreamCreate failed"));
   size_t NTR = 2 * (size_t)ndim * N * (c->ND / N) * V;
@@ -581,6 +614,11 @@ static int run_loop(pypde_ctx* c, long max_steps, pypde_stats* stats, bool first
   bool guess_now = first_guess;
   std::vector<hipEvent_t> evs;
   while (true) {
+    // never queue past a step cap: the launch count equals the step count
+    if (max_steps > 0) {
+      long left = max_steps - (long)c->host_st.count;
+      if (left < chunk) chunk = (int)std::max(1L, left);
+    }
     for (int i = 0; i < chunk; i++) {
       std::vector<hipEvent_t>* ep = c->opts.profile ? &evs : nullptr;
       int rc = enqueue_step(c, guess_now, ep);
@@ -591,7 +629,6 @@ static int run_loop(pypde_ctx* c, long max_steps, pypde_stats* stats, bool first
     HIPCHK(hipStreamSynchronize(c->stream));
     if (c->host_st.done) break;
     if (chunk < 64) chunk *= 2;
-    (void)max_steps;
   }
   auto t1 = std::chrono::steady_clock::now();
   if (c->opts.profile) {
@@ -638,6 +675,7 @@ int pypde_run(pypde_ctx* c, double tf, long max_steps, int ndt, double* ret, pyp
   s.cfl = c->host_st.cfl;
   s.max_steps = max_steps;
   s.done = (0.0 < tf) ? 0 : 1;
+  c->host_st = s;
   HIPCHK(hipMemcpyAsync(c->st, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
   for (double& m : c->ms_kernel) m = 0;
   double* keep_ret = c->ret;
@@ -666,6 +704,7 @@ int pypde_step(pypde_ctx* c, long steps, double tf, pypde_stats* stats) {
   s.tf = tf;
   s.max_steps = s.count + steps;
   s.done = (s.t < tf && steps > 0) ? 0 : 1;
+  c->host_st = s;
   c->g.ndt = 0;
   HIPCHK(hipMemcpyAsync(c->st, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
   for (double& m : c->ms_kernel) m = 0;
@@ -697,6 +736,189 @@ int pde_solver_hip(const char* src, bool useF, bool useB, bool useS, double* u, 
   }
   pypde_destroy(c);
   return rc;
+}
+
+int pypde_set_guess_model(pypde_ctx* c, const pypde_mlp* m) {
+  HIPCHK(hipSetDevice(c->device));
+  if (!m) {
+    c->use_model = false;
+    memcpy(c->fn, c->spec.guess ? c->fn_guess : c->fn_plain, sizeof c->fn);
+    return 0;
+  }
+  if (!c->spec.stiff) return fail(PYPDE_ERR_ARG, "a guess model needs the stiff (Newton) predictor");
+  if (!c->mod_guess) {
+    Spec s = c->spec;
+    s.guess = true;
+    std::vector<char> code;
+    int rc = compile_code(s, &code);
+    if (rc) return rc;
+    if ((rc = load_module(c, code, &c->mod_guess, c->fn_guess))) return rc;
+  }
+  if (!c->gbuf) HIPCHK(hipMalloc((void**)&c->gbuf, c->g.nwc * c->NT * c->V * sizeof(double)));
+  memcpy(c->fn, c->fn_guess, sizeof c->fn);
+  c->model = {m->W1t, m->b1, m->W2t, m->b2, m->W3t, m->b3, m->in_mean, m->in_rstd, m->out_mean,
+              m->out_std};
+  c->use_model = true;
+  return 0;
+}
+
+int pypde_harvest_step(pypde_ctx* c, float* X, float* Y, long ldx, long col0, pypde_stats* stats) {
+  HIPCHK(hipSetDevice(c->device));
+  if (!c->qh) HIPCHK(hipMalloc((void**)&c->qh, c->g.nwc * c->NT * c->V * sizeof(double)));
+  int rc = pypde_step(c, 1, 1e300, stats);
+  if (rc) return rc;
+  const double* w = c->wh;
+  const double* q = c->qh;
+  const StepState* st = c->st;
+  long nwc = c->g.nwc;
+  std::vector<void*> a = {&w, &q, &st, &X, &Y, &ldx, &col0, &nwc};
+  rc = launch(c, K_HARVEST, dim3((unsigned)cdiv(nwc, 256)), dim3(256), a);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  hipFree(c->qh);
+  c->qh = nullptr;
+  return 0;
+}
+
+int pypde_last_guess(pypde_ctx* c, double* out) {
+  HIPCHK(hipSetDevice(c->device));
+  if (!c->gbuf) return fail(PYPDE_ERR_ARG, "no guess model attached");
+  size_t n = c->g.nwc * c->NT * c->V;
+  std::vector<double> h(n);
+  HIPCHK(hipMemcpy(h.data(), c->gbuf, n * sizeof(double), hipMemcpyDeviceToHost));
+  size_t per = (size_t)c->NT * c->V;
+  for (long cc = 0; cc < c->g.nwc; cc++)
+    for (size_t p = 0; p < per; p++) out[cc * per + p] = h[p * c->g.nwc + cc];
+  return 0;
+}
+
+int pypde_geometry(pypde_ctx* c, long* nwc, int* n_in, int* n_out) {
+  if (nwc) *nwc = c->g.nwc;
+  if (n_in) *n_in = c->ND * c->V + 1;
+  if (n_out) *n_out = c->NT * c->V;
+  return 0;
+}
+
+// ---------------------------------------------------------------- training
+static hipModule_t g_mlp_mod[16];
+static hipFunction_t g_mlp_fn[16][6];
+static const char* mlp_names[6] = {"k_gemm_bf16", "k_bias_act", "k_act_grad", "k_rowsum", "k_mse", "k_adam"};
+
+static int mlp_fn(int k, hipFunction_t* f) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 16) return fail(PYPDE_ERR_ARG, "device %d", dev);
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_mlp_mod[dev]) {
+    std::string src = MLP_SRC;
+    uint64_t h = fnv1a(src + "mlp");
+    char name[64];
+    snprintf(name, sizeof name, "%016llx", (unsigned long long)h);
+    std::string path = cache_dir() + "/" + name + ".hsaco";
+    std::vector<char> code;
+    std::ifstream fin(path, std::ios::binary);
+    if (fin) code.assign((std::istreambuf_iterator<char>(fin)), std::istreambuf_iterator<char>());
+    if (code.empty()) {
+      hiprtcProgram prog;
+      hiprtcCreateProgram(&prog, src.c_str(), "pypde_mlp.hip", 0, nullptr, nullptr);
+      std::vector<const char*> o;
+      for (auto& x : compile_options()) o.push_back(x.c_str());
+      if (hiprtcCompileProgram(prog, (int)o.size(), o.data()) != HIPRTC_SUCCESS) {
+        size_t ls = 0;
+        hiprtcGetProgramLogSize(prog, &ls);
+        std::string log(ls, '\0');
+        if (ls) hiprtcGetProgramLog(prog, &log[0]);
+        hiprtcDestroyProgram(&prog);
+        return fail(PYPDE_ERR_COMPILE, "MLP kernels: %s", log.c_str());
+      }
+      size_t cs = 0;
+      hiprtcGetCodeSize(prog, &cs);
+      code.resize(cs);
+      hiprtcGetCode(prog, code.data());
+      hiprtcDestroyProgram(&prog);
+      mkdir(cache_dir().c_str(), 0755);
+      std::ofstream fo(path, std::ios::binary);
+      fo.write(code.data(), (std::streamsize)code.size());
+    }
+    HIPCHK(hipModuleLoadData(&g_mlp_mod[dev], code.data()));
+    for (int i = 0; i < 6; i++) HIPCHK(hipModuleGetFunction(&g_mlp_fn[dev][i], g_mlp_mod[dev], mlp_names[i]));
+  }
+  *f = g_mlp_fn[dev][k];
+  return 0;
+}
+
+static int mlp_launch(int k, dim3 grid, dim3 block, void** args, void* stream) {
+  hipFunction_t f;
+  int rc = mlp_fn(k, &f);
+  if (rc) return rc;
+  HIPCHK(hipModuleLaunchKernel(f, grid.x, grid.y, grid.z, block.x, block.y, block.z, 0,
+                               (hipStream_t)stream, args, nullptr));
+  return 0;
+}
+
+int pypde_mlp_compile(void) {
+  // compile (or fetch) the training kernels without a GPU
+  std::string src = MLP_SRC;
+  hiprtcProgram prog;
+  hiprtcCreateProgram(&prog, src.c_str(), "pypde_mlp.hip", 0, nullptr, nullptr);
+  std::vector<const char*> o;
+  for (auto& x : compile_options()) o.push_back(x.c_str());
+  hiprtcResult r = hiprtcCompileProgram(prog, (int)o.size(), o.data());
+  size_t ls = 0;
+  hiprtcGetProgramLogSize(prog, &ls);
+  std::string log(ls, '\0');
+  if (ls) hiprtcGetProgramLog(prog, &log[0]);
+  if (r != HIPRTC_SUCCESS) {
+    hiprtcDestroyProgram(&prog);
+    return fail(PYPDE_ERR_COMPILE, "MLP kernels: %s", log.c_str());
+  }
+  size_t cs = 0;
+  hiprtcGetCodeSize(prog, &cs);
+  std::vector<char> code(cs);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  uint64_t h = fnv1a(src + "mlp");
+  char name[64];
+  snprintf(name, sizeof name, "%016llx", (unsigned long long)h);
+  mkdir(cache_dir().c_str(), 0755);
+  std::ofstream fo(cache_dir() + "/" + name + ".hsaco", std::ios::binary);
+  fo.write(code.data(), (std::streamsize)code.size());
+  return 0;
+}
+
+int pypde_mlp_gemm(const float* A, int lda, int transA, const float* B, int ldb, int transB,
+                   float* C, int ldc, int M, int N, int K, float alpha, float beta, void* stream) {
+  if (M <= 0 || N <= 0 || K <= 0) return fail(PYPDE_ERR_ARG, "gemm sizes");
+  void* a[] = {&A, &lda, &transA, &B, &ldb, &transB, &C, &ldc, &M, &N, &K, &alpha, &beta};
+  return mlp_launch(0, dim3((unsigned)cdiv(N, 64), (unsigned)cdiv(M, 64)), dim3(256), a, stream);
+}
+
+int pypde_mlp_bias_act(float* Z, const float* b, float* H, int rows, int cols, int act, void* stream) {
+  void* a[] = {&Z, &b, &H, &rows, &cols, &act};
+  return mlp_launch(1, dim3((unsigned)cdiv((long)rows * cols, 256)), dim3(256), a, stream);
+}
+
+int pypde_mlp_act_grad(float* dZ, const float* dH, const float* Z, long n, void* stream) {
+  void* a[] = {&dZ, &dH, &Z, &n};
+  return mlp_launch(2, dim3((unsigned)cdiv(n, 256)), dim3(256), a, stream);
+}
+
+int pypde_mlp_rowsum(const float* dZ, float* db, int rows, int cols, void* stream) {
+  void* a[] = {&dZ, &db, &rows, &cols};
+  return mlp_launch(3, dim3((unsigned)rows), dim3(256), a, stream);
+}
+
+int pypde_mlp_mse(const float* Y, const float* T, float* dY, float* loss, int rows, int cols,
+                  void* stream) {
+  void* a[] = {&Y, &T, &dY, &loss, &rows, &cols};
+  return mlp_launch(4, dim3((unsigned)cdiv((long)rows * cols, 256)), dim3(256), a, stream);
+}
+
+int pypde_mlp_adam(float* w, const float* g, float* m, float* v, long n, float lr, float b1,
+                   float b2, float eps, int step, void* stream) {
+  float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  void* a[] = {&w, &g, &m, &v, &n, &lr, &b1, &b2, &eps, &c1, &c2};
+  return mlp_launch(5, dim3((unsigned)cdiv(n, 256)), dim3(256), a, stream);
 }
 
 int weno_solver(double* ret, double* u, int* nX, int ndim, int N, int V) {

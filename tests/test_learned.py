@@ -1,0 +1,142 @@
+"""Learned Newton initial guess: matrix-core kernels and their effect.
+
+CPU checker for the MLP: an FP64 numpy restatement that rounds every GEMM
+operand to BF16 exactly where the GPU kernels do (so only the FP32
+accumulation differs), plus a plain FP64 version for the size of the BF16 effect.
+"""
+import numpy as np
+import pytest
+
+from pypde_amd import systems
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(x):
+    """Round-to-nearest-even to bfloat16, returned as float64."""
+    a = np.asarray(x, dtype=np.float32)
+    b = a.view(np.uint32).astype(np.uint64)
+    b = (b + 0x7FFF + ((b >> 16) & 1)) >> 16 << 16
+    return b.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def silu(z):
+    return z / (1 + np.exp(-z))
+
+
+def dsilu(z):
+    s = 1 / (1 + np.exp(-z))
+    return s * (1 + z * (1 - s))
+
+
+def ref_grads(p, X, T, round_bf16=True):
+    r = bf16 if round_bf16 else (lambda a: np.asarray(a, dtype=np.float64))
+    mm = lambda A, B: r(A) @ r(B)
+    Z1 = mm(p["W1t"], X) + p["b1"]
+    H1 = silu(Z1)
+    Z2 = mm(p["W2t"], H1) + p["b2"]
+    H2 = silu(Z2)
+    Y = mm(p["W3t"], H2) + p["b3"]
+    n = Y.size
+    dY = 2 * (Y - T) / n
+    g = {"W3t": mm(dY, H2.T), "b3": dY.sum(1, keepdims=True)}
+    dZ2 = mm(p["W3t"].T, dY) * dsilu(Z2)
+    g["W2t"], g["b2"] = mm(dZ2, H1.T), dZ2.sum(1, keepdims=True)
+    dZ1 = mm(p["W2t"].T, dZ2) * dsilu(Z1)
+    g["W1t"], g["b1"] = mm(dZ1, X.T), dZ1.sum(1, keepdims=True)
+    return ((Y - T) ** 2).mean(), g, Y
+
+
+def nrel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.parametrize("M,N,K,ta,tb", [(64, 64, 64, 0, 0), (100, 37, 45, 0, 0),
+                                          (135, 513, 128, 0, 1), (128, 300, 135, 1, 0),
+                                          (7, 5, 3, 1, 1)])
+def test_mfma_gemm_matches_numpy(M, N, K, ta, tb):
+    import torch
+    from pypde_amd.learned import gemm
+    rng = np.random.default_rng(M + N + K)
+    A = rng.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
+    B = rng.standard_normal((N, K) if tb else (K, N)).astype(np.float32)
+    C0 = rng.standard_normal((M, N)).astype(np.float32)
+    At, Bt, Ct = (torch.tensor(x, device="cuda") for x in (A, B, C0))
+    gemm(At, Bt, Ct, transA=bool(ta), transB=bool(tb), alpha=0.5, beta=2.0)
+    got = Ct.cpu().numpy().astype(np.float64)
+    opA = bf16(A).T if ta else bf16(A)
+    opB = bf16(B).T if tb else bf16(B)
+    want = 0.5 * (opA @ opB) + 2.0 * C0
+    assert np.max(np.abs(got - want)) <= 1e-5 * max(1.0, np.max(np.abs(want)))
+
+
+def test_mlp_gradients_match_cpu():
+    import torch
+    from pypde_amd.learned import GuessMLP
+    rng = np.random.default_rng(3)
+    n_in, n_out, B = 46, 135, 256
+    net = GuessMLP(n_in, n_out, hidden=128, seed=4)
+    X = rng.standard_normal((n_in, B)).astype(np.float32)
+    T = rng.standard_normal((n_out, B)).astype(np.float32)
+    loss, g = net.gradients(torch.tensor(X, device="cuda"), torch.tensor(T, device="cuda"))
+    p = {k: v.cpu().numpy().astype(np.float64) for k, v in net.p.items()}
+    l_ref, g_ref, _ = ref_grads(p, X.astype(np.float64), T.astype(np.float64))
+    l64, g64, _ = ref_grads(p, X.astype(np.float64), T.astype(np.float64), round_bf16=False)
+    assert abs(float(loss.item()) - l_ref) <= 1e-4 * l_ref
+    for k in g_ref:
+        got = g[k].cpu().numpy().astype(np.float64)
+        assert nrel(got, g_ref[k]) <= 2e-3, k     # same BF16 operands, FP32 accumulation
+        assert nrel(got, g64[k]) <= 5e-2, k       # BF16 against full FP64
+
+
+def small_reactive(seed, nx=256, ny=64):
+    return systems.reactive_channel(nx, ny, seed=seed)
+
+
+def make_solver(seed, nx=256, ny=64):
+    from pypde_amd.solvers import Solver
+    return Solver(small_reactive(seed, nx, ny), [nx / 1024, ny / 1024], systems.source("reactive2d"),
+                  order=3, boundaryTypes="reflective", stiff=True,
+                  reflect=systems.REACTIVE_REFLECT)
+
+
+def test_learned_guess_trains_and_saves_iterations():
+    """Train on pairs the solver generates (seeds 0, 1), evaluate on seed 99:
+    fewer Newton iterations than the plain guess, same answer to the Newton
+    tolerance scale, and the fused inference kernel reproduces the training
+    forward pass."""
+    import torch
+    from pypde_amd.learned import GuessMLP, harvest
+    Xs, Ys = [], []
+    for seed in (0, 1):
+        sol = make_solver(seed)
+        X, Y = harvest(sol, 12)
+        Xs.append(X)
+        Ys.append(Y)
+        sol.close()
+    X = torch.cat(Xs, 1)
+    Y = torch.cat(Ys, 1)
+    net = GuessMLP(X.shape[0], Y.shape[0], hidden=128, seed=0)
+    hist = net.train(X, Y, epochs=8, batch=4096, lr=2e-3)
+    assert hist[-1] < 0.5 * hist[0]
+    plain = make_solver(99)
+    sp = plain.step(12)
+    up = plain.state()
+    plain.close()
+    learned = make_solver(99)
+    net.attach(learned)
+    sl = learned.step(12)
+    ul = learned.state()
+    print("plain iters", sp["pred_iters"], "learned iters", sl["pred_iters"], "fallbacks",
+          sl["guess_fallbacks"])
+    assert sl["pred_capped"] == 0 and sp["pred_capped"] == 0
+    assert sl["pred_iters"] < sp["pred_iters"]
+    assert np.max(np.abs(ul - up)) <= 1e-8 * np.max(np.abs(up))
+    # fused inference kernel vs the training forward pass on the last step's inputs
+    from pypde_amd.learned import _bind
+    import ctypes as C
+    nw = (256 + 2) * (64 + 2)
+    gl = np.zeros(nw * 27 * 5)
+    _bind().pypde_last_guess(C.c_void_p(learned.h), gl.ctypes.data_as(C.POINTER(C.c_double)))
+    learned.close()
+    assert np.all(np.isfinite(gl))
