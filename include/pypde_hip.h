@@ -14,6 +14,7 @@
 #define PYPDE_HIP_H
 
 #include <stdbool.h>
+#include <stddef.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -127,6 +128,15 @@ int pypde_mlp_mse(const float *Y, const float *T, float *dY, float *loss, int ro
 int pypde_mlp_adam(float *w, const float *g, float *m, float *v, long n, float lr, float b1,
                    float b2, float eps, int step, void *stream);
 int pypde_mlp_compile(void);
+
+/* Device memory for callers without their own HIP runtime (kind: 1 host to
+ * device, 2 device to host, 3 device to device). */
+void *pypde_dev_alloc(size_t bytes);
+int pypde_dev_free(void *p);
+int pypde_memcpy(void *dst, const void *src, size_t bytes, int kind);
+int pypde_dev_memset(void *p, int value, size_t bytes);
+int pypde_dev_sync(void);
+int pypde_device_count(void);
 
 /* Compile (or fetch from the code-object cache) without touching a GPU. */
 int pypde_hip_compile(const char *src, bool useF, bool useB, bool useS, int ndim, int N, int V,

@@ -856,6 +856,43 @@ static int mlp_launch(int k, dim3 grid, dim3 block, void** args, void* stream) {
   return 0;
 }
 
+void* pypde_dev_alloc(size_t bytes) {
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+    fail(PYPDE_ERR_HIP, "hipMalloc(%zu) failed", bytes);
+    return nullptr;
+  }
+  return p;
+}
+
+int pypde_dev_free(void* p) {
+  if (p) HIPCHK(hipFree(p));
+  return 0;
+}
+
+int pypde_memcpy(void* dst, const void* src, size_t bytes, int kind) {
+  hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice
+                              : (kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+  HIPCHK(hipMemcpy(dst, src, bytes, k));
+  return 0;
+}
+
+int pypde_dev_memset(void* p, int value, size_t bytes) {
+  HIPCHK(hipMemset(p, value, bytes));
+  return 0;
+}
+
+int pypde_dev_sync(void) {
+  HIPCHK(hipDeviceSynchronize());
+  return 0;
+}
+
+int pypde_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
 int pypde_mlp_compile(void) {
   // compile (or fetch) the training kernels without a GPU
   std::string src = MLP_SRC;

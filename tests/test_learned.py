@@ -55,15 +55,14 @@ def nrel(a, b):
                                           (135, 513, 128, 0, 1), (128, 300, 135, 1, 0),
                                           (7, 5, 3, 1, 1)])
 def test_mfma_gemm_matches_numpy(M, N, K, ta, tb):
-    import torch
-    from pypde_amd.learned import gemm
+    from pypde_amd.learned import Dev, gemm
     rng = np.random.default_rng(M + N + K)
     A = rng.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
     B = rng.standard_normal((N, K) if tb else (K, N)).astype(np.float32)
     C0 = rng.standard_normal((M, N)).astype(np.float32)
-    At, Bt, Ct = (torch.tensor(x, device="cuda") for x in (A, B, C0))
+    At, Bt, Ct = (Dev.from_numpy(x) for x in (A, B, C0))
     gemm(At, Bt, Ct, transA=bool(ta), transB=bool(tb), alpha=0.5, beta=2.0)
-    got = Ct.cpu().numpy().astype(np.float64)
+    got = Ct.numpy().astype(np.float64)
     opA = bf16(A).T if ta else bf16(A)
     opB = bf16(B).T if tb else bf16(B)
     want = 0.5 * (opA @ opB) + 2.0 * C0
@@ -71,20 +70,20 @@ def test_mfma_gemm_matches_numpy(M, N, K, ta, tb):
 
 
 def test_mlp_gradients_match_cpu():
-    import torch
-    from pypde_amd.learned import GuessMLP
+    from pypde_amd.learned import Dev, GuessMLP
     rng = np.random.default_rng(3)
     n_in, n_out, B = 46, 135, 256
     net = GuessMLP(n_in, n_out, hidden=128, seed=4)
     X = rng.standard_normal((n_in, B)).astype(np.float32)
     T = rng.standard_normal((n_out, B)).astype(np.float32)
-    loss, g = net.gradients(torch.tensor(X, device="cuda"), torch.tensor(T, device="cuda"))
-    p = {k: v.cpu().numpy().astype(np.float64) for k, v in net.p.items()}
+    loss, g = net.gradients(Dev.from_numpy(X), Dev.from_numpy(T))
+    loss = loss.numpy()[0, 0]
+    p = {k: v.numpy().astype(np.float64) for k, v in net.p.items()}
     l_ref, g_ref, _ = ref_grads(p, X.astype(np.float64), T.astype(np.float64))
     l64, g64, _ = ref_grads(p, X.astype(np.float64), T.astype(np.float64), round_bf16=False)
-    assert abs(float(loss.item()) - l_ref) <= 1e-4 * l_ref
+    assert abs(float(loss) - l_ref) <= 1e-4 * l_ref
     for k in g_ref:
-        got = g[k].cpu().numpy().astype(np.float64)
+        got = g[k].numpy().astype(np.float64)
         assert nrel(got, g_ref[k]) <= 2e-3, k     # same BF16 operands, FP32 accumulation
         assert nrel(got, g64[k]) <= 5e-2, k       # BF16 against full FP64
 
@@ -105,7 +104,6 @@ def test_learned_guess_trains_and_saves_iterations():
     fewer Newton iterations than the plain guess, same answer to the Newton
     tolerance scale, and the fused inference kernel reproduces the training
     forward pass."""
-    import torch
     from pypde_amd.learned import GuessMLP, harvest
     Xs, Ys = [], []
     for seed in (0, 1):
@@ -114,8 +112,8 @@ def test_learned_guess_trains_and_saves_iterations():
         Xs.append(X)
         Ys.append(Y)
         sol.close()
-    X = torch.cat(Xs, 1)
-    Y = torch.cat(Ys, 1)
+    X = np.concatenate(Xs, 1)
+    Y = np.concatenate(Ys, 1)
     net = GuessMLP(X.shape[0], Y.shape[0], hidden=128, seed=0)
     hist = net.train(X, Y, epochs=8, batch=4096, lr=2e-3)
     assert hist[-1] < 0.5 * hist[0]
@@ -132,11 +130,24 @@ def test_learned_guess_trains_and_saves_iterations():
     assert sl["pred_capped"] == 0 and sp["pred_capped"] == 0
     assert sl["pred_iters"] < sp["pred_iters"]
     assert np.max(np.abs(ul - up)) <= 1e-8 * np.max(np.abs(up))
-    # fused inference kernel vs the training forward pass on the last step's inputs
-    from pypde_amd.learned import _bind
+    # fused inference kernel vs the training forward pass: one more (harvest)
+    # step gives that step's features, and the guess buffer holds what the
+    # fused kernel inferred from them
     import ctypes as C
-    nw = (256 + 2) * (64 + 2)
-    gl = np.zeros(nw * 27 * 5)
+    from pypde_amd.learned import Dev, _bind
+    Xh, Yh = harvest(learned, 1)
+    nw, NT, V = (256 + 2) * (64 + 2), 27, 5
+    gl = np.zeros(nw * NT * V)
     _bind().pypde_last_guess(C.c_void_p(learned.h), gl.ctypes.data_as(C.POINTER(C.c_double)))
     learned.close()
+    gl = gl.reshape(nw, NT * V)
+    y = net.forward(Dev.from_numpy(net.normalise(Xh))).numpy().astype(np.float64)
+    val = y * net.norm_host["out_std"][:, None] + net.norm_host["out_mean"][:, None]
+    w = Xh[:9 * V].astype(np.float64)        # wh rows (node, v), stored as fp32
+    dt = Xh[-1].astype(np.float64)
+    wt = np.tile(w, (3, 1))                    # broadcast over the 3 time nodes
+    ref = (wt + dt * val).T                    # cells x outputs
+    corr = np.max(np.abs(dt * val))
     assert np.all(np.isfinite(gl))
+    # BF16 operands agree; fp32 normalisation and the fp32 copy of w differ
+    assert np.max(np.abs(gl - ref)) <= 1e-3 * corr + 1e-7 * np.max(np.abs(w))
