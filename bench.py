@@ -1,12 +1,19 @@
 """Headline benchmark: 2D reactive Euler (Arrhenius, nvar 5), 1024 x 256
 channel, order 3, stiff per-cell Newton predictor, reflective walls, FP64.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-learned]
 
 Prints one JSON line.  A step is one full ADER-WENO timestep of the whole
-grid (boundaries + WENO, CFL reduction, predictor, face fluxes, corrector);
-the state is resident in HBM before the timed region.  `value` is
+grid (boundaries + WENO, CFL reduction, learned guess, predictor, face fluxes,
+corrector); the state is resident in HBM before the timed region.
 cell-updates/s = cells x K / wall time of the K timed steps.
+
+Two legs on the same seeded initial state (seed 99): the plain FP64 method
+(constant-in-time Newton initial guess) and the learned one (an MLP trained
+here, on the GPU, on predictor solutions harvested from seeds 0..15 of the same
+channel at 256 x 64 cells and the same cell size).  `value` is the learned
+leg's rate; both legs' rates, Newton iteration counts and their difference in
+the final state are reported beside it.
 """
 import argparse
 import ctypes as C
@@ -24,6 +31,10 @@ sys.path.insert(0, ROOT)
 
 from pypde_amd import systems  # noqa: E402
 from pypde_amd.solvers import Solver  # noqa: E402
+
+EVAL_SEED = 99
+TRAIN_SEEDS = range(16)
+TRAIN_NX, TRAIN_NY = 256, 64
 
 NX, NY, ORDER, V, D = 1024, 256, 3, 5, 2
 L = [1.0, NY / NX]
@@ -60,11 +71,47 @@ def predictor_flops_epilogue(N=ORDER, D=D, V=V, cS=FLOPS_S):
     return traces + volume
 
 
-def gpu_run(steps, warmup, profile):
-    u0 = systems.reactive_channel(NX, NY, seed=0)
-    sol = Solver(u0, L, systems.source("reactive2d"), order=ORDER, boundaryTypes="reflective",
-                 stiff=True, reflect=systems.REACTIVE_REFLECT, newton_tol=1e-12,
-                 newton_maxit=100, profile=profile)
+def make_solver(u0, L_, profile=False):
+    return Solver(u0, L_, systems.source("reactive2d"), order=ORDER, boundaryTypes="reflective",
+                  stiff=True, reflect=systems.REACTIVE_REFLECT, newton_tol=1e-12,
+                  newton_maxit=100, profile=profile)
+
+
+def train_guess_model(steps, pairs, epochs, log):
+    """Harvest (reconstruction, dt) -> predictor-solution pairs from the solver
+    on seeds 0..15 (256 x 64 cells, dx = 1/1024 as in the full channel), keep
+    `pairs` of them at random, and train the guess MLP on the GPU."""
+    from pypde_amd.learned import GuessMLP, harvest
+    t0 = time.perf_counter()
+    per_seed = pairs // len(TRAIN_SEEDS)
+    rng = np.random.default_rng(1234)
+    Xs, Ys = [], []
+    for seed in TRAIN_SEEDS:
+        sol = make_solver(systems.reactive_channel(TRAIN_NX, TRAIN_NY, seed=seed),
+                          [TRAIN_NX / NX, TRAIN_NY / NX])
+        X, Y = harvest(sol, steps)
+        sol.close()
+        keep = rng.choice(X.shape[1], size=min(per_seed, X.shape[1]), replace=False)
+        Xs.append(X[:, keep])
+        Ys.append(Y[:, keep])
+    X = np.concatenate(Xs, 1)
+    Y = np.concatenate(Ys, 1)
+    del Xs, Ys
+    t1 = time.perf_counter()
+    net = GuessMLP(X.shape[0], Y.shape[0], hidden=128, seed=0)
+    hist = net.train(X, Y, epochs=epochs, batch=4096, lr=2e-3, log=log)
+    t2 = time.perf_counter()
+    info = dict(pairs=int(X.shape[1]), seeds=f"{TRAIN_SEEDS[0]}..{TRAIN_SEEDS[-1]}",
+                grid=f"{TRAIN_NX}x{TRAIN_NY}", steps_per_seed=steps, epochs=epochs,
+                loss_first=hist[0], loss_last=hist[-1], harvest_s=t1 - t0, train_s=t2 - t1)
+    return net, info
+
+
+def gpu_run(steps, warmup, profile, net=None):
+    u0 = systems.reactive_channel(NX, NY, seed=EVAL_SEED)
+    sol = make_solver(u0, L, profile=profile)
+    if net is not None:
+        net.attach(sol)
     w = sol.step(warmup) if warmup > 0 else None
     t0 = time.perf_counter()
     st = sol.step(steps)
@@ -72,6 +119,23 @@ def gpu_run(steps, warmup, profile):
     state = sol.state()
     sol.close()
     return st, t1 - t0, w, state
+
+
+def rel_linf(a, b):
+    """max over cells and variables of |a - b| / max(|b[var]|_inf, 1e-300)."""
+    scale = np.maximum(np.max(np.abs(b.reshape(-1, V)), 0), 1e-300)
+    return float(np.max(np.abs((a - b).reshape(-1, V)) / scale))
+
+
+def predictor_roofline(st, K):
+    ms = st["ms_kernel"]
+    pred_ms = ms["predictor"]
+    flops = (predictor_flops_per_iteration() * st["pred_iters"]
+             + predictor_flops_epilogue() * st["pred_cells"])
+    achieved = flops / (pred_ms * 1e-3) if pred_ms > 0 else 0.0
+    return dict(bound="fp64", achieved=achieved / 1e12, peak=PEAK_FP64 / 1e12, unit="TFLOP/s",
+                frac=achieved / PEAK_FP64, traffic=None, kernel="k_predictor",
+                avg_launch_ms=pred_ms / K)
 
 
 def cpu_baseline(t_target, threads):
@@ -83,7 +147,7 @@ def cpu_baseline(t_target, threads):
     if not cc.ref_available():
         return None
     s = cc.system("reactive2d")
-    u = np.ascontiguousarray(systems.reactive_channel(NX, NY, seed=0)).copy()
+    u = np.ascontiguousarray(systems.reactive_channel(NX, NY, seed=EVAL_SEED)).copy()
     nX = np.array([NX, NY], dtype=np.int32)
     dX = np.array([L[0] / NX, L[1] / NY])
     bc = np.array([0, 0], dtype=np.int32)
@@ -115,28 +179,43 @@ def cpu_baseline(t_target, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--train-steps", type=int, default=12)
+    ap.add_argument("--train-pairs", type=int, default=2_000_000)
+    ap.add_argument("--epochs", type=int, default=6)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     K, W = args.steps, args.warmup
-    st, el, wst, state = gpu_run(K, W, profile=True)
     cells = NX * NY
-    value = cells * K / el
-    # roofline of the dominant kernel: the predictor, FP64-bound
-    ms = st["ms_kernel"]
-    pred_ms = ms["predictor"]
-    flops = (predictor_flops_per_iteration() * st["pred_iters"]
-             + predictor_flops_epilogue() * st["pred_cells"])
-    achieved = flops / (pred_ms * 1e-3) if pred_ms > 0 else 0.0
-    roof = dict(bound="fp64", achieved=achieved / 1e12, peak=PEAK_FP64 / 1e12, unit="TFLOP/s",
-                frac=achieved / PEAK_FP64, traffic=None, kernel="k_predictor",
-                avg_launch_ms=pred_ms / K)
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    st_p, el_p, _, u_p = gpu_run(K, W, profile=True)
+    log(f"plain: {el_p * 1e3 / K:.3f} ms/step")
+    plain = dict(value=cells * K / el_p, ms_per_step=el_p * 1e3 / K,
+                 iters_per_cell=st_p["pred_iters"] / max(1, st_p["pred_cells"]),
+                 max_iters=st_p["pred_maxit"], capped=st_p["pred_capped"],
+                 kernel_ms_per_step={k: v / K for k, v in st_p["ms_kernel"].items()},
+                 roofline=predictor_roofline(st_p, K))
+    learned = None
+    if not args.no_learned:
+        net, info = train_guess_model(args.train_steps, args.train_pairs, args.epochs, log)
+        log(f"trained: {info}")
+        st_l, el_l, _, u_l = gpu_run(K, W, profile=True, net=net)
+        log(f"learned: {el_l * 1e3 / K:.3f} ms/step")
+        learned = dict(value=cells * K / el_l, ms_per_step=el_l * 1e3 / K,
+                       iters_per_cell=st_l["pred_iters"] / max(1, st_l["pred_cells"]),
+                       max_iters=st_l["pred_maxit"], capped=st_l["pred_capped"],
+                       guess_fallbacks=st_l["guess_fallbacks"],
+                       kernel_ms_per_step={k: v / K for k, v in st_l["ms_kernel"].items()},
+                       linf_vs_plain=rel_linf(u_l, u_p), training=info,
+                       roofline=predictor_roofline(st_l, K))
+    head = learned if learned is not None else plain
     cpu = None
     if not args.no_cpu:
         # t after args.cpu_steps steps from t = 0, from a short GPU run
@@ -144,29 +223,30 @@ def main():
         cpu = cpu_baseline(st2["t"], threads=min(16, os.cpu_count() or 1))
     out = {
         "metric": "cell-updates/s, 2D reactive Euler O3: learned+mixed vs plain FP64 @Linf<=1e-8",
-        "value": value,
+        "value": head["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": K,
         "warmup": W,
-        "ms_per_step": el * 1e3 / K,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded initial condition, SplitMix64 seed 0)",
+        "data": f"synthetic (seeded initial condition, SplitMix64 seed {EVAL_SEED})",
         "config": {"workload": "2D reactive Euler, Arrhenius (Q=25, Ea=20, K=1e4, gamma=1.2), "
                                "1024x256 channel, order 3, Rusanov, CFL 0.9, reflective walls, "
-                               "stiff per-cell Newton predictor (tol 1e-12), plain FP64",
+                               "stiff per-cell Newton predictor (tol 1e-12), "
+                               + ("learned Newton initial guess (BF16 MLP)"
+                                  if learned is not None else "plain FP64"),
                    "cells": cells, "order": ORDER, "nvar": V},
-        "roofline": roof,
+        "roofline": head["roofline"],
         "cpu_baseline": cpu,
-        "kernel_ms_per_step": {k: v / K for k, v in ms.items()},
-        "predictor_iters_per_cell": st["pred_iters"] / max(1, st["pred_cells"]),
-        "predictor_max_iters": st["pred_maxit"],
-        "predictor_capped": st["pred_capped"],
-        "t_end": st["t"],
-        "finite": bool(np.isfinite(state).all()),
+        "speedup_vs_plain": (learned["value"] / plain["value"]) if learned else None,
+        "plain": plain,
+        "learned": learned,
+        "t_end": st_p["t"],
+        "finite": bool(np.isfinite(u_p).all()),
     }
     print(json.dumps(out))
 

@@ -53,8 +53,8 @@ typedef struct {
   long guess_fallbacks;     /* learned guesses abandoned for the plain guess */
   int nan_step;             /* first step with NaNs (0: none) */
   double ms_loop;           /* wall time of the step loop */
-  double ms_kernel[8];      /* profile: summed time per kernel (weno, dt, predictor,
-                               flux0, flux1, update, bookkeeping, other) */
+  double ms_kernel[8];      /* profile: summed time per kernel (weno, dt, learned guess,
+                               predictor, flux0, flux1, update, bookkeeping) */
 } pypde_stats;
 
 void pypde_default_options(pypde_options *opts);

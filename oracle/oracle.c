@@ -18,6 +18,7 @@
 #define MAXV 8
 #define MAXD 3
 #define MAXND 216 /* MAXN^MAXD */
+#define HOUSE_TINY 0x1p-900
 
 /* Constants of the reference: types.h:18, weno.h:8-10, dg.h:8-9,
  * NumericalDiff.h:9. */
@@ -359,28 +360,31 @@ static void node_deriv(const Tab *T, const double *q, int idx, int d,
 
 /* ---------------------------------------------------------- eigenvalues */
 
-/* Householder vector for x[0..m-1] (Golub & Van Loan, Alg. 5.1.1): P x = alpha e_1,
- * P = I - beta v v^T with v[0] = 1. */
+/* Householder reflector for x[0..m-1] (Golub & Van Loan, Alg. 5.1.1, with v
+ * left unnormalised): P x = mu e_1, P = I - beta v v^T, v = (v0, x[1..m-1]),
+ * beta = 2 / (v^T v).  Two divisions per reflector instead of m + 1.  Columns
+ * whose tail is below 2^-900 in square norm are treated as already reduced
+ * (the matrix is scaled to entries <= 1 first), which keeps beta finite. */
 static double house(const double *x, int m, double *v) {
   double sigma = 0.0;
   for (int i = 1; i < m; i++) sigma = fma(x[i], x[i], sigma);
-  v[0] = 1.0;
-  if (sigma == 0.0) {
+  if (sigma <= HOUSE_TINY) {
+    v[0] = 1.0;
     for (int i = 1; i < m; i++) v[i] = 0.0;
     return 0.0;
   }
   double mu = sqrt(x[0] * x[0] + sigma);
   double v0 = (x[0] <= 0.0) ? x[0] - mu : -sigma / (x[0] + mu);
-  double beta = 2.0 * (v0 * v0) / (sigma + v0 * v0);
-  for (int i = 1; i < m; i++) v[i] = x[i] / v0;
-  return beta;
+  v[0] = v0;
+  for (int i = 1; i < m; i++) v[i] = x[i];
+  return 2.0 / (v0 * v0 + sigma);
 }
 
 /* rows r0..r0+m-1, columns c0..c1: H = P H */
 static void house_left(double *H, int n, const double *v, int m, double beta,
                        int r0, int c0, int c1) {
   for (int j = c0; j <= c1; j++) {
-    double s = H[r0 * n + j];
+    double s = v[0] * H[r0 * n + j];
     for (int i = 1; i < m; i++) s = fma(v[i], H[(r0 + i) * n + j], s);
     s = beta * s;
     for (int i = 0; i < m; i++) H[(r0 + i) * n + j] = fma(-s, v[i], H[(r0 + i) * n + j]);
@@ -391,7 +395,7 @@ static void house_left(double *H, int n, const double *v, int m, double beta,
 static void house_right(double *H, int n, const double *v, int m, double beta,
                         int r0, int r1, int c0) {
   for (int i = r0; i <= r1; i++) {
-    double s = H[i * n + c0];
+    double s = v[0] * H[i * n + c0];
     for (int j = 1; j < m; j++) s = fma(v[j], H[i * n + c0 + j], s);
     s = beta * s;
     for (int j = 0; j < m; j++) H[i * n + c0 + j] = fma(-s, v[j], H[i * n + c0 + j]);
@@ -416,8 +420,9 @@ static int rest_bounded(const double *H, int n, int hi, double ex, double best) 
 
 /* max |lambda| of a real n x n matrix, the quantity the reference takes from
  * Eigen's EigenSolver (system.cpp:28-43).  Method (EISPACK hqr structure,
- * eigenvalues only): scale by the largest |entry| (as Eigen's RealSchur,
- * RealSchur.h:258-276), Householder reduction to Hessenberg form, then Francis
+ * eigenvalues only): scale by the power of two at or above the largest |entry|
+ * (exact; Eigen's RealSchur divides by the largest |entry|, RealSchur.h:258-276),
+ * Householder reduction to Hessenberg form, then Francis
  * double-shift QR steps (Golub & Van Loan Alg. 7.5.1-7.5.2) on the active
  * block, deflating on subdiagonals below eps (|h_kk| + |h_k-1,k-1|).  After 10
  * and 30 steps without deflation the origin is moved (exceptional shifts of
@@ -430,7 +435,10 @@ double oracle_max_abs_eig(const double *A, int n) {
   double scale = 0.0;
   for (int i = 0; i < n * n; i++) scale = fmax(scale, fabs(A[i]));
   if (!(scale >= DBL_MIN)) return (scale != scale) ? scale : 0.0;
-  for (int i = 0; i < n * n; i++) H[i] = A[i] / scale;
+  int e2;
+  (void)frexp(scale, &e2);                 /* scale = m 2^e2, m in [0.5, 1) */
+  double pw = ldexp(1.0, -e2);
+  for (int i = 0; i < n * n; i++) H[i] = A[i] * pw;
   double v[3], x[MAXV];
   for (int k = 0; k + 2 < n; k++) {
     int m = n - k - 1;
@@ -550,11 +558,13 @@ double oracle_max_abs_eig(const double *A, int n) {
       house_right(H, n, v, 2, beta, lo, hi, hi - 1);
     }
   }
-  return best * scale;
+  return ldexp(best, e2);
 }
 
 /* A = dF/dQ by forward differences + B (system.cpp:6-26, NumericalDiff.h:6-44):
- * h_i = max(sqrt(eps)|q_i|, sqrt(eps)), column i = (F(q + h_i e_i) - F(q)) / h_i.
+ * h_i = max(sqrt(eps)|q_i|, sqrt(eps)), column i = (F(q + h_i e_i) - F(q)) (1 / h_i)
+ * (one division per column; the reference divides every entry, which differs
+ * by at most one rounding).
  * f0 (optional) receives F(q). */
 static void system_matrix(oracle_F F, oracle_B B, const double *q,
                           const double *dq, int d, int V, double *A,
@@ -570,7 +580,8 @@ static void system_matrix(oracle_F F, oracle_B B, const double *q,
       qq[i] = qi0 + h;
       F(v2, qq, dqq, d);
       qq[i] = qi0;
-      for (int r = 0; r < V; r++) A[r * V + i] = (v2[r] - v1[r]) / h;
+      double rh = 1.0 / h;
+      for (int r = 0; r < V; r++) A[r * V + i] = (v2[r] - v1[r]) * rh;
     }
     if (f0) memcpy(f0, v1, V * sizeof(double));
   } else {
@@ -1048,7 +1059,8 @@ static int newton_cell(const Ctx *C, const double *w, double *q, double *rhs,
           qq[j] = q0 + hh;
           C->S(s1, qq);
           qq[j] = q0;
-          for (int i = 0; i < V; i++) JS[i * V + j] = (s1[i] - s0[i]) / hh;
+          double rhh = 1.0 / hh;
+          for (int i = 0; i < V; i++) JS[i * V + j] = (s1[i] - s0[i]) * rhh;
         }
       } else {
         for (int i = 0; i < V * V; i++) JS[i] = 0.0;

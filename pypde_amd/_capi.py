@@ -29,8 +29,8 @@ class Stats(C.Structure):
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("struct_size", "ms_kernel")}
-        d["ms_kernel"] = dict(zip(["weno", "dt", "predictor", "flux0", "flux1", "update",
-                                   "advance"], list(self.ms_kernel)[:7]))
+        d["ms_kernel"] = dict(zip(["weno", "dt", "guess", "predictor", "flux0", "flux1",
+                                   "update", "advance"], list(self.ms_kernel)))
         return d
 
 

@@ -103,24 +103,27 @@ DEV int ghost_src(int i, int type, int n, bool* flip) {
 }
 
 // ------------------------------------------------------------ eigenvalues
-// Householder vector (Golub & Van Loan Alg. 5.1.1), v[0] = 1.
+// Householder reflector (Golub & Van Loan Alg. 5.1.1, v left unnormalised):
+// P x = mu e_1, P = I - beta v v^T, v = (v0, x[1..M-1]), beta = 2 / (v^T v).
+// A tail below 2^-900 in square norm counts as reduced (entries are <= 1).
+constexpr double HOUSE_TINY = 0x1p-900;
 template <int M>
 DEV double house(const double* x, double* v) {
   double sigma = 0.0;
 #pragma unroll
   for (int i = 1; i < M; i++) sigma = dfma(x[i], x[i], sigma);
-  v[0] = 1.0;
-  if (sigma == 0.0) {
+  if (sigma <= HOUSE_TINY) {
+    v[0] = 1.0;
 #pragma unroll
     for (int i = 1; i < M; i++) v[i] = 0.0;
     return 0.0;
   }
   double mu = dsqrt(x[0] * x[0] + sigma);
   double v0 = (x[0] <= 0.0) ? x[0] - mu : -sigma / (x[0] + mu);
-  double beta = 2.0 * (v0 * v0) / (sigma + v0 * v0);
+  v[0] = v0;
 #pragma unroll
-  for (int i = 1; i < M; i++) v[i] = x[i] / v0;
-  return beta;
+  for (int i = 1; i < M; i++) v[i] = x[i];
+  return 2.0 / (v0 * v0 + sigma);
 }
 
 // H = P H on rows r0..r0+M-1 for the columns j with lo_c <= j <= hi_c.
@@ -129,7 +132,7 @@ DEV void house_left(double* H, const double* v, double beta, int lo_c, int hi_c)
 #pragma unroll
   for (int j = 0; j < V; j++) {
     if (j >= lo_c && j <= hi_c) {
-      double s = H[R0 * V + j];
+      double s = v[0] * H[R0 * V + j];
 #pragma unroll
       for (int i = 1; i < M; i++) s = dfma(v[i], H[(R0 + i) * V + j], s);
       s = beta * s;
@@ -145,7 +148,7 @@ DEV void house_right(double* H, const double* v, double beta, int lo_r, int hi_r
 #pragma unroll
   for (int i = 0; i < V; i++) {
     if (i >= lo_r && i <= hi_r) {
-      double s = H[i * V + C0];
+      double s = v[0] * H[i * V + C0];
 #pragma unroll
       for (int j = 1; j < M; j++) s = dfma(v[j], H[i * V + C0 + j], s);
       s = beta * s;
@@ -186,8 +189,9 @@ DEV bool rest_bounded(const double* H, int hi, double ex, double best) {
 }
 
 // max |lambda| of A (V x V, row-major; destroyed), as the reference takes it from
-// Eigen's EigenSolver (system.cpp:28-43): scale by the largest |entry| (Eigen's
-// RealSchur, RealSchur.h:258-276), Householder Hessenberg reduction, then
+// Eigen's EigenSolver (system.cpp:28-43): scale by the power of two at or above
+// the largest |entry| (exact; Eigen's RealSchur divides by the largest |entry|,
+// RealSchur.h:258-276), Householder Hessenberg reduction, then
 // Francis double-shift QR steps (Golub & Van Loan Alg. 7.5.1-7.5.2, EISPACK
 // hqr structure) with deflation on eps-small subdiagonals and origin-moving
 // exceptional shifts after 10 and 30 steps without deflation; after 40 V steps
@@ -198,8 +202,11 @@ DEV double max_abs_eig(double* H) {
 #pragma unroll
   for (int i = 0; i < V * V; i++) scale = dmax(scale, dabs(H[i]));
   if (!(scale >= 2.2250738585072014e-308)) return (scale != scale) ? scale : 0.0;
+  int e2;
+  (void)__builtin_frexp(scale, &e2);  // scale = m 2^e2, m in [0.5, 1)
+  const double pw = __builtin_ldexp(1.0, -e2);
 #pragma unroll
-  for (int i = 0; i < V * V; i++) H[i] = H[i] / scale;
+  for (int i = 0; i < V * V; i++) H[i] = H[i] * pw;
   static_for<0, (V >= 3 ? V - 2 : 0)>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
     constexpr int m = V - k - 1;
@@ -344,7 +351,7 @@ DEV double max_abs_eig(double* H) {
       });
     }
   }
-  return best * scale;
+  return __builtin_ldexp(best, e2);
 }
 
 // ------------------------------------------------------ system functions
@@ -358,7 +365,8 @@ DEV void callF(double* out, const double* q, const double* dq, int d) {
 }
 
 // A = dF/dQ by forward differences + B(q, d) (system.cpp:6-26,
-// NumericalDiff.h:6-44); f0 receives F(q).
+// NumericalDiff.h:6-44), column i scaled by 1 / h_i (one division per column);
+// f0 receives F(q).
 DEV void system_matrix(const double* q_in, const double* dq, int d, double* A, double* f0) {
   double q[V];
 #pragma unroll
@@ -373,8 +381,9 @@ DEV void system_matrix(const double* q_in, const double* dq, int d, double* A, d
     q[i] = qi0 + h;
     F(v2, q, dq, d);
     q[i] = qi0;
+    double rh = 1.0 / h;
 #pragma unroll
-    for (int r = 0; r < V; r++) A[r * V + i] = (v2[r] - v1[r]) / h;
+    for (int r = 0; r < V; r++) A[r * V + i] = (v2[r] - v1[r]) * rh;
   }
 #pragma unroll
   for (int i = 0; i < V; i++) f0[i] = v1[i];

@@ -192,8 +192,9 @@ DEV void newton_node(const NodeCtx& c, const double (&q)[N][V], double (&G)[N][V
       qq[j] = q0 + hh;
       S(s1, qq);
       qq[j] = q0;
+      double rhh = 1.0 / hh;
 #pragma unroll
-      for (int i = 0; i < V; i++) A[i * V + j] = (s1[i] - s0[i]) / hh;
+      for (int i = 0; i < V; i++) A[i * V + j] = (s1[i] - s0[i]) * rhh;
     }
   }
 #else

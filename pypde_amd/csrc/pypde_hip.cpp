@@ -336,6 +336,7 @@ int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
     std::vector<void*> a = {&w, &st, &m, &gb, &nwc};
     if ((rc = launch(c, K_MLPGUESS, dim3((unsigned)cdiv(g.nwc, 128)), dim3(256), a))) return rc;
   }
+  mark();
   {
     const double* w = c->wh;
     const double* gs = c->use_model ? c->gbuf : (guess_now ? c->guess : nullptr);
@@ -632,9 +633,9 @@ static int run_loop(pypde_ctx* c, long max_steps, pypde_stats* stats, bool first
   }
   auto t1 = std::chrono::steady_clock::now();
   if (c->opts.profile) {
-    // 8 marks per step bracket: weno | dt | predictor | flux0 | flux1 | update | advance
-    for (size_t s0 = 0; s0 + 8 <= evs.size(); s0 += 8)
-      for (int k = 0; k < 7; k++) {
+    // 9 marks per step bracket: weno | dt | guess | predictor | flux0 | flux1 | update | advance
+    for (size_t s0 = 0; s0 + 9 <= evs.size(); s0 += 9)
+      for (int k = 0; k < 8; k++) {
         float ms = 0;
         hipEventElapsedTime(&ms, evs[s0 + k], evs[s0 + k + 1]);
         c->ms_kernel[k] += ms;
