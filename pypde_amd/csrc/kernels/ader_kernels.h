@@ -32,6 +32,7 @@ struct StepState {
   unsigned int nan_flag, pad2;
   unsigned long long pred_iters, pred_cells, pred_capped, pred_fallbacks;
   unsigned int pred_maxit, pad3;
+  unsigned long long qhead;     // predictor work queue: next unclaimed w-cell
 };
 
 DEV double wave_max(double x) {
@@ -222,4 +223,5 @@ extern "C" __global__ void k_dt_finalize(StepState* st) {
   if (st->t + dt > st->tf) dt = st->tf - st->t;
   st->dt = dt;
   st->maxbits = 0ull;
+  st->qhead = 0ull;
 }

@@ -244,20 +244,21 @@ DEV void newton_node(const NodeCtx& c, const double (&q)[N][V], double (&G)[N][V
   for (int i = 0; i < V; i++) P[i * V + i] = P[i * V + i] + T_FLC[N - 1];
 #pragma unroll
   for (int j = N - 2; j >= 0; j--) {
-    double PA[V * V];
+    // P <- P A + c_j I, one row at a time (row i of P A needs only row i of P)
 #pragma unroll
-    for (int i = 0; i < V; i++)
+    for (int i = 0; i < V; i++) {
+      double row[V];
 #pragma unroll
       for (int k = 0; k < V; k++) {
         double acc = P[i * V] * A[k];
 #pragma unroll
         for (int m = 1; m < V; m++) acc = dfma(P[i * V + m], A[m * V + k], acc);
-        PA[i * V + k] = acc;
+        row[k] = acc;
       }
 #pragma unroll
-    for (int i = 0; i < V * V; i++) P[i] = PA[i];
-#pragma unroll
-    for (int i = 0; i < V; i++) P[i * V + i] = P[i * V + i] + T_FLC[j];
+      for (int k = 0; k < V; k++) P[i * V + k] = row[k];
+      P[i * V + i] = P[i * V + i] + T_FLC[j];
+    }
   }
   // LU with partial pivoting (first largest pivot), static indices only
   int piv[V];
@@ -325,6 +326,12 @@ DEV void newton_node(const NodeCtx& c, const double (&q)[N][V], double (&G)[N][V
 }
 #endif
 
+// Persistent, work-queued: each wavefront claims cells from st->qhead (reset
+// by k_dt_finalize every step), iterates its CPW slots, and as soon as a slot's
+// cell has converged writes that cell's outputs and refills the slot with the
+// next unclaimed cell, so cells with many Newton iterations do not hold the
+// lanes of cells that need few.  Every cell's arithmetic is independent of
+// which slot or wavefront runs it.
 extern "C" __global__ void __launch_bounds__(64)
 k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
             double* __restrict__ tr, double* __restrict__ vol, double* __restrict__ qh,
@@ -335,30 +342,51 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
   const int lane = threadIdx.x;
   NodeCtx c;
   node_ctx(c, lane);
-  const long cell = (long)blockIdx.x * CPW + c.slot;
-  const bool valid = (c.slot < CPW) && (cell < g.nwc);
-  double w[V], q[N][V], rhs[N][V], dqs[N][D][V];
-  if (valid) {
-#pragma unroll
-    for (int v = 0; v < V; v++) w[v] = wh[(long)(c.idx * V + v) * g.nwc + cell];
-#pragma unroll
-    for (int t = 0; t < N; t++)
-#pragma unroll
-      for (int v = 0; v < V; v++)
-        q[t][v] = (PDE_GUESS && guess) ? guess[(long)((t * ND + c.idx) * V + v) * g.nwc + cell] : w[v];
-  }
+  const bool lane_ok = c.slot < CPW;
   const unsigned long long cellmask =
-      (ND == 64 ? ~0ull : ((1ull << ND) - 1ull)) << (c.slot < CPW ? c.slot * ND : 0);
+      (ND == 64 ? ~0ull : ((1ull << ND) - 1ull)) << (lane_ok ? c.slot * ND : 0);
 #if PDE_STIFF
   const int maxit = newton_maxit;
 #else
   const int maxit = DG_IT;
 #endif
-  bool active = valid;
+  double w[V], q[N][V], rhs[N][V], dqs[N][D][V];
+  long cell = -1;
+  bool active = false;
   int it = 0;
-  int learned = (PDE_GUESS && guess) ? 1 : 0;
+  int learned = 0;
   int fellback = 0;
   double r0 = 0.0;
+  // per-lane statistics over the cells this lane's slot has run
+  unsigned long long s_iters = 0ull, s_cells = 0ull, s_cap = 0ull, s_fb = 0ull;
+  unsigned int s_max = 0u;
+
+  auto start_cell = [&](long cl) {
+    cell = cl;
+    active = cl >= 0;
+    it = 0;
+    fellback = 0;
+    r0 = 0.0;
+    learned = (PDE_GUESS && guess) ? 1 : 0;
+    if (active) {
+#pragma unroll
+      for (int v = 0; v < V; v++) w[v] = wh[(long)(c.idx * V + v) * g.nwc + cl];
+#pragma unroll
+      for (int t = 0; t < N; t++)
+#pragma unroll
+        for (int v = 0; v < V; v++)
+          q[t][v] = (PDE_GUESS && guess) ? guess[(long)((t * ND + c.idx) * V + v) * g.nwc + cl]
+                                         : w[v];
+    }
+  };
+  {
+    long long base = 0;
+    if (lane == 0) base = (long long)atomicAdd(&st->qhead, (unsigned long long)CPW);
+    base = __shfl(base, 0, 64);
+    long cl = (long)base + c.slot;
+    start_cell((lane_ok && cl < g.nwc) ? cl : -1);
+  }
+
   while (__ballot(active) != 0ull) {
     if (active) {
 #pragma unroll
@@ -380,7 +408,7 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
         for (int v = 0; v < V; v++) {
           double acc = (T_DGMAT[t][0] * c.WP) * q[0][v];
 #pragma unroll
-          for (int s = 1; s < N; s++) acc = dfma(T_DGMAT[t][s] * c.WP, q[s][v], acc);
+          for (int s2 = 1; s2 < N; s2++) acc = dfma(T_DGMAT[t][s2] * c.WP, q[s2][v], acc);
           rhs[t][v] = rhs[t][v] - acc;
           double a = dabs(rhs[t][v]);
           rmax = (a > rmax || a != a) ? a : rmax;  // keeps a NaN
@@ -396,7 +424,7 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
       double cm = 0.0;
 #pragma unroll
       for (int i = 0; i < ND; i++) {
-        double e = sm.red[(c.slot < CPW ? c.slot : 0) * ND + i];
+        double e = sm.red[(lane_ok ? c.slot : 0) * ND + i];
         cm = (e > cm || e != e) ? e : cm;  // max that keeps a NaN
       }
       if (active && learned) {
@@ -437,7 +465,7 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
         for (int v = 0; v < V; v++) {
           double acc = T_DGINV[t][0] * rhs[0][v];
 #pragma unroll
-          for (int s = 1; s < N; s++) acc = dfma(T_DGINV[t][s], rhs[s][v], acc);
+          for (int s2 = 1; s2 < N; s2++) acc = dfma(T_DGINV[t][s2], rhs[s2][v], acc);
           double q1 = acc * c.IWP;
           if (dabs(q1 - q[t][v]) > DG_TOL * (1.0 + dabs(q[t][v]))) more = true;
           q[t][v] = q1;
@@ -445,107 +473,126 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
     }
 #endif
     const unsigned long long mb = __ballot(more);
+    bool fin = false;
     if (active) {
       it++;
-      if ((mb & cellmask) == 0ull || it >= maxit) active = false;
+      if ((mb & cellmask) == 0ull || it >= maxit) {
+        active = false;
+        fin = true;
+      }
     }
     __syncthreads();
-  }
-  // ---- epilogue: traces, volume term, optional full qh
-  if (valid) {
-#pragma unroll
-    for (int t = 0; t < N; t++)
-#pragma unroll
-      for (int v = 0; v < V; v++) sm.q[c.slot][t][c.idx][v] = q[t][v];
-  }
-  __syncthreads();
-  if (valid) {
-    static_for<0, D>([&](auto dc) {
-      constexpr int d = decltype(dc)::value;
-      constexpr int stride = ipow_c(N, D - 1 - d);
-      if (c.kd[d] == 0) {
-        const int j = (c.idx / (stride * N)) * stride + c.idx % stride;
-#pragma unroll
-        for (int e = 0; e < 2; e++)
-#pragma unroll
-          for (int t = 0; t < N; t++)
-#pragma unroll
-            for (int v = 0; v < V; v++) {
-              double acc = (e ? T_END1[0] : T_END0[0]) * sm.q[c.slot][t][c.idx][v];
-#pragma unroll
-              for (int m = 1; m < N; m++)
-                acc = dfma(e ? T_END1[m] : T_END0[m], sm.q[c.slot][t][c.idx + m * stride][v], acc);
-              const int plane = (((d * 2 + e) * N + t) * ND1 + j) * V + v;
-              tr[(long)plane * g.nwc + cell] = acc;
-            }
-      }
-    });
-    if (qh) {
+    if (__ballot(fin) == 0ull) continue;
+
+    // ---- finished slots: traces, volume term, optional full qh
+    if (fin) {
 #pragma unroll
       for (int t = 0; t < N; t++)
 #pragma unroll
-        for (int v = 0; v < V; v++) qh[(long)((t * ND + c.idx) * V + v) * g.nwc + cell] = q[t][v];
+        for (int v = 0; v < V; v++) sm.q[c.slot][t][c.idx][v] = q[t][v];
     }
-  }
-#if PDE_USE_S || PDE_USE_B
-  // volume term (fv.cpp:33-85): per node sum over t, then over nodes in order
-  double part[V];
-  if (valid) {
-#pragma unroll
-    for (int t = 0; t < N; t++) {
-      double s[V];
-      callS(s, q[t]);
-#if PDE_USE_B
+    __syncthreads();
+    if (fin) {
       static_for<0, D>([&](auto dc) {
         constexpr int d = decltype(dc)::value;
-        double dqd[V], b[V * V], bd[V];
-        lds_deriv<d>(c, sm.q, t, g.dX[d], dqd);
-        B(b, q[t], d);
-        b_times<PDE_B_QUIRK ? 2 : 0>(b, dqd, bd);
+        constexpr int stride = ipow_c(N, D - 1 - d);
+        if (c.kd[d] == 0) {
+          const int j = (c.idx / (stride * N)) * stride + c.idx % stride;
 #pragma unroll
-        for (int v = 0; v < V; v++) s[v] = s[v] - bd[v];
+          for (int e = 0; e < 2; e++)
+#pragma unroll
+            for (int t = 0; t < N; t++)
+#pragma unroll
+              for (int v = 0; v < V; v++) {
+                double acc = (e ? T_END1[0] : T_END0[0]) * sm.q[c.slot][t][c.idx][v];
+#pragma unroll
+                for (int m = 1; m < N; m++)
+                  acc = dfma(e ? T_END1[m] : T_END0[m], sm.q[c.slot][t][c.idx + m * stride][v], acc);
+                const int plane = (((d * 2 + e) * N + t) * ND1 + j) * V + v;
+                tr[(long)plane * g.nwc + cell] = acc;
+              }
+        }
       });
+      if (qh) {
+#pragma unroll
+        for (int t = 0; t < N; t++)
+#pragma unroll
+          for (int v = 0; v < V; v++) qh[(long)((t * ND + c.idx) * V + v) * g.nwc + cell] = q[t][v];
+      }
+    }
+#if PDE_USE_S || PDE_USE_B
+    // volume term (fv.cpp:33-85): per node sum over t, then over nodes in order
+    double part[V];
+    if (fin) {
+#pragma unroll
+      for (int t = 0; t < N; t++) {
+        double sv[V];
+        callS(sv, q[t]);
+#if PDE_USE_B
+        static_for<0, D>([&](auto dc) {
+          constexpr int d = decltype(dc)::value;
+          double dqd[V], b[V * V], bd[V];
+          lds_deriv<d>(c, sm.q, t, g.dX[d], dqd);
+          B(b, q[t], d);
+          b_times<PDE_B_QUIRK ? 2 : 0>(b, dqd, bd);
+#pragma unroll
+          for (int v = 0; v < V; v++) sv[v] = sv[v] - bd[v];
+        });
 #endif
-      double tmp = dt * T_W[t];
+        double tmp = dt * T_W[t];
 #pragma unroll
-      for (int d = 0; d < D; d++) tmp = tmp * c.wk[d];
+        for (int d = 0; d < D; d++) tmp = tmp * c.wk[d];
 #pragma unroll
-      for (int v = 0; v < V; v++) part[v] = (t == 0) ? tmp * s[v] : dfma(tmp, s[v], part[v]);
+        for (int v = 0; v < V; v++) part[v] = (t == 0) ? tmp * sv[v] : dfma(tmp, sv[v], part[v]);
+      }
+    }
+    __syncthreads();
+    if (fin) {
+#pragma unroll
+      for (int v = 0; v < V; v++) sm.q[c.slot][0][c.idx][v] = part[v];
+    }
+    __syncthreads();
+    if (fin && c.idx == 0) {
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        double acc = sm.q[c.slot][0][0][v];
+#pragma unroll
+        for (int i = 1; i < ND; i++) acc = acc + sm.q[c.slot][0][i][v];
+        vol[(long)v * g.nwc + cell] = acc;
+      }
+    }
+    __syncthreads();
+#endif
+    if (fin && c.idx == 0) {
+      s_iters += (unsigned long long)it;
+      s_cells += 1ull;
+      s_cap += (it >= maxit) ? 1ull : 0ull;
+      s_fb += (unsigned long long)fellback;
+      s_max = (unsigned int)it > s_max ? (unsigned int)it : s_max;
+    }
+    // ---- refill finished slots with the next unclaimed cells, in slot order
+    const unsigned long long lead = __ballot(fin && c.idx == 0);
+    long long base = 0;
+    if (lane == 0) base = (long long)atomicAdd(&st->qhead, (unsigned long long)__popcll(lead));
+    base = __shfl(base, 0, 64);
+    if (fin) {
+      const unsigned long long below = lead & ((1ull << (c.slot * ND)) - 1ull);
+      long cl = (long)base + __popcll(below);
+      start_cell(cl < g.nwc ? cl : -1);
     }
   }
-  __syncthreads();
-  if (valid) {
-#pragma unroll
-    for (int v = 0; v < V; v++) sm.q[c.slot][0][c.idx][v] = part[v];
-  }
-  __syncthreads();
-  if (valid && c.idx == 0) {
-#pragma unroll
-    for (int v = 0; v < V; v++) {
-      double acc = sm.q[c.slot][0][0][v];
-#pragma unroll
-      for (int i = 1; i < ND; i++) acc = acc + sm.q[c.slot][0][i][v];
-      vol[(long)v * g.nwc + cell] = acc;
-    }
-  }
-#endif
-  // ---- statistics (one contribution per cell)
-  unsigned long long mine = (valid && c.idx == 0) ? (unsigned long long)it : 0ull;
-  unsigned long long cnt = (valid && c.idx == 0) ? 1ull : 0ull;
-  unsigned long long cap = (valid && c.idx == 0 && it >= maxit) ? 1ull : 0ull;
-  unsigned long long fb = (valid && c.idx == 0) ? (unsigned long long)fellback : 0ull;
-  unsigned int mx = (valid && c.idx == 0) ? (unsigned int)it : 0u;
-  mine = wave_sum_u64(mine);
-  cnt = wave_sum_u64(cnt);
-  cap = wave_sum_u64(cap);
-  fb = wave_sum_u64(fb);
-  mx = wave_max_u32(mx);
-  if (lane == 0) {
-    atomicAdd(&st->pred_iters, mine);
-    atomicAdd(&st->pred_cells, cnt);
-    if (cap) atomicAdd(&st->pred_capped, cap);
-    if (fb) atomicAdd(&st->pred_fallbacks, fb);
-    atomicMax(&st->pred_maxit, mx);
+  // ---- statistics
+  s_iters = wave_sum_u64(s_iters);
+  s_cells = wave_sum_u64(s_cells);
+  s_cap = wave_sum_u64(s_cap);
+  s_fb = wave_sum_u64(s_fb);
+  s_max = wave_max_u32(s_max);
+  if (lane == 0 && s_cells) {
+    atomicAdd(&st->pred_iters, s_iters);
+    atomicAdd(&st->pred_cells, s_cells);
+    if (s_cap) atomicAdd(&st->pred_capped, s_cap);
+    if (s_fb) atomicAdd(&st->pred_fallbacks, s_fb);
+    atomicMax(&st->pred_maxit, s_max);
   }
 }
 

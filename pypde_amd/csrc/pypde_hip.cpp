@@ -75,6 +75,7 @@ struct StepState {
   unsigned int nan_flag, pad2;
   unsigned long long pred_iters, pred_cells, pred_capped, pred_fallbacks;
   unsigned int pred_maxit, pad3;
+  unsigned long long qhead;
 };
 
 struct FaceBufs {
@@ -263,6 +264,9 @@ struct pypde_ctx {
   MLPW model;
   double* gbuf = nullptr;   // per-step MLP guesses
   double* qh = nullptr;     // full predictor output (harvest)
+  int n_cu = 0;             // compute units of the device
+  hipFunction_t pred_fn = nullptr;
+  long pred_blocks = 0;     // persistent predictor workgroups for pred_fn
 };
 
 namespace {
@@ -290,6 +294,27 @@ int launch(pypde_ctx* c, int k, dim3 grid, dim3 block, std::vector<void*>& args)
 }
 
 long cdiv(long a, long b) { return (a + b - 1) / b; }
+
+// Workgroups of the persistent predictor: as many as can be resident at once
+// (occupancy x compute units; PYPDE_PRED_BLOCKS overrides), never more than
+// there are groups of CPW cells.
+long predictor_blocks(pypde_ctx* c) {
+  if (c->pred_fn != c->fn[K_PRED]) {
+    c->pred_fn = c->fn[K_PRED];
+    int per_cu = 0;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, c->pred_fn, 64, 0) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 4;
+    if (c->n_cu < 1) c->n_cu = 256;
+    c->pred_blocks = (long)per_cu * c->n_cu;
+    if (const char* e = getenv("PYPDE_PRED_BLOCKS")) {
+      long v = atol(e);
+      if (v > 0) c->pred_blocks = v;
+    }
+  }
+  long groups = cdiv(c->g.nwc, c->CPW);
+  return c->pred_blocks < groups ? c->pred_blocks : groups;
+}
 
 // One timestep: boundaries + WENO, CFL reduction, predictor, face fluxes,
 // corrector, bookkeeping (iterator.cpp:99-148).  Every kernel returns at once
@@ -345,7 +370,7 @@ int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
     double tol = c->opts.newton_tol;
     int maxit = c->opts.newton_maxit;
     std::vector<void*> a = {&w, &gs, &t, &v, &qh, &st, &g, &tol, &maxit};
-    if ((rc = launch(c, K_PRED, dim3((unsigned)cdiv(g.nwc, c->CPW)), dim3(64), a))) return rc;
+    if ((rc = launch(c, K_PRED, dim3((unsigned)predictor_blocks(c)), dim3(64), a))) return rc;
   }
   mark();
   for (int d = 0; d < c->D; d++) {
@@ -548,6 +573,9 @@ pypde_ctx* pypde_create(const char* src, bool useF, bool useB, bool useS, int* n
   };
   if (hipSetDevice(c->device) != hipSuccess)
     return bail(fail(PYPDE_ERR_HIP, "hipSetDevice(%d) failed: no usable GPU", c->device));
+  if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, c->device) !=
+      hipSuccess)
+    c->n_cu = 0;
   if (load_module(c, code, &c->mod, c->fn_plain)) return bail(PYPDE_ERR_HIP);
   memcpy(c->fn, c->fn_plain, sizeof c->fn);
   if (hipStreamCreate(&c->stream) != hipSuccess)

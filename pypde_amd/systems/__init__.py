@@ -57,6 +57,25 @@ def sod1d(n=200, gamma=1.4):
     return u
 
 
+def density_wave(n, ndim, t=0.0, amp=0.2, vel=0.5, p=1.0, gamma=1.4):
+    """Smooth Euler solution for convergence studies: rho = 1 + amp sin(2 pi
+    (sum_d x_d - ndim vel t)) advected at constant velocity vel (every axis)
+    and pressure p on the periodic unit square/line.  Returns exact cell
+    averages (n^ndim cells) of (rho, rho u_d..., E)."""
+    k = 2 * np.pi
+    e = (np.arange(n + 1) / n) - vel * t      # shifted cell edges along each axis
+    if ndim == 1:
+        s = -(np.cos(k * e[1:]) - np.cos(k * e[:-1])) / k * n
+    else:
+        a, b = e[:, None], e[None, :]
+        S = lambda x, y: np.sin(k * (x + y))  # noqa: E731
+        s = -(S(a[1:], b[:, 1:]) - S(a[:-1], b[:, 1:]) - S(a[1:], b[:, :-1])
+              + S(a[:-1], b[:, :-1])) / k ** 2 * n * n
+    rho = 1.0 + amp * s
+    comps = [rho] + [vel * rho] * ndim + [p / (gamma - 1) + 0.5 * ndim * vel * vel * rho]
+    return np.ascontiguousarray(np.stack(comps, -1))
+
+
 def splitmix64(seed, count):
     """`count` uniforms in [0, 1) from the SplitMix64 stream of `seed`:
     x_i = seed + (i + 1) * 0x9E3779B97F4A7C15, mixed, top 53 bits scaled."""

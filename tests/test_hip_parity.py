@@ -233,3 +233,24 @@ def test_learned_guess_path_matches_oracle():
                                reflect=refl.ravel())
     # both runs stop at |dq| <= 1e-12 (1 + |q|); the states agree to that scale
     assert np.max(np.abs(uo - up)) <= 1e-9 * np.max(np.abs(up))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N,lo", [(3, 2.9), (4, 3.7)])
+def test_convergence_order_on_smooth_wave_gpu(N, lo):
+    """C1-style 2D periodic Euler on the GPU: an advected density wave (exact
+    cell averages) converges at the scheme's order N from 32 to 128 cells per
+    side, and the 32-cell run equals the oracle's."""
+    errs = []
+    for n in (32, 64, 128):
+        u0 = systems.density_wave(n, 2)
+        ug, _, sg = gpu_run("euler2d", u0, 0.1, [1.0, 1.0], "periodic", N, False)
+        assert sg["rc"] == 0
+        if n == 32:
+            uo, _, st, rc = oracle_run("euler2d", u0, 0.1, [1.0, 1.0], 1, N, False)
+            assert relerr(ug, uo) <= TOL
+        ex = systems.density_wave(n, 2, t=0.1)
+        errs.append(np.mean(np.abs(ug[..., 0] - ex[..., 0])))
+    orders = [np.log2(errs[i] / errs[i + 1]) for i in range(2)]
+    print(f"gpu N={N}: L1 errors {errs}, orders {orders}")
+    assert orders[-1] >= lo and orders[0] >= lo - 0.5

@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import cpu_checkers as cc
+from pypde_amd import systems
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -458,3 +459,21 @@ def test_reflective_equals_mirrored_periodic():
         # mirrored order, and the finite-difference wave speeds amplify that
         ref = ub[:nx, :ny]
         assert np.max(np.abs(uh - ref)) <= tol * np.max(np.abs(ref)), tf
+
+
+@pytest.mark.parametrize("N,ndim,tf,lo", [(3, 1, 0.2, 2.9), (4, 1, 0.2, 3.7), (3, 2, 0.1, 2.9)])
+def test_convergence_order_on_smooth_wave(N, ndim, tf, lo):
+    """Empirical order of the ADER-WENO scheme on an advected density wave
+    (exact cell averages at start and end): the L1 error of rho falls like
+    dx^N between 16, 32 and 64 cells per side."""
+    errs = []
+    for n in (16, 32, 64):
+        u0 = systems.density_wave(n, ndim)
+        uf, _, st, rc = cc.oracle_solve("euler1d" if ndim == 1 else "euler2d", u0, tf,
+                                        [1.0] * ndim, 1, N)
+        assert rc == 0
+        ex = systems.density_wave(n, ndim, t=tf)
+        errs.append(np.mean(np.abs(uf[..., 0] - ex[..., 0])))
+    orders = [np.log2(errs[i] / errs[i + 1]) for i in range(2)]
+    print(f"N={N} ndim={ndim}: L1 errors {errs}, orders {orders}")
+    assert orders[-1] >= lo
