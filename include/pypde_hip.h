@@ -40,6 +40,8 @@ typedef struct {
   int profile;              /* time every kernel with HIP events [0] */
   const double *guess;      /* host array: predictor initial guesses for the first step,
                                nwc x N^(ndim+1) x V (testing the learned-guess path) */
+  int use_graph;            /* capture one timestep in a HIP graph and replay it [0]
+                               (ignored while profiling) */
 } pypde_options;
 
 typedef struct {
@@ -92,6 +94,24 @@ int pypde_step(pypde_ctx *c, long steps, double tf, pypde_stats *stats);
 void *pypde_device_state(pypde_ctx *c);
 void *pypde_stream(pypde_ctx *c);
 void pypde_destroy(pypde_ctx *c);
+
+/* Ensembles: n independent grids of one system (same ndim, order, nvar,
+ * physics and boundary types), grid i with nX[i*ndim ..] cells and spacing
+ * dX[i*ndim ..], each with its own dt.  Every timestep of all members is one
+ * HIP graph launch (each member on its own stream inside the graph), so small
+ * grids run side by side.  No reference counterpart: the reference solves
+ * one grid per pde_solver call (api.cpp:9-30). */
+typedef struct pypde_ensemble pypde_ensemble;
+pypde_ensemble *pypde_ensemble_create(int n, const char *src, bool useF, bool useB, bool useS,
+                                      const int *nX, int ndim, const double *dX, double CFL,
+                                      const int *boundaryTypes, bool STIFF, int FLUX, int N, int V,
+                                      const pypde_options *opts, int *rc);
+int pypde_ensemble_set_state(pypde_ensemble *e, int i, const double *u);
+int pypde_ensemble_get_state(pypde_ensemble *e, int i, double *u);
+/* Advance every member from t = 0 to tf (or max_steps); stats: n entries or NULL. */
+int pypde_ensemble_run(pypde_ensemble *e, double tf, long max_steps, pypde_stats *stats,
+                       double *ms_loop);
+void pypde_ensemble_destroy(pypde_ensemble *e);
 
 /* Learned Newton initial guess (no counterpart in the reference; its hook is
  * DGSolver::initial_guess, dg.cpp:12-20,216).  A 3-layer SiLU MLP, weights as

@@ -18,7 +18,7 @@ class Options(C.Structure):
     _fields_ = [("struct_size", C.c_int), ("newton_tol", C.c_double),
                 ("newton_maxit", C.c_int), ("max_steps", C.c_long), ("reflect", PD),
                 ("b_quirk", C.c_int), ("use_M", C.c_int), ("device", C.c_int),
-                ("profile", C.c_int), ("guess", PD)]
+                ("profile", C.c_int), ("guess", PD), ("use_graph", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -77,6 +77,14 @@ def lib():
         L.pypde_hip_compile.argtypes = [C.c_char_p, C.c_bool, C.c_bool, C.c_bool, C.c_int,
                                         C.c_int, C.c_int, C.c_bool, C.POINTER(Options)]
         L.pypde_hip_tables.argtypes = [C.c_int] + [PD] * 10
+        L.pypde_ensemble_create.restype = C.c_void_p
+        L.pypde_ensemble_create.argtypes = [C.c_int, C.c_char_p, C.c_bool, C.c_bool, C.c_bool, PI,
+                                            C.c_int, PD, C.c_double, PI, C.c_bool, C.c_int,
+                                            C.c_int, C.c_int, C.POINTER(Options), PI]
+        L.pypde_ensemble_set_state.argtypes = [C.c_void_p, C.c_int, PD]
+        L.pypde_ensemble_get_state.argtypes = [C.c_void_p, C.c_int, PD]
+        L.pypde_ensemble_run.argtypes = [C.c_void_p, C.c_double, C.c_long, C.POINTER(Stats), PD]
+        L.pypde_ensemble_destroy.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -102,7 +110,7 @@ def ip(a):
 
 
 def make_options(newton_tol=1e-12, newton_maxit=100, max_steps=0, reflect=None, b_quirk=0,
-                 use_M=False, device=0, profile=False, guess=None):
+                 use_M=False, device=0, profile=False, guess=None, use_graph=False):
     o = Options()
     lib().pypde_default_options(C.byref(o))
     o.newton_tol = newton_tol
@@ -112,6 +120,7 @@ def make_options(newton_tol=1e-12, newton_maxit=100, max_steps=0, reflect=None, 
     o.use_M = int(bool(use_M))
     o.device = device
     o.profile = int(bool(profile))
+    o.use_graph = int(bool(use_graph))
     keep = []
     if reflect is not None:
         r = np.ascontiguousarray(reflect, dtype=np.float64).ravel()

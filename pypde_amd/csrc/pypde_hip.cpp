@@ -637,28 +637,65 @@ int pypde_get_state(pypde_ctx* c, double* u) {
 void* pypde_device_state(pypde_ctx* c) { return c->u; }
 void* pypde_stream(pypde_ctx* c) { return c->stream; }
 
+// One timestep of each context captured into a graph: a fork event on the
+// first context's stream, every context's step on its own stream, joined back.
+static int capture_steps(const std::vector<pypde_ctx*>& cs, hipGraphExec_t* exec) {
+  hipStream_t s0 = cs[0]->stream;
+  std::vector<hipEvent_t> evs(cs.size() + 1, nullptr);
+  for (auto& e : evs) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  int rc = 0;
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+  hipEventRecord(evs[0], s0);
+  for (size_t i = 0; i < cs.size() && !rc; i++) {
+    if (i > 0) hipStreamWaitEvent(cs[i]->stream, evs[0], 0);
+    rc = enqueue_step(cs[i], false, nullptr);
+    if (!rc && i > 0) {
+      hipEventRecord(evs[i + 1], cs[i]->stream);
+      hipStreamWaitEvent(s0, evs[i + 1], 0);
+    }
+  }
+  hipError_t e = hipStreamEndCapture(s0, &graph);
+  if (!rc && e != hipSuccess) rc = fail(PYPDE_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+  if (!rc && hipGraphInstantiate(exec, graph, nullptr, nullptr, 0) != hipSuccess)
+    rc = fail(PYPDE_ERR_HIP, "graph instantiation failed");
+  if (graph) hipGraphDestroy(graph);
+  for (auto ev : evs) hipEventDestroy(ev);
+  return rc;
+}
+
 static int run_loop(pypde_ctx* c, long max_steps, pypde_stats* stats, bool first_guess) {
   auto t0 = std::chrono::steady_clock::now();
   int chunk = 4;
   bool guess_now = first_guess;
   std::vector<hipEvent_t> evs;
+  hipGraphExec_t exec = nullptr;
+  const bool graph = c->opts.use_graph && !c->opts.profile;
+  int rc = 0;
   while (true) {
     // never queue past a step cap: the launch count equals the step count
     if (max_steps > 0) {
       long left = max_steps - (long)c->host_st.count;
       if (left < chunk) chunk = (int)std::max(1L, left);
     }
-    for (int i = 0; i < chunk; i++) {
-      std::vector<hipEvent_t>* ep = c->opts.profile ? &evs : nullptr;
-      int rc = enqueue_step(c, guess_now, ep);
-      if (rc) return rc;
+    for (int i = 0; i < chunk && !rc; i++) {
+      if (graph && !guess_now) {
+        if (!exec && (rc = capture_steps({c}, &exec))) break;
+        if (hipGraphLaunch(exec, c->stream) != hipSuccess) rc = fail(PYPDE_ERR_HIP, "graph launch failed");
+      } else {
+        std::vector<hipEvent_t>* ep = c->opts.profile ? &evs : nullptr;
+        rc = enqueue_step(c, guess_now, ep);
+      }
       guess_now = false;
     }
+    if (rc) break;
     HIPCHK(hipMemcpyAsync(&c->host_st, c->st, sizeof(StepState), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (c->host_st.done) break;
     if (chunk < 64) chunk *= 2;
   }
+  if (exec) hipGraphExecDestroy(exec);
+  if (rc) return rc;
   auto t1 = std::chrono::steady_clock::now();
   if (c->opts.profile) {
     // 9 marks per step bracket: weno | dt | guess | predictor | flux0 | flux1 | update | advance
@@ -742,6 +779,122 @@ int pypde_step(pypde_ctx* c, long steps, double tf, pypde_stats* stats) {
   int rc = run_loop(c, s.max_steps, stats, false);
   c->ret = keep_ret;
   return rc;
+}
+
+// ------------------------------------------------------------ ensembles
+struct pypde_ensemble {
+  std::vector<pypde_ctx*> m;
+};
+
+void pypde_ensemble_destroy(pypde_ensemble* e) {
+  if (!e) return;
+  for (auto c : e->m) pypde_destroy(c);
+  delete e;
+}
+
+pypde_ensemble* pypde_ensemble_create(int n, const char* src, bool useF, bool useB, bool useS,
+                                      const int* nX, int ndim, const double* dX, double CFL,
+                                      const int* boundaryTypes, bool STIFF, int FLUX, int N, int V,
+                                      const pypde_options* opts, int* rc_out) {
+  int dummy;
+  int& rc = rc_out ? *rc_out : dummy;
+  if (n < 1 || !nX || !dX || !boundaryTypes || ndim < 1 || ndim > 2) {
+    rc = fail(PYPDE_ERR_ARG, "ensemble: n >= 1, nX, dX, boundaryTypes and ndim 1..2 required");
+    return nullptr;
+  }
+  if (opts && opts->guess) {
+    rc = fail(PYPDE_ERR_ARG, "ensemble: per-member guesses are not supported");
+    return nullptr;
+  }
+  auto* e = new pypde_ensemble();
+  for (int i = 0; i < n; i++) {
+    int nx[2] = {nX[i * ndim], ndim > 1 ? nX[i * ndim + 1] : 1};
+    double dx[2] = {dX[i * ndim], ndim > 1 ? dX[i * ndim + 1] : 1.0};
+    int bc[2] = {boundaryTypes[0], ndim > 1 ? boundaryTypes[1] : 0};
+    pypde_ctx* c = pypde_create(src, useF, useB, useS, nx, ndim, dx, CFL, bc, STIFF, FLUX, N, V,
+                                false, opts, &rc);
+    if (!c) {
+      pypde_ensemble_destroy(e);
+      return nullptr;
+    }
+    e->m.push_back(c);
+  }
+  rc = 0;
+  return e;
+}
+
+int pypde_ensemble_set_state(pypde_ensemble* e, int i, const double* u) {
+  if (!e || i < 0 || i >= (int)e->m.size()) return fail(PYPDE_ERR_ARG, "ensemble member %d", i);
+  return pypde_set_state(e->m[i], u);
+}
+
+int pypde_ensemble_get_state(pypde_ensemble* e, int i, double* u) {
+  if (!e || i < 0 || i >= (int)e->m.size()) return fail(PYPDE_ERR_ARG, "ensemble member %d", i);
+  return pypde_get_state(e->m[i], u);
+}
+
+int pypde_ensemble_run(pypde_ensemble* e, double tf, long max_steps, pypde_stats* stats,
+                       double* ms_loop) {
+  if (!e) return fail(PYPDE_ERR_ARG, "null ensemble");
+  pypde_ctx* c0 = e->m[0];
+  HIPCHK(hipSetDevice(c0->device));
+  for (auto c : e->m) {
+    StepState s;
+    memset(&s, 0, sizeof s);
+    s.tf = tf;
+    s.cfl = c->host_st.cfl;
+    s.max_steps = max_steps;
+    s.done = (0.0 < tf) ? 0 : 1;
+    c->host_st = s;
+    c->g.ndt = 0;
+    c->ret = nullptr;
+    HIPCHK(hipMemcpyAsync(c->st, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  hipGraphExec_t exec = nullptr;
+  int rc = capture_steps(e->m, &exec);
+  int chunk = 4;
+  long launched = 0;
+  while (!rc) {
+    if (max_steps > 0 && max_steps - launched < chunk) chunk = (int)std::max(1L, max_steps - launched);
+    for (int i = 0; i < chunk && !rc; i++)
+      if (hipGraphLaunch(exec, c0->stream) != hipSuccess) rc = fail(PYPDE_ERR_HIP, "graph launch failed");
+    launched += chunk;
+    if (rc) break;
+    bool all = true;
+    for (auto c : e->m)
+      if (hipMemcpyAsync(&c->host_st, c->st, sizeof(StepState), hipMemcpyDeviceToHost, c0->stream) !=
+          hipSuccess)
+        rc = fail(PYPDE_ERR_HIP, "state readback failed");
+    if (hipStreamSynchronize(c0->stream) != hipSuccess) rc = fail(PYPDE_ERR_HIP, "ensemble step failed");
+    for (auto c : e->m) all = all && c->host_st.done;
+    if (all) break;
+    if (chunk < 64) chunk *= 2;
+  }
+  if (exec) hipGraphExecDestroy(exec);
+  auto t1 = std::chrono::steady_clock::now();
+  if (rc) return rc;
+  if (ms_loop) *ms_loop = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  int nan = 0;
+  for (size_t i = 0; i < e->m.size(); i++) {
+    const StepState& h = e->m[i]->host_st;
+    if (h.nan_step && !nan) nan = h.nan_step;
+    if (stats) {
+      pypde_stats* o = stats + i;
+      o->steps = h.count;
+      o->t = h.t;
+      o->pred_iters = (long)h.pred_iters;
+      o->pred_cells = (long)h.pred_cells;
+      o->pred_maxit = (long)h.pred_maxit;
+      o->pred_capped = (long)h.pred_capped;
+      o->guess_fallbacks = (long)h.pred_fallbacks;
+      o->nan_step = h.nan_step;
+      o->ms_loop = std::chrono::duration<double, std::milli>(t1 - t0).count();
+      for (int k = 0; k < 8; k++) o->ms_kernel[k] = 0.0;
+    }
+  }
+  return nan ? -nan : 0;
 }
 
 int pde_solver_hip(const char* src, bool useF, bool useB, bool useS, double* u, double tf,

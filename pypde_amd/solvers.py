@@ -113,7 +113,7 @@ class Solver:
 
     def __init__(self, u0, L, system, ndim=None, order=3, boundaryTypes="transitive", cfl=0.9,
                  stiff=False, reflect=None, newton_tol=1e-12, newton_maxit=100, b_quirk=False,
-                 profile=False, device=0, guess=None):
+                 profile=False, device=0, guess=None, use_graph=False):
         u0 = np.ascontiguousarray(u0, dtype=np.float64)
         self.shape = u0.shape
         self.nX = np.array(u0.shape[:-1], dtype=np.int32)
@@ -126,7 +126,8 @@ class Solver:
             reflect = np.ones((nd, V))
         self.opts = _capi.make_options(newton_tol=newton_tol, newton_maxit=newton_maxit,
                                        reflect=reflect, b_quirk=int(b_quirk), use_M=useM,
-                                       profile=profile, device=device, guess=guess)
+                                       profile=profile, device=device, guess=guess,
+                                       use_graph=use_graph)
         rc = C.c_int(0)
         self.h = _capi.lib().pypde_create(src.encode(), useF, useB, useS, _capi.ip(self.nX), nd,
                                           _capi.dp(self.dX), float(cfl), _capi.ip(self.bcs),
@@ -168,6 +169,67 @@ class Solver:
     def close(self):
         if getattr(self, "h", None):
             _capi.lib().pypde_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Ensemble:
+    """Independent grids of one system advanced together: every timestep of
+    every member is one HIP graph launch (members side by side on their own
+    streams), each member with its own dt.  ``grids``: list of initial states,
+    each of shape (n_x, [n_y,] V); ``L``: list of domain lengths per member."""
+
+    def __init__(self, grids, L, system, order=3, boundaryTypes="transitive", cfl=0.9,
+                 stiff=False, reflect=None, newton_tol=1e-12, newton_maxit=100, device=0):
+        grids = [np.ascontiguousarray(g, dtype=np.float64) for g in grids]
+        self.shapes = [g.shape for g in grids]
+        nd = len(self.shapes[0]) - 1
+        V = self.shapes[0][-1]
+        if any(len(s) - 1 != nd or s[-1] != V for s in self.shapes):
+            raise ValueError("ensemble members must share ndim and nvar")
+        nX = np.array([s[:-1] for s in self.shapes], dtype=np.int32).ravel()
+        dX = np.array([[L[i][d] / self.shapes[i][d] for d in range(nd)]
+                       for i in range(len(grids))], dtype=np.float64).ravel()
+        bcs = parse_boundary_types(boundaryTypes, nd)
+        src, useF, useB, useS, useM = _source(None, None, None, None, system)
+        if reflect is None and 2 in bcs:
+            reflect = np.ones((nd, V))
+        self.opts = _capi.make_options(newton_tol=newton_tol, newton_maxit=newton_maxit,
+                                       reflect=reflect, use_M=useM, device=device)
+        rc = C.c_int(0)
+        self.h = _capi.lib().pypde_ensemble_create(len(grids), src.encode(), useF, useB, useS,
+                                                   _capi.ip(nX), nd, _capi.dp(dX), float(cfl),
+                                                   _capi.ip(bcs), bool(stiff), 0, int(order), V,
+                                                   C.byref(self.opts), C.byref(rc))
+        if not self.h:
+            _capi.check(rc.value or 3, "pypde_ensemble_create")
+        for i, g in enumerate(grids):
+            _capi.check(_capi.lib().pypde_ensemble_set_state(self.h, i, _capi.dp(g)), "set_state")
+
+    def run(self, tf, max_steps=0):
+        n = len(self.shapes)
+        st = (_capi.Stats * n)()
+        for s in st:
+            s.struct_size = C.sizeof(_capi.Stats)
+        ms = np.zeros(1)
+        rc = _capi.lib().pypde_ensemble_run(self.h, float(tf), int(max_steps), st, _capi.dp(ms))
+        _capi.check(rc, "ensemble_run")
+        out = [s.as_dict() for s in st]
+        return out, float(ms[0])
+
+    def state(self, i):
+        out = np.zeros(self.shapes[i])
+        _capi.check(_capi.lib().pypde_ensemble_get_state(self.h, i, _capi.dp(out)), "get_state")
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            _capi.lib().pypde_ensemble_destroy(self.h)
             self.h = None
 
     def __del__(self):

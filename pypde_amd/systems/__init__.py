@@ -105,6 +105,29 @@ def reactive_channel(nx=1024, ny=256, seed=0, gamma=1.2, Qc=25.0):
     return np.ascontiguousarray(np.stack([rho, z, z, E, rho * lam], -1))
 
 
+def riemann4(nx, ny, seed, gamma=1.4):
+    """Four-quadrant 2D Riemann problem on the unit square (x, y split at 1/2):
+    per quadrant rho ~ U(0.1, 2), u, v ~ U(-1, 1), p ~ U(0.1, 2) from the
+    SplitMix64 stream `seed`.  Conserved (rho, rho u, rho v, E)."""
+    r = splitmix64(seed, 16).reshape(4, 4)
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    qd = (x[:, None] >= 0.5).astype(int) * 2 + (y[None, :] >= 0.5).astype(int)
+    rho = 0.1 + 1.9 * r[qd, 0]
+    u = -1 + 2 * r[qd, 1]
+    v = -1 + 2 * r[qd, 2]
+    p = 0.1 + 1.9 * r[qd, 3]
+    E = p / (gamma - 1) + 0.5 * rho * (u * u + v * v)
+    return np.ascontiguousarray(np.stack([rho, rho * u, rho * v, E], -1))
+
+
+def ensemble_sizes(n=48, sizes=(64, 128, 256, 512), seed=7):
+    """n grid shapes (n_x, n_y), each side drawn uniformly from `sizes`."""
+    r = splitmix64(seed, 2 * n).reshape(n, 2)
+    k = np.minimum((r * len(sizes)).astype(int), len(sizes) - 1)
+    return [(sizes[a], sizes[b]) for a, b in k]
+
+
 REACTIVE_REFLECT = np.array([[1, -1, 1, 1, 1], [1, 1, -1, 1, 1]], dtype=np.float64)
 
 

@@ -254,3 +254,51 @@ def test_convergence_order_on_smooth_wave_gpu(N, lo):
     orders = [np.log2(errs[i] / errs[i + 1]) for i in range(2)]
     print(f"gpu N={N}: L1 errors {errs}, orders {orders}")
     assert orders[-1] >= lo and orders[0] >= lo - 0.5
+
+
+@pytest.mark.gpu
+def test_graph_replay_equals_eager():
+    """A run whose timestep is one replayed HIP graph gives the eager run's
+    bits (and the oracle's)."""
+    u0 = systems.euler2d_modes(40, seed=5)
+    L = [1.0, 1.0]
+    outs = []
+    for graph in (False, True):
+        s = Solver(u0, L, systems.source("euler2d"), order=3, boundaryTypes="periodic",
+                   use_graph=graph)
+        st = s.run(0.05)
+        outs.append((s.state(), st))
+        s.close()
+    (ue, se), (ugr, sgr) = outs
+    assert se["steps"] == sgr["steps"] and np.array_equal(ue, ugr)
+    uo, _, st, rc = oracle_run("euler2d", u0, 0.05, L, 1, 3, False)
+    assert relerr(ugr, uo) <= TOL
+
+
+@pytest.mark.gpu
+def test_ensemble_members_equal_single_runs():
+    """C4 shape: independent grids of different sizes with seeded 4-quadrant
+    Riemann data, advanced together (one graph launch per step, each grid
+    with its own dt); every member equals its own single-grid run bit for
+    bit, and the smallest ones equal the oracle."""
+    from pypde_amd.solvers import Ensemble
+    shapes = systems.ensemble_sizes(6, sizes=(16, 24, 32, 48))
+    grids = [systems.riemann4(nx, ny, seed=i) for i, (nx, ny) in enumerate(shapes)]
+    Ls = [[1.0, 1.0]] * len(grids)
+    ens = Ensemble(grids, Ls, systems.source("euler2d"), order=3)
+    stats, ms = ens.run(0.05)
+    outs = [ens.state(i) for i in range(len(grids))]
+    ens.close()
+    print("ensemble shapes", shapes, "steps", [s["steps"] for s in stats], f"{ms:.2f} ms")
+    for i, g in enumerate(grids):
+        s = Solver(g, Ls[i], systems.source("euler2d"), order=3)
+        st = s.run(0.05)
+        ui = s.state()
+        s.close()
+        assert stats[i]["steps"] == st["steps"]
+        assert np.array_equal(outs[i], ui), i
+    small = sorted(range(len(grids)), key=lambda i: grids[i].size)[:2]
+    for i in small:
+        uo, _, st, rc = oracle_run("euler2d", grids[i], 0.05, Ls[i], 0, 3, False)
+        assert rc == 0 and st.steps == stats[i]["steps"]
+        assert relerr(outs[i], uo) <= TOL
