@@ -40,6 +40,10 @@ NX, NY, ORDER, V, D = 1024, 256, 3, 5, 2
 L = [1.0, NY / NX]
 PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
+# HBM bytes per predictor launch on this workload, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs scaled by a 256 MiB copy kernel's readings
+# (profiles/reactive_channel_1024x256_kernels.md)
+PREDICTOR_HBM_BYTES = 863.3e6
 
 # Flop counts of the reactive system functions (pypde_amd/systems/reactive2d.pde),
 # counted by hand: + - * / sqrt each 1, fma 2; pde_exp = 33.
@@ -134,8 +138,8 @@ def predictor_roofline(st, K):
              + predictor_flops_epilogue() * st["pred_cells"])
     achieved = flops / (pred_ms * 1e-3) if pred_ms > 0 else 0.0
     return dict(bound="fp64", achieved=achieved / 1e12, peak=PEAK_FP64 / 1e12, unit="TFLOP/s",
-                frac=achieved / PEAK_FP64, traffic=None, kernel="k_predictor",
-                avg_launch_ms=pred_ms / K)
+                frac=achieved / PEAK_FP64, traffic=PREDICTOR_HBM_BYTES,
+                kernel="k_predictor_static", avg_launch_ms=pred_ms / K)
 
 
 def cpu_baseline(t_target, threads):

@@ -596,6 +596,232 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
   }
 }
 
+// Static schedule: one wavefront per group of CPW consecutive cells, which
+// stays until its slowest cell has converged.
+extern "C" __global__ void __launch_bounds__(64)
+k_predictor_static(const double* __restrict__ wh, const double* __restrict__ guess,
+            double* __restrict__ tr, double* __restrict__ vol, double* __restrict__ qh,
+            StepState* st, Geom g, double newton_tol, int newton_maxit) {
+  if (st->done) return;
+  __shared__ PredLDS sm;
+  const double dt = st->dt;
+  const int lane = threadIdx.x;
+  NodeCtx c;
+  node_ctx(c, lane);
+  const long cell = (long)blockIdx.x * CPW + c.slot;
+  const bool valid = (c.slot < CPW) && (cell < g.nwc);
+  double w[V], q[N][V], rhs[N][V], dqs[N][D][V];
+  if (valid) {
+#pragma unroll
+    for (int v = 0; v < V; v++) w[v] = wh[(long)(c.idx * V + v) * g.nwc + cell];
+#pragma unroll
+    for (int t = 0; t < N; t++)
+#pragma unroll
+      for (int v = 0; v < V; v++)
+        q[t][v] = (PDE_GUESS && guess) ? guess[(long)((t * ND + c.idx) * V + v) * g.nwc + cell] : w[v];
+  }
+  const unsigned long long cellmask =
+      (ND == 64 ? ~0ull : ((1ull << ND) - 1ull)) << (c.slot < CPW ? c.slot * ND : 0);
+#if PDE_STIFF
+  const int maxit = newton_maxit;
+#else
+  const int maxit = DG_IT;
+#endif
+  bool active = valid;
+  int it = 0;
+  int learned = (PDE_GUESS && guess) ? 1 : 0;
+  int fellback = 0;
+  double r0 = 0.0;
+  while (__ballot(active) != 0ull) {
+    if (active) {
+#pragma unroll
+      for (int t = 0; t < N; t++)
+#pragma unroll
+        for (int v = 0; v < V; v++) sm.q[c.slot][t][c.idx][v] = q[t][v];
+    }
+    __syncthreads();
+    node_rhs(c, sm, q, w, dt, g, active, rhs, dqs);
+    bool more = false;
+    bool skip = false;
+#if PDE_STIFF
+    // residual G = rhs - (DG_MAT (x) W) q   (dg.cpp:128-154)
+    double rmax = 0.0;
+    if (active) {
+#pragma unroll
+      for (int t = 0; t < N; t++)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          double acc = (T_DGMAT[t][0] * c.WP) * q[0][v];
+#pragma unroll
+          for (int s = 1; s < N; s++) acc = dfma(T_DGMAT[t][s] * c.WP, q[s][v], acc);
+          rhs[t][v] = rhs[t][v] - acc;
+          double a = dabs(rhs[t][v]);
+          rmax = (a > rmax || a != a) ? a : rmax;  // keeps a NaN
+        }
+    }
+#if PDE_GUESS
+    // while a cell is still on its learned guess, a residual that is not finite
+    // or larger than at the previous iterate sends it back to the plain guess
+    if (__ballot(active && learned) != 0ull) {
+      __syncthreads();
+      sm.red[lane] = rmax;
+      __syncthreads();
+      double cm = 0.0;
+#pragma unroll
+      for (int i = 0; i < ND; i++) {
+        double e = sm.red[(c.slot < CPW ? c.slot : 0) * ND + i];
+        cm = (e > cm || e != e) ? e : cm;  // max that keeps a NaN
+      }
+      if (active && learned) {
+        if (!(cm <= (it > 0 ? r0 : 1.7976931348623157e308))) {
+#pragma unroll
+          for (int t = 0; t < N; t++)
+#pragma unroll
+            for (int v = 0; v < V; v++) q[t][v] = w[v];
+          fellback = 1;
+          learned = 0;
+          skip = true;
+        } else {
+          r0 = cm;
+        }
+      }
+    }
+#endif
+    if (active && !skip) {
+      newton_node(c, q, rhs, dt);
+#pragma unroll
+      for (int t = 0; t < N; t++)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          if (!(dabs(rhs[t][v]) <= newton_tol * (1.0 + dabs(q[t][v])))) more = true;
+          q[t][v] = q[t][v] + rhs[t][v];
+        }
+    }
+    if (skip) more = true;
+#else
+    (void)learned;
+    (void)r0;
+    (void)newton_tol;
+    if (active) {
+      // Picard: q1 = DG_MAT^-1 rhs / prod W  (dg.cpp:187-202)
+#pragma unroll
+      for (int t = 0; t < N; t++)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          double acc = T_DGINV[t][0] * rhs[0][v];
+#pragma unroll
+          for (int s = 1; s < N; s++) acc = dfma(T_DGINV[t][s], rhs[s][v], acc);
+          double q1 = acc * c.IWP;
+          if (dabs(q1 - q[t][v]) > DG_TOL * (1.0 + dabs(q[t][v]))) more = true;
+          q[t][v] = q1;
+        }
+    }
+#endif
+    const unsigned long long mb = __ballot(more);
+    if (active) {
+      it++;
+      if ((mb & cellmask) == 0ull || it >= maxit) active = false;
+    }
+    __syncthreads();
+  }
+  // ---- epilogue: traces, volume term, optional full qh
+  if (valid) {
+#pragma unroll
+    for (int t = 0; t < N; t++)
+#pragma unroll
+      for (int v = 0; v < V; v++) sm.q[c.slot][t][c.idx][v] = q[t][v];
+  }
+  __syncthreads();
+  if (valid) {
+    static_for<0, D>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      constexpr int stride = ipow_c(N, D - 1 - d);
+      if (c.kd[d] == 0) {
+        const int j = (c.idx / (stride * N)) * stride + c.idx % stride;
+#pragma unroll
+        for (int e = 0; e < 2; e++)
+#pragma unroll
+          for (int t = 0; t < N; t++)
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+              double acc = (e ? T_END1[0] : T_END0[0]) * sm.q[c.slot][t][c.idx][v];
+#pragma unroll
+              for (int m = 1; m < N; m++)
+                acc = dfma(e ? T_END1[m] : T_END0[m], sm.q[c.slot][t][c.idx + m * stride][v], acc);
+              const int plane = (((d * 2 + e) * N + t) * ND1 + j) * V + v;
+              tr[(long)plane * g.nwc + cell] = acc;
+            }
+      }
+    });
+    if (qh) {
+#pragma unroll
+      for (int t = 0; t < N; t++)
+#pragma unroll
+        for (int v = 0; v < V; v++) qh[(long)((t * ND + c.idx) * V + v) * g.nwc + cell] = q[t][v];
+    }
+  }
+#if PDE_USE_S || PDE_USE_B
+  // volume term (fv.cpp:33-85): per node sum over t, then over nodes in order
+  double part[V];
+  if (valid) {
+#pragma unroll
+    for (int t = 0; t < N; t++) {
+      double s[V];
+      callS(s, q[t]);
+#if PDE_USE_B
+      static_for<0, D>([&](auto dc) {
+        constexpr int d = decltype(dc)::value;
+        double dqd[V], b[V * V], bd[V];
+        lds_deriv<d>(c, sm.q, t, g.dX[d], dqd);
+        B(b, q[t], d);
+        b_times<PDE_B_QUIRK ? 2 : 0>(b, dqd, bd);
+#pragma unroll
+        for (int v = 0; v < V; v++) s[v] = s[v] - bd[v];
+      });
+#endif
+      double tmp = dt * T_W[t];
+#pragma unroll
+      for (int d = 0; d < D; d++) tmp = tmp * c.wk[d];
+#pragma unroll
+      for (int v = 0; v < V; v++) part[v] = (t == 0) ? tmp * s[v] : dfma(tmp, s[v], part[v]);
+    }
+  }
+  __syncthreads();
+  if (valid) {
+#pragma unroll
+    for (int v = 0; v < V; v++) sm.q[c.slot][0][c.idx][v] = part[v];
+  }
+  __syncthreads();
+  if (valid && c.idx == 0) {
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      double acc = sm.q[c.slot][0][0][v];
+#pragma unroll
+      for (int i = 1; i < ND; i++) acc = acc + sm.q[c.slot][0][i][v];
+      vol[(long)v * g.nwc + cell] = acc;
+    }
+  }
+#endif
+  // ---- statistics (one contribution per cell)
+  unsigned long long mine = (valid && c.idx == 0) ? (unsigned long long)it : 0ull;
+  unsigned long long cnt = (valid && c.idx == 0) ? 1ull : 0ull;
+  unsigned long long cap = (valid && c.idx == 0 && it >= maxit) ? 1ull : 0ull;
+  unsigned long long fb = (valid && c.idx == 0) ? (unsigned long long)fellback : 0ull;
+  unsigned int mx = (valid && c.idx == 0) ? (unsigned int)it : 0u;
+  mine = wave_sum_u64(mine);
+  cnt = wave_sum_u64(cnt);
+  cap = wave_sum_u64(cap);
+  fb = wave_sum_u64(fb);
+  mx = wave_max_u32(mx);
+  if (lane == 0) {
+    atomicAdd(&st->pred_iters, mine);
+    atomicAdd(&st->pred_cells, cnt);
+    if (cap) atomicAdd(&st->pred_capped, cap);
+    if (fb) atomicAdd(&st->pred_fallbacks, fb);
+    atomicMax(&st->pred_maxit, mx);
+  }
+}
+
 // ------------------------------------------------------------ face flux
 // Rusanov flux and B path integral at every face point (fluxes.cpp:72-116).
 // Faces along d: (nx_d + 1) x prod_{e != d} nx_e, all touching an interior cell.

@@ -80,7 +80,8 @@ class Dev:
         return out
 
     def view_rows(self, r0, r1):
-        return Dev(r1 - r0, self.shape[1], ptr=self.ptr + 4 * r0 * self.shape[1], owner=self)
+        r0, r1 = int(r0), int(r1)   # plain ints: ctypes rejects numpy integers as pointers
+        return Dev(r1 - r0, self.shape[1], ptr=int(self.ptr) + 4 * r0 * self.shape[1], owner=self)
 
     def __del__(self):
         if getattr(self, "owned", False) and self.ptr:

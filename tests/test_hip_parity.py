@@ -302,3 +302,26 @@ def test_ensemble_members_equal_single_runs():
         uo, _, st, rc = oracle_run("euler2d", grids[i], 0.05, Ls[i], 0, 3, False)
         assert rc == 0 and st.steps == stats[i]["steps"]
         assert relerr(outs[i], uo) <= TOL
+
+
+@pytest.mark.gpu
+def test_predictor_schedules_agree():
+    """The static (one wavefront per cell group) and the persistent work-queued
+    predictor give the same bits and the same iteration statistics."""
+    u0 = reactive_small(64, 16, seed=2)
+    L = [64 / 1024, 16 / 1024]
+    outs = []
+    for sched in ("static", "queue"):
+        os.environ["PYPDE_PRED_SCHEDULE"] = sched
+        try:
+            s = Solver(u0, L, systems.source("reactive2d"), order=3, boundaryTypes="reflective",
+                       stiff=True, reflect=systems.REACTIVE_REFLECT)
+        finally:
+            del os.environ["PYPDE_PRED_SCHEDULE"]
+        st = s.step(8)
+        outs.append((s.state(), st))
+        s.close()
+    (ua, sa), (ub, sb) = outs
+    assert np.array_equal(ua, ub)
+    for k in ("steps", "pred_iters", "pred_cells", "pred_maxit", "pred_capped"):
+        assert sa[k] == sb[k], k

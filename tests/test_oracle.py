@@ -477,3 +477,59 @@ def test_convergence_order_on_smooth_wave(N, ndim, tf, lo):
     orders = [np.log2(errs[i] / errs[i + 1]) for i in range(2)]
     print(f"N={N} ndim={ndim}: L1 errors {errs}, orders {orders}")
     assert orders[-1] >= lo
+
+
+def test_newton_iterations_against_guess_accuracy():
+    """The stiff predictor from supplied initial guesses (the hook of
+    dg.cpp:216) on a reactive channel state: the converged solution as guess
+    takes one iteration; guesses within 1e-4 (relative, in qh - w) take far
+    fewer iterations than the plain guess; 1e-2 noise takes more.  Every run
+    converges to the same solution to the Newton tolerance."""
+    N, V, nx, ny = 3, 5, 48, 12
+    s = cc.system("reactive2d")
+    O = cc.oracle()
+    refl = systems.REACTIVE_REFLECT
+    u0 = systems.reactive_channel(nx, ny, seed=3)
+    u, _, st, rc = cc.oracle_solve("reactive2d", u0, 1.0, [nx / 1024, ny / 1024], 2, N,
+                                   stiff=True, max_steps=8, reflect=refl)
+    nX = np.array([nx, ny], dtype=np.int32)
+    nb = nX + 2 * N
+    dX = np.array([1 / 1024, 1 / 1024])
+    ub = np.zeros(int(np.prod(nb)) * V)
+    r = np.ascontiguousarray(refl.ravel())
+    O.oracle_boundaries(cc.dp(ub), cc.dp(np.ascontiguousarray(u)), cc.ip(nX), 2,
+                        cc.ip(np.array([2, 2], dtype=np.int32)), cc.dp(r), N, V)
+    nw = int(np.prod(nX + 2))
+    wh = np.zeros(nw * N * N * V)
+    O.oracle_weno(cc.dp(wh), cc.dp(ub), cc.ip(np.ascontiguousarray(nb, dtype=np.int32)), 2, N, V)
+    dt = O.oracle_timestep(s.F, None, cc.dp(wh), nw, cc.dp(dX), 2, N, V, 0.9, 1.0, 0.0, 8)
+
+    def solve(guess=None):
+        qh = np.zeros(nw * N ** 3 * V)
+        it = np.zeros(nw, dtype=np.int32)
+        o = cc.OracleOpts()
+        o.newton_tol = 1e-12
+        o.newton_maxit = 100
+        if guess is not None:
+            g = np.ascontiguousarray(guess.ravel())
+            o.guess = cc.dp(g)
+            o.guess_check = 1
+        O.oracle_predictor(s.F, None, s.S, s.M, cc.dp(wh), nw, cc.dp(dX), 2, N, V, dt, 1,
+                           C.byref(o), cc.dp(qh), it.ctypes.data_as(C.POINTER(C.c_int)))
+        return qh.reshape(nw, N ** 3, V), it
+
+    qh, itp = solve()
+    wt = np.tile(wh.reshape(nw, N * N, V), (1, N, 1))
+    d = qh - wt
+    rng = np.random.default_rng(0)
+    _, it_exact = solve(qh.copy())
+    q4, it4 = solve(wt + d * (1 + 1e-4 * rng.uniform(-1, 1, d.shape)))
+    q2, it2 = solve(wt + d * (1 + 1e-2 * rng.uniform(-1, 1, d.shape)))
+    print("mean iterations: plain", itp.mean(), "exact", it_exact.mean(), "1e-4", it4.mean(),
+          "1e-2", it2.mean())
+    assert np.all(it_exact == 1)
+    assert it4.mean() < 0.6 * itp.mean()
+    assert it2.mean() > itp.mean()
+    scale = np.max(np.abs(qh), axis=(0, 1))
+    for q in (q4, q2):
+        assert np.max(np.abs(q - qh) / scale) <= 1e-10
