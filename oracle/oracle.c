@@ -402,20 +402,135 @@ static void house_right(double *H, int n, const double *v, int m, double beta,
   }
 }
 
-/* True when no eigenvalue of the undeflated leading block H[0..hi][0..hi]
- * (shifted back by ex) can exceed `best` in modulus: |lambda| <= the block's
- * infinity norm + |ex|.  Only the largest modulus is wanted, so the iteration
- * stops there with the same answer (typically after the two acoustic
- * eigenvalues, leaving the cluster of the advected waves unresolved). */
-static int rest_bounded(const double *H, int n, int hi, double ex, double best) {
-  if (hi < 0) return 0;
-  double bound = 0.0;
-  for (int i = 0; i <= hi; i++) {
-    double rs = 0.0;
-    for (int j = 0; j <= hi; j++) rs = rs + fabs(H[i * n + j]);
-    bound = fmax(bound, rs);
+/* Characteristic polynomial p(x) = det(xI - H) of an upper Hessenberg H
+ * (n <= MAXV), monic, coefficients c[0..n]: leading principal minors expanded
+ * along their last column,
+ *   D_k = (x - h_kk) D_k-1 - sum_{m=1}^{k-1} h_{k-m,k} (prod_{j=k-m+1}^{k} h_{j,j-1}) D_{k-m-1}
+ * (1-based; Wilkinson, The Algebraic Eigenvalue Problem, ch. 7). */
+static void hess_charpoly(const double *H, int n, double *c) {
+  double D[MAXV + 1][MAXV + 1];
+  D[0][0] = 1.0;
+  for (int k = 1; k <= n; k++) {
+    double hkk = H[(k - 1) * n + (k - 1)];
+    for (int i = 0; i <= k; i++) {
+      double v = (i >= 1) ? D[k - 1][i - 1] : 0.0;
+      if (i <= k - 1) v = v - hkk * D[k - 1][i];
+      D[k][i] = v;
+    }
+    double prod = 1.0;
+    for (int m = 1; m <= k - 1; m++) {
+      /* prod = h_{k,k-1} h_{k-1,k-2} ... h_{k-m+1,k-m} (1-based) */
+      prod = prod * H[(k - m) * n + (k - m - 1)];
+      double t = H[(k - m - 1) * n + (k - 1)] * prod;
+      for (int i = 0; i <= k - m - 1; i++) D[k][i] = D[k][i] - t * D[k - m - 1][i];
+    }
   }
-  return bound + fabs(ex) <= best;
+  for (int i = 0; i <= n; i++) c[i] = D[n][i];
+}
+
+/* One real root of the monic degree-n polynomial c by Laguerre's method from
+ * x0 (cubic convergence at simple roots; from above the largest real root of
+ * a real-rooted polynomial it decreases monotonically onto it).  Returns 0
+ * when it does not converge in LAG_IT steps. */
+#define LAG_IT 40
+static int laguerre_real(const double *c, int n, double x0, double *root) {
+  double x = x0;
+  for (int it = 0; it < LAG_IT; it++) {
+    double b = c[n], d = 0.0, f = 0.0;
+    for (int j = n - 1; j >= 0; j--) {
+      f = x * f + d;
+      d = x * d + b;
+      b = x * b + c[j];
+    }
+    if (b == 0.0) {
+      *root = x;
+      return 1;
+    }
+    double G = d / b;
+    double Hh = G * G - 2.0 * (f / b);
+    double r = (double)(n - 1) * ((double)n * Hh - G * G);
+    double sq = r > 0.0 ? sqrt(r) : 0.0;
+    double den = (G >= 0.0) ? G + sq : G - sq;
+    if (!(den != 0.0)) return 0;
+    double step = (double)n / den;
+    x = x - step;
+    if (!(fabs(step) > 4.0 * DBL_EPSILON * fabs(x)) || fabs(step) <= DBL_MIN) {
+      *root = x;
+      return fabs(x) <= 1e300; /* finite */
+    }
+  }
+  return 0;
+}
+
+/* max |lambda| of an upper Hessenberg H (n >= 3, entries scaled to <= 1) from
+ * its characteristic polynomial: the largest and smallest real roots by
+ * Laguerre from +-||H||_inf, then a certificate that every other root is
+ * smaller in modulus -- the remaining factor q = p / ((x - l1)(x - l2)) is
+ * centred on the mean of its roots, alpha, and its roots bounded by
+ * |z - alpha| <= R, with R at or above the Cauchy radius of q(y + alpha)
+ * (the positive root of y^m - sum |t_k| y^k).  Returns 0 (caller falls back
+ * to QR) when a root does not converge or the certificate fails: complex or
+ * repeated extreme eigenvalues. */
+static int charpoly_max_abs(const double *H, int n, double *out) {
+  double c[MAXV + 1], q[MAXV + 1], t[MAXV + 1];
+  hess_charpoly(H, n, c);
+  double U = 0.0;
+  for (int i = 0; i < n; i++) {
+    double rs = 0.0;
+    for (int j = 0; j < n; j++) rs = rs + fabs(H[i * n + j]);
+    U = fmax(U, rs);
+  }
+  double l1, l2;
+  if (!laguerre_real(c, n, U, &l1) || !laguerre_real(c, n, -U, &l2)) return 0;
+  if (!(l1 > l2)) return 0;
+  /* q = p / (x - l1) / (x - l2), degree m = n - 2, monic */
+  for (int i = 0; i <= n; i++) q[i] = c[i];
+  int deg = n;
+  double rr[2] = {l1, l2};
+  for (int k = 0; k < 2; k++) {
+    double nq[MAXV + 1];
+    nq[deg - 1] = q[deg];
+    for (int j = deg - 2; j >= 0; j--) nq[j] = q[j + 1] + rr[k] * nq[j + 1];
+    deg -= 1;
+    for (int j = 0; j <= deg; j++) q[j] = nq[j];
+  }
+  int m = deg;
+  double M = fmax(fabs(l1), fabs(l2));
+  double alpha = -(q[m - 1] / (double)m);
+  /* Taylor shift t(y) = q(y + alpha) */
+  for (int i = 0; i <= m; i++) t[i] = q[i];
+  for (int k = 0; k < m; k++)
+    for (int j = m - 1; j >= k; j--) t[j] = t[j] + alpha * t[j + 1];
+  /* Cauchy radius rho of t: the positive root of f(y) = y^m - sum_k |t_k| y^k
+   * (f < 0 below rho, >= 0 above).  R: the smallest power-of-two fraction of
+   * 1 + max |t_k| (>= rho) with f(R) >= 0, then Newton steps from above, each
+   * kept only while f stays >= 0, so R >= rho >= |z - alpha| for every root z. */
+  double amax = 0.0;
+  for (int k = 0; k < m; k++) amax = fmax(amax, fabs(t[k]));
+  double R = 1.0 + amax;
+  for (int e = 0; e < 64; e++) {
+    double yh = 0.5 * R;
+    double f = 1.0;
+    for (int k = m - 1; k >= 0; k--) f = f * yh - fabs(t[k]);
+    if (!(f >= 0.0)) break;
+    R = yh;
+  }
+  for (int it = 0; it < 3; it++) {
+    double f = 1.0, fp = 0.0;
+    for (int k = m - 1; k >= 0; k--) {
+      fp = fp * R + f;
+      f = f * R - fabs(t[k]);
+    }
+    if (!(fp > 0.0)) break;
+    double Rn = R - f / fp;
+    double g = 1.0;
+    for (int k = m - 1; k >= 0; k--) g = g * Rn - fabs(t[k]);
+    if (!(g >= 0.0) || !(Rn > 0.0)) break;
+    R = Rn;
+  }
+  if (!(fabs(alpha) + R < M)) return 0;
+  *out = M;
+  return 1;
 }
 
 /* max |lambda| of a real n x n matrix, the quantity the reference takes from
@@ -450,6 +565,10 @@ double oracle_max_abs_eig(const double *A, int n) {
       house_right(H, n, vv, m, beta, 0, n - 1, k + 1);
     }
   }
+  {
+    double r;
+    if (n >= 3 && charpoly_max_abs(H, n, &r)) return ldexp(r, e2);
+  }
   double anorm = 0.0;
   for (int i = 0; i < n * n; i++) anorm = anorm + fabs(H[i]);
   double best = 0.0, ex = 0.0;
@@ -469,7 +588,6 @@ double oracle_max_abs_eig(const double *A, int n) {
       best = fmax(best, fabs(H[hi * n + hi] + ex));
       hi -= 1;
       iter = 0;
-      if (rest_bounded(H, n, hi, ex, best)) break;
       continue;
     }
     if (lo == hi - 1) {
@@ -490,7 +608,6 @@ double oracle_max_abs_eig(const double *A, int n) {
       }
       hi -= 2;
       iter = 0;
-      if (rest_bounded(H, n, hi, ex, best)) break;
       continue;
     }
     if (total >= 40 * n) {

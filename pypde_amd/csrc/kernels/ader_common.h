@@ -158,35 +158,154 @@ DEV void house_right(double* H, const double* v, double beta, int lo_r, int hi_r
   }
 }
 
-// element H[r][c] for run-time r, c with every index static (stays in registers)
-DEV double hget(const double* H, int r, int c) {
+// H[i][i + off] for a run-time i, every array index static (H stays in
+// registers); entries outside the matrix read as 0.
+template <int OFF>
+DEV double hband(const double* H, int i) {
   double out = 0.0;
 #pragma unroll
-  for (int i = 0; i < V; i++)
-#pragma unroll
-    for (int j = 0; j < V; j++)
-      if (i == r && j == c) out = H[i * V + j];
+  for (int r = 0; r < V; r++)
+    if (r + OFF >= 0 && r + OFF < V && r == i) out = H[r * V + r + OFF];
   return out;
 }
 
-// No eigenvalue of the undeflated leading block H[0..hi][0..hi] (shifted back
-// by ex) can exceed `best` in modulus when its infinity norm + |ex| <= best:
-// the largest modulus is then final.
-DEV bool rest_bounded(const double* H, int hi, double ex, double best) {
-  if (hi < 0) return false;
-  double bound = 0.0;
+#if PDE_V >= 3
+// Characteristic polynomial det(xI - H) of the upper Hessenberg H, monic,
+// c[0..V]: leading principal minors expanded along their last column,
+//   D_k = (x - h_kk) D_k-1 - sum_{m=1}^{k-1} h_{k-m,k} (prod_{j=k-m+1}^{k} h_{j,j-1}) D_{k-m-1}
+// (1-based; Wilkinson, The Algebraic Eigenvalue Problem, ch. 7).
+DEV void hess_charpoly(const double* H, double* c) {
+  double D[V + 1][V + 1];
+  D[0][0] = 1.0;
 #pragma unroll
-  for (int i = 0; i < V; i++) {
-    if (i <= hi) {
-      double rs = 0.0;
+  for (int k = 1; k <= V; k++) {
+    const double hkk = H[(k - 1) * V + (k - 1)];
 #pragma unroll
-      for (int j = 0; j < V; j++)
-        if (j <= hi) rs = rs + dabs(H[i * V + j]);
-      bound = dmax(bound, rs);
+    for (int i = 0; i <= k; i++) {
+      double v = (i >= 1) ? D[k - 1][i - 1] : 0.0;
+      if (i <= k - 1) v = v - hkk * D[k - 1][i];
+      D[k][i] = v;
+    }
+    double prod = 1.0;
+#pragma unroll
+    for (int m = 1; m <= k - 1; m++) {
+      prod = prod * H[(k - m) * V + (k - m - 1)];
+      const double t = H[(k - m - 1) * V + (k - 1)] * prod;
+#pragma unroll
+      for (int i = 0; i <= k - m - 1; i++) D[k][i] = D[k][i] - t * D[k - m - 1][i];
     }
   }
-  return bound + dabs(ex) <= best;
+#pragma unroll
+  for (int i = 0; i <= V; i++) c[i] = D[V][i];
 }
+
+// One real root of the monic degree-V polynomial c by Laguerre's method from
+// x0 (cubic at simple roots; from above the largest real root of a
+// real-rooted polynomial it decreases monotonically onto it).  False when it
+// does not converge in 40 steps.
+DEV bool laguerre_real(const double* c, double x0, double* root) {
+  double x = x0;
+  for (int it = 0; it < 40; it++) {
+    double b = c[V], d = 0.0, f = 0.0;
+#pragma unroll
+    for (int j = V - 1; j >= 0; j--) {
+      f = x * f + d;
+      d = x * d + b;
+      b = x * b + c[j];
+    }
+    if (b == 0.0) {
+      *root = x;
+      return true;
+    }
+    const double G = d / b;
+    const double Hh = G * G - 2.0 * (f / b);
+    const double r = (double)(V - 1) * ((double)V * Hh - G * G);
+    const double sq = r > 0.0 ? dsqrt(r) : 0.0;
+    const double den = (G >= 0.0) ? G + sq : G - sq;
+    if (!(den != 0.0)) return false;
+    const double step = (double)V / den;
+    x = x - step;
+    if (!(dabs(step) > 4.0 * 2.220446049250313e-16 * dabs(x)) ||
+        dabs(step) <= 2.2250738585072014e-308) {
+      *root = x;
+      return dabs(x) <= 1e300;
+    }
+  }
+  return false;
+}
+
+// max |lambda| of the scaled upper Hessenberg H from its characteristic
+// polynomial: the largest and smallest real roots by Laguerre from
+// +-||H||_inf, then a certificate that every other root is smaller in modulus
+// -- q = p / ((x - l1)(x - l2)) centred on the mean of its roots, alpha, and
+// its roots bounded by |z - alpha| <= R, R at or above the Cauchy radius of
+// q(y + alpha) (positive root of y^m - sum |t_k| y^k).  False (the caller
+// falls back to QR) when a root does not converge or the certificate fails:
+// complex or repeated extreme eigenvalues.
+DEV bool charpoly_max_abs(const double* H, double* out) {
+  constexpr int m = V - 2;
+  double c[V + 1];
+  hess_charpoly(H, c);
+  double U = 0.0;
+#pragma unroll
+  for (int i = 0; i < V; i++) {
+    double rs = 0.0;
+#pragma unroll
+    for (int j = 0; j < V; j++) rs = rs + dabs(H[i * V + j]);
+    U = dmax(U, rs);
+  }
+  double l1, l2;
+  if (!laguerre_real(c, U, &l1) || !laguerre_real(c, -U, &l2)) return false;
+  if (!(l1 > l2)) return false;
+  // q = p / (x - l1) / (x - l2)
+  double q1[V], q[V - 1];
+  q1[V - 1] = c[V];
+#pragma unroll
+  for (int j = V - 2; j >= 0; j--) q1[j] = c[j + 1] + l1 * q1[j + 1];
+  q[V - 2] = q1[V - 1];
+#pragma unroll
+  for (int j = V - 3; j >= 0; j--) q[j] = q1[j + 1] + l2 * q[j + 1];
+  const double M = dmax(dabs(l1), dabs(l2));
+  const double alpha = -(q[m - 1] / (double)m);
+  double t[m + 1];
+#pragma unroll
+  for (int i = 0; i <= m; i++) t[i] = q[i];
+#pragma unroll
+  for (int k = 0; k < m; k++)
+#pragma unroll
+    for (int j = m - 1; j >= k; j--) t[j] = t[j] + alpha * t[j + 1];
+  double amax = 0.0;
+#pragma unroll
+  for (int k = 0; k < m; k++) amax = dmax(amax, dabs(t[k]));
+  double R = 1.0 + amax;
+  for (int e = 0; e < 64; e++) {
+    const double yh = 0.5 * R;
+    double f = 1.0;
+#pragma unroll
+    for (int k = m - 1; k >= 0; k--) f = f * yh - dabs(t[k]);
+    if (!(f >= 0.0)) break;
+    R = yh;
+  }
+  for (int it = 0; it < 3; it++) {
+    double f = 1.0, fp = 0.0;
+#pragma unroll
+    for (int k = m - 1; k >= 0; k--) {
+      fp = fp * R + f;
+      f = f * R - dabs(t[k]);
+    }
+    if (!(fp > 0.0)) break;
+    const double Rn = R - f / fp;
+    double g = 1.0;
+#pragma unroll
+    for (int k = m - 1; k >= 0; k--) g = g * Rn - dabs(t[k]);
+    if (!(g >= 0.0) || !(Rn > 0.0)) break;
+    R = Rn;
+  }
+  if (!(dabs(alpha) + R < M)) return false;
+  *out = M;
+  return true;
+}
+#endif
 
 // max |lambda| of A (V x V, row-major; destroyed), as the reference takes it from
 // Eigen's EigenSolver (system.cpp:28-43): scale by the power of two at or above
@@ -219,6 +338,12 @@ DEV double max_abs_eig(double* H) {
       house_right<m, k + 1>(H, vv, beta, 0, V - 1);
     }
   });
+#if PDE_V >= 3
+  {
+    double r;
+    if (charpoly_max_abs(H, &r)) return __builtin_ldexp(r, e2);
+  }
+#endif
   double anorm = 0.0;
 #pragma unroll
   for (int i = 0; i < V * V; i++) anorm = anorm + dabs(H[i]);
@@ -240,15 +365,14 @@ DEV double max_abs_eig(double* H) {
       }
     }
     if (lo == hi) {
-      best = dmax(best, dabs(hget(H, hi, hi) + ex));
+      best = dmax(best, dabs(hband<0>(H, hi) + ex));
       hi -= 1;
       iter = 0;
-      if (rest_bounded(H, hi, ex, best)) break;
       continue;
     }
     if (lo == hi - 1) {
-      double a = hget(H, hi - 1, hi - 1) + ex, b = hget(H, hi - 1, hi);
-      double c = hget(H, hi, hi - 1), dd = hget(H, hi, hi) + ex;
+      double a = hband<0>(H, hi - 1) + ex, b = hband<1>(H, hi - 1);
+      double c = hband<-1>(H, hi), dd = hband<0>(H, hi) + ex;
       double p = 0.5 * (a - dd);
       double bc = b * c;
       double disc = p * p + bc;
@@ -264,7 +388,6 @@ DEV double max_abs_eig(double* H) {
       }
       hi -= 2;
       iter = 0;
-      if (rest_bounded(H, hi, ex, best)) break;
       continue;
     }
     if constexpr (V >= 3) {
@@ -274,14 +397,14 @@ DEV double max_abs_eig(double* H) {
           if (i <= hi) best = dmax(best, dabs(H[i * V + i] + ex));
         break;
       }
-      double sx = hget(H, hi, hi), sy = hget(H, hi - 1, hi - 1);
-      double sw = hget(H, hi, hi - 1) * hget(H, hi - 1, hi);
+      double sx = hband<0>(H, hi), sy = hband<0>(H, hi - 1);
+      double sw = hband<-1>(H, hi) * hband<1>(H, hi - 1);
       if (iter == 10) {
         ex = ex + sx;
 #pragma unroll
         for (int i = 0; i < V; i++)
           if (i <= hi) H[i * V + i] = H[i * V + i] - sx;
-        double s = dabs(hget(H, hi, hi - 1)) + dabs(hget(H, hi - 1, hi - 2));
+        double s = dabs(hband<-1>(H, hi)) + dabs(hband<-1>(H, hi - 1));
         sx = 0.75 * s;
         sy = sx;
         sw = -0.4375 * s * s;
@@ -304,12 +427,12 @@ DEV double max_abs_eig(double* H) {
       }
       iter++;
       total++;
-      double h00 = hget(H, lo, lo);
+      double h00 = hband<0>(H, lo);
       double r = sx - h00, q = sy - h00;
       double x[3], v[3];
-      x[0] = (r * q - sw) / hget(H, lo + 1, lo) + hget(H, lo, lo + 1);
-      x[1] = hget(H, lo + 1, lo + 1) - h00 - r - q;
-      x[2] = hget(H, lo + 2, lo + 1);
+      x[0] = (r * q - sw) / hband<-1>(H, lo + 1) + hband<1>(H, lo);
+      x[1] = hband<0>(H, lo + 1) - h00 - r - q;
+      x[2] = hband<-1>(H, lo + 2);
       static_for<0, (V >= 3 ? V - 2 : 0)>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
         if (k >= lo && k <= hi - 2) {
