@@ -19,6 +19,7 @@
 #define MAXD 3
 #define MAXND 216 /* MAXN^MAXD */
 #define HOUSE_TINY 0x1p-900
+#define NEWTON_A_TINY 1e-6 /* max |dt dS/dQ| below which the stiff step drops A */
 
 /* Constants of the reference: types.h:18, weno.h:8-10, dg.h:8-9,
  * NumericalDiff.h:9. */
@@ -343,9 +344,9 @@ static int ipow(int b, int e) {
 
 /* Derivative along axis d at node idx of a nodal tensor q (Nd x V rows):
  * DERVALS row k_d against the line through idx, then / dX[d]
- * (evaluations.cpp:23-47). */
+ * (evaluations.cpp:23-47), as a product with rdx = 1 / dX[d]. */
 static void node_deriv(const Tab *T, const double *q, int idx, int d,
-                       double dx, double *out) {
+                       double rdx, double *out) {
   int N = T->N, V = T->V;
   int stride = ipow(N, T->D - 1 - d);
   int kd = T->kk[idx][d];
@@ -354,7 +355,7 @@ static void node_deriv(const Tab *T, const double *q, int idx, int d,
     double acc = T->der[kd][0] * q[base * V + v];
     for (int j = 1; j < N; j++)
       acc = fma(T->der[kd][j], q[(base + j * stride) * V + v], acc);
-    out[v] = acc / dx;
+    out[v] = acc * rdx;
   }
 }
 
@@ -905,7 +906,7 @@ static double cell_speed(const Tab *T, oracle_F F, oracle_B B,
   for (int d = 0; d < D; d++) {
     double acc[MAXV];
     for (int idx = 0; idx < Nd; idx++) {
-      node_deriv(T, wc, idx, d, dX[d], tmp);
+      node_deriv(T, wc, idx, d, 1.0 / dX[d], tmp);
       for (int v = 0; v < V; v++)
         acc[v] = (idx == 0) ? T->WP[0] * tmp[v] : fma(T->WP[idx], tmp[v], acc[v]);
     }
@@ -983,7 +984,7 @@ static void dg_rhs(const Ctx *C, const double *w, const double *q, double *rhs) 
     const double *qt = q + (long)t * Nd * V;
     for (int idx = 0; idx < Nd; idx++)
       for (int d = 0; d < D; d++)
-        node_deriv(T, qt, idx, d, C->dX[d], dq + (idx * MAXD + d) * V);
+        node_deriv(T, qt, idx, d, 1.0 / C->dX[d], dq + (idx * MAXD + d) * V);
     if (C->F)
       for (int idx = 0; idx < Nd; idx++) {
         double qq[MAXV], dqq[MAXD * MAXV] = {0};
@@ -1008,7 +1009,7 @@ static void dg_rhs(const Ctx *C, const double *w, const double *q, double *rhs) 
         }
         if (C->F) {
           double df[MAXV];
-          node_deriv(T, f + d * MAXND * V, idx, d, C->dX[d], df);
+          node_deriv(T, f + d * MAXND * V, idx, d, 1.0 / C->dX[d], df);
           for (int v = 0; v < V; v++) r[v] = r[v] - df[v];
         }
       }
@@ -1107,7 +1108,10 @@ static void lu_solve_neg(int V, double *P, int n, double h[MAXN][MAXV]) {
  * frozen per spatial node at the time-weighted mean state (flux and B terms
  * lagged):  J = W_x diag(W_t) (x) I (dt JS - Tw), Tw = diag(1/W) DG_MAT, solved
  * through (Tw - A)^-1 = -adj(xI - Tw)(A) p(A)^-1 with A = dt JS (the entries of
- * the polynomial matrix commute).  The fixed point does not depend on J.
+ * the polynomial matrix commute).  Where max |A| < NEWTON_A_TINY (the source
+ * is not stiff there: unburnt gas with exp(-Ea rho/p) ~ 1e-9) the step uses
+ * A = 0 exactly, which costs N^2 V multiply-adds instead of the V x V matrix
+ * polynomial and LU.  The fixed point does not depend on J.
  *
  * Learned initial guess (guess != NULL): while the cell is still on its
  * guess, a cell residual max|obj| that is not finite or larger than at the
@@ -1183,44 +1187,58 @@ static int newton_cell(const Ctx *C, const double *w, double *q, double *rhs,
         for (int i = 0; i < V * V; i++) JS[i] = 0.0;
       }
       for (int i = 0; i < V * V; i++) A[i] = C->dt * JS[i];
-      /* h_t = sum_j A^j u_{t,j},  u_{t,j} = sum_s M_{N-j}[t][s] g_s  (Horner in A) */
-      for (int t = 0; t < N; t++) {
-        for (int j = N - 1; j >= 0; j--) {
-          double u[MAXV];
-          const double(*Mk)[MAXN] = T->flm[N - j - 1];
+      double amax = 0.0;
+      for (int i = 0; i < V * V; i++) amax = fmax(amax, fabs(A[i]));
+      if (amax < NEWTON_A_TINY) {
+        /* negligible source Jacobian: (Tw - A)^-1 = Tw^-1, i.e. dq_t =
+         * -(sum_s M_N[t][s] g_s) (1 / c_0) (the A = 0 case of the general step) */
+        for (int t = 0; t < N; t++)
           for (int v = 0; v < V; v++) {
+            const double(*Mk)[MAXN] = T->flm[N - 1];
             double acc = Mk[t][0] * g[0][v];
-            for (int s = 1; s < N; s++) acc = fma(Mk[t][s], g[s][v], acc);
-            u[v] = acc;
+            for (int s2 = 1; s2 < N; s2++) acc = fma(Mk[t][s2], g[s2][v], acc);
+            h[t][v] = -(acc * (1.0 / T->flc[0]));
           }
-          if (j == N - 1) {
-            for (int v = 0; v < V; v++) h[t][v] = u[v];
-          } else {
-            double Ah[MAXV];
-            for (int i = 0; i < V; i++) {
-              double acc = A[i * V] * h[t][0];
-              for (int k = 1; k < V; k++) acc = fma(A[i * V + k], h[t][k], acc);
-              Ah[i] = acc;
+      } else {
+        /* h_t = sum_j A^j u_{t,j},  u_{t,j} = sum_s M_{N-j}[t][s] g_s  (Horner in A) */
+        for (int t = 0; t < N; t++) {
+          for (int j = N - 1; j >= 0; j--) {
+            double u[MAXV];
+            const double(*Mk)[MAXN] = T->flm[N - j - 1];
+            for (int v = 0; v < V; v++) {
+              double acc = Mk[t][0] * g[0][v];
+              for (int s = 1; s < N; s++) acc = fma(Mk[t][s], g[s][v], acc);
+              u[v] = acc;
             }
-            for (int v = 0; v < V; v++) h[t][v] = Ah[v] + u[v];
+            if (j == N - 1) {
+              for (int v = 0; v < V; v++) h[t][v] = u[v];
+            } else {
+              double Ah[MAXV];
+              for (int i = 0; i < V; i++) {
+                double acc = A[i * V] * h[t][0];
+                for (int k = 1; k < V; k++) acc = fma(A[i * V + k], h[t][k], acc);
+                Ah[i] = acc;
+              }
+              for (int v = 0; v < V; v++) h[t][v] = Ah[v] + u[v];
+            }
           }
         }
+        /* P = p(A) = A^N + c_{N-1} A^{N-1} + ... + c_0 I  (Horner) */
+        for (int i = 0; i < V * V; i++) P[i] = A[i];
+        for (int i = 0; i < V; i++) P[i * V + i] = P[i * V + i] + T->flc[N - 1];
+        for (int j = N - 2; j >= 0; j--) {
+          double PA[MAXV * MAXV];
+          for (int i = 0; i < V; i++)
+            for (int k = 0; k < V; k++) {
+              double acc = P[i * V] * A[k];
+              for (int m = 1; m < V; m++) acc = fma(P[i * V + m], A[m * V + k], acc);
+              PA[i * V + k] = acc;
+            }
+          for (int i = 0; i < V * V; i++) P[i] = PA[i];
+          for (int i = 0; i < V; i++) P[i * V + i] = P[i * V + i] + T->flc[j];
+        }
+        lu_solve_neg(V, P, N, h);
       }
-      /* P = p(A) = A^N + c_{N-1} A^{N-1} + ... + c_0 I  (Horner) */
-      for (int i = 0; i < V * V; i++) P[i] = A[i];
-      for (int i = 0; i < V; i++) P[i * V + i] = P[i * V + i] + T->flc[N - 1];
-      for (int j = N - 2; j >= 0; j--) {
-        double PA[MAXV * MAXV];
-        for (int i = 0; i < V; i++)
-          for (int k = 0; k < V; k++) {
-            double acc = P[i * V] * A[k];
-            for (int m = 1; m < V; m++) acc = fma(P[i * V + m], A[m * V + k], acc);
-            PA[i * V + k] = acc;
-          }
-        for (int i = 0; i < V * V; i++) P[i] = PA[i];
-        for (int i = 0; i < V; i++) P[i * V + i] = P[i * V + i] + T->flc[j];
-      }
-      lu_solve_neg(V, P, N, h);
       for (int t = 0; t < N; t++)
         for (int v = 0; v < V; v++) {
           long e = ((long)t * Nd + idx) * V + v;
@@ -1402,7 +1420,7 @@ static void cell_volume(const Tab *T, oracle_F F, oracle_B B, oracle_S S,
       if (B)
         for (int d = 0; d < D; d++) {
           double dqd[MAXV], b[MAXV * MAXV], bd[MAXV];
-          node_deriv(T, qt, idx, d, dX[d], dqd);
+          node_deriv(T, qt, idx, d, 1.0 / dX[d], dqd);
           B(b, qq, d);
           b_times(b, dqd, V, quirk ? 2 : 0, bd);
           for (int v = 0; v < V; v++) s[v] = s[v] - bd[v];

@@ -156,7 +156,7 @@ k_weno(const double* __restrict__ u, double* __restrict__ wh, const StepState* _
 // Largest sum_d lambda_d / dX_d over every reconstructed cell
 // (stepper.cpp:42-65), reduced as IEEE bit patterns (all values >= 0).
 template <int d>
-DEV void cell_deriv(const double (&w)[ND][V], int idx, double dx, double* out) {
+DEV void cell_deriv(const double (&w)[ND][V], int idx, double rdx, double* out) {
   constexpr int stride = ipow_c(N, D - 1 - d);
   // idx is a compile-time constant at every call site after unrolling
   const int kd = (idx / stride) % N;
@@ -166,7 +166,7 @@ DEV void cell_deriv(const double (&w)[ND][V], int idx, double dx, double* out) {
     double acc = T_DER[kd][0] * w[base][v];
 #pragma unroll
     for (int j = 1; j < N; j++) acc = dfma(T_DER[kd][j], w[base + j * stride][v], acc);
-    out[v] = acc / dx;
+    out[v] = acc * rdx;
   }
 }
 
@@ -194,7 +194,7 @@ k_dt(const double* __restrict__ wh, StepState* st, Geom g) {
       double acc[V], tmp[V];
 #pragma unroll
       for (int idx = 0; idx < ND; idx++) {
-        cell_deriv<d>(w, idx, g.dX[d], tmp);
+        cell_deriv<d>(w, idx, 1.0 / g.dX[d], tmp);
 #pragma unroll
         for (int v = 0; v < V; v++) acc[v] = (idx == 0) ? T_WP[0] * tmp[v] : dfma(T_WP[idx], tmp[v], acc[v]);
       }

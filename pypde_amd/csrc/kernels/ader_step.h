@@ -36,6 +36,7 @@ struct NodeCtx {
   double wk[D];       // W[k_d]
   double WP, IWP;     // prod_d W[k_d], 1 / WP
   double ct[N];       // W_t prod W (dg.cpp:105-116)
+  double isc[N];      // 1 / (W_t prod W)
   double c0[N];       // psi_t(0) prod W (dg.cpp:156-171)
 };
 
@@ -66,19 +67,20 @@ DEV void node_ctx(NodeCtx& c, int lane) {
     }
     c.ct[t] = a;
     c.c0[t] = b;
+    c.isc[t] = 1.0 / (T_W[t] * p);
   }
 }
 
 // derivative along d at this lane's node of buffer buf[slot][t][node][v]
 template <int d>
-DEV void lds_deriv(const NodeCtx& c, const double (*buf)[N][ND][V], int t, double dx, double* out) {
+DEV void lds_deriv(const NodeCtx& c, const double (*buf)[N][ND][V], int t, double rdx, double* out) {
   constexpr int stride = ipow_c(N, D - 1 - d);
 #pragma unroll
   for (int v = 0; v < V; v++) {
     double acc = c.der[d][0] * buf[c.slot][t][c.base[d]][v];
 #pragma unroll
     for (int j = 1; j < N; j++) acc = dfma(c.der[d][j], buf[c.slot][t][c.base[d] + j * stride][v], acc);
-    out[v] = acc / dx;
+    out[v] = acc * rdx;
   }
 }
 
@@ -110,7 +112,7 @@ DEV void node_rhs(const NodeCtx& c, PredLDS& sm, const double (&q)[N][V], const 
     for (int t = 0; t < N; t++) {
       static_for<0, D>([&](auto dc) {
         constexpr int d = decltype(dc)::value;
-        lds_deriv<d>(c, sm.q, t, g.dX[d], dqs[t][d]);
+        lds_deriv<d>(c, sm.q, t, 1.0 / g.dX[d], dqs[t][d]);
       });
 #if PDE_USE_F
 #pragma unroll
@@ -142,7 +144,7 @@ DEV void node_rhs(const NodeCtx& c, PredLDS& sm, const double (&q)[N][V], const 
 #endif
 #if PDE_USE_F
         double df[V];
-        lds_deriv<d>(c, sm.f[d], t, g.dX[d], df);
+        lds_deriv<d>(c, sm.f[d], t, 1.0 / g.dX[d], df);
 #pragma unroll
         for (int v = 0; v < V; v++) r[v] = r[v] - df[v];
 #endif
@@ -157,17 +159,16 @@ DEV void node_rhs(const NodeCtx& c, PredLDS& sm, const double (&q)[N][V], const 
 }
 
 #if PDE_STIFF
+constexpr double NEWTON_A_TINY = 1e-6;  // max |dt dS/dQ| below which the step drops A
 // Newton correction at one spatial node (see the CPU restatement for the
 // derivation): g_t = G_t / (W_t WP), A = dt dS/dQ at the time-weighted mean,
 // h_t = sum_j A^j sum_s M_{N-j}[t][s] g_s, P = p(A), dq = -P^-1 h.
 DEV void newton_node(const NodeCtx& c, const double (&q)[N][V], double (&G)[N][V], double dt) {
   double g[N][V];
 #pragma unroll
-  for (int t = 0; t < N; t++) {
-    double sc = 1.0 / (T_W[t] * c.WP);
+  for (int t = 0; t < N; t++)
 #pragma unroll
-    for (int v = 0; v < V; v++) g[t][v] = G[t][v] * sc;
-  }
+    for (int v = 0; v < V; v++) g[t][v] = G[t][v] * c.isc[t];
   double qbar[V];
 #pragma unroll
   for (int v = 0; v < V; v++) {
@@ -203,6 +204,22 @@ DEV void newton_node(const NodeCtx& c, const double (&q)[N][V], double (&G)[N][V
 #endif
 #pragma unroll
   for (int i = 0; i < V * V; i++) A[i] = dt * A[i];
+  double amax = 0.0;
+#pragma unroll
+  for (int i = 0; i < V * V; i++) amax = dmax(amax, dabs(A[i]));
+  if (amax < NEWTON_A_TINY) {
+    // negligible source Jacobian: dq_t = -(sum_s M_N[t][s] g_s) (1 / c_0)
+#pragma unroll
+    for (int t = 0; t < N; t++)
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        double acc = T_FLM[N - 1][t][0] * g[0][v];
+#pragma unroll
+        for (int s2 = 1; s2 < N; s2++) acc = dfma(T_FLM[N - 1][t][s2], g[s2][v], acc);
+        G[t][v] = -(acc * (1.0 / T_FLC[0]));
+      }
+    return;
+  }
   // h_t (Horner in A); result left in G
 #pragma unroll
   for (int t = 0; t < N; t++) {
@@ -532,7 +549,7 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
         static_for<0, D>([&](auto dc) {
           constexpr int d = decltype(dc)::value;
           double dqd[V], b[V * V], bd[V];
-          lds_deriv<d>(c, sm.q, t, g.dX[d], dqd);
+          lds_deriv<d>(c, sm.q, t, 1.0 / g.dX[d], dqd);
           B(b, q[t], d);
           b_times<PDE_B_QUIRK ? 2 : 0>(b, dqd, bd);
 #pragma unroll
@@ -772,7 +789,7 @@ k_predictor_static(const double* __restrict__ wh, const double* __restrict__ gue
       static_for<0, D>([&](auto dc) {
         constexpr int d = decltype(dc)::value;
         double dqd[V], b[V * V], bd[V];
-        lds_deriv<d>(c, sm.q, t, g.dX[d], dqd);
+        lds_deriv<d>(c, sm.q, t, 1.0 / g.dX[d], dqd);
         B(b, q[t], d);
         b_times<PDE_B_QUIRK ? 2 : 0>(b, dqd, bd);
 #pragma unroll
