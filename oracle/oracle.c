@@ -727,6 +727,336 @@ static double max_abs_eigs(oracle_F F, oracle_B B, const double *q,
   return oracle_max_abs_eig(A, V);
 }
 
+/* ------------------------------------------- |A| x for Roe and Osher fluxes */
+
+/* Real Schur form T = Z^T H Z of an n x n matrix (H is overwritten by T, Z
+ * receives the orthogonal factor): Householder Hessenberg reduction and
+ * Francis double-shift QR as in oracle_max_abs_eig, here applied to the whole
+ * matrix with the transformations accumulated in Z (the EISPACK hqr2
+ * arrangement), exceptional shifts restored on the diagonal as each block
+ * deflates, and every 2 x 2 block with real eigenvalues split by a rotation.
+ * Returns 0, or 1 when the QR did not converge in 40 n steps (T is then
+ * only partly reduced). */
+static int real_schur(double *H, double *Z, int n) {
+  double v[3], x[MAXV];
+  for (int i = 0; i < n * n; i++) Z[i] = (i % (n + 1) == 0) ? 1.0 : 0.0;
+  for (int k = 0; k + 2 < n; k++) {
+    int m = n - k - 1;
+    double vv[MAXV];
+    for (int i = 0; i < m; i++) x[i] = H[(k + 1 + i) * n + k];
+    double beta = house(x, m, vv);
+    if (beta != 0.0) {
+      house_left(H, n, vv, m, beta, k + 1, k, n - 1);
+      house_right(H, n, vv, m, beta, 0, n - 1, k + 1);
+      house_right(Z, n, vv, m, beta, 0, n - 1, k + 1);
+    }
+    for (int i = k + 2; i < n; i++) H[i * n + k] = 0.0;
+  }
+  double anorm = 0.0;
+  for (int i = 0; i < n * n; i++) anorm = anorm + fabs(H[i]);
+  double ex = 0.0;
+  int hi = n - 1, iter = 0, total = 0;
+  while (hi >= 0) {
+    int lo = hi;
+    while (lo > 0) {
+      double s = fabs(H[(lo - 1) * n + lo - 1]) + fabs(H[lo * n + lo]);
+      if (s == 0.0) s = anorm;
+      if (fabs(H[lo * n + lo - 1]) <= DBL_EPSILON * s) {
+        H[lo * n + lo - 1] = 0.0;
+        break;
+      }
+      lo--;
+    }
+    if (lo == hi) {
+      H[hi * n + hi] = H[hi * n + hi] + ex;
+      hi -= 1;
+      iter = 0;
+      continue;
+    }
+    if (lo == hi - 1) {
+      int na = hi - 1;
+      H[na * n + na] = H[na * n + na] + ex;
+      H[hi * n + hi] = H[hi * n + hi] + ex;
+      double a = H[na * n + na], b = H[na * n + hi], c = H[hi * n + na], d = H[hi * n + hi];
+      double p = 0.5 * (a - d);
+      double disc = p * p + b * c;
+      if (disc >= 0.0) {
+        /* real pair: rotate so that the block becomes upper triangular */
+        double z = sqrt(disc);
+        z = (p >= 0.0) ? p + z : p - z;
+        double xr = c, sr = fabs(xr) + fabs(z);
+        double pr = xr / sr, qr = z / sr;
+        double rr = sqrt(pr * pr + qr * qr);
+        pr = pr / rr;
+        qr = qr / rr;
+        for (int j = na; j < n; j++) {
+          double t = H[na * n + j];
+          H[na * n + j] = qr * t + pr * H[hi * n + j];
+          H[hi * n + j] = qr * H[hi * n + j] - pr * t;
+        }
+        for (int i = 0; i <= hi; i++) {
+          double t = H[i * n + na];
+          H[i * n + na] = qr * t + pr * H[i * n + hi];
+          H[i * n + hi] = qr * H[i * n + hi] - pr * t;
+        }
+        for (int i = 0; i < n; i++) {
+          double t = Z[i * n + na];
+          Z[i * n + na] = qr * t + pr * Z[i * n + hi];
+          Z[i * n + hi] = qr * Z[i * n + hi] - pr * t;
+        }
+        H[hi * n + na] = 0.0;
+      }
+      hi -= 2;
+      iter = 0;
+      continue;
+    }
+    if (total >= 40 * n) {
+      for (int i = 0; i <= hi; i++) H[i * n + i] = H[i * n + i] + ex;
+      return 1;
+    }
+    double sx = H[hi * n + hi], sy = H[(hi - 1) * n + hi - 1];
+    double sw = H[hi * n + hi - 1] * H[(hi - 1) * n + hi];
+    if (iter == 10) {
+      ex = ex + sx;
+      for (int i = 0; i <= hi; i++) H[i * n + i] = H[i * n + i] - sx;
+      double s = fabs(H[hi * n + hi - 1]) + fabs(H[(hi - 1) * n + hi - 2]);
+      sx = 0.75 * s;
+      sy = sx;
+      sw = -0.4375 * s * s;
+    } else if (iter == 30) {
+      double s = (sy - sx) / 2.0;
+      s = s * s + sw;
+      if (s > 0.0) {
+        s = sqrt(s);
+        if (sy < sx) s = -s;
+        s = s + (sy - sx) / 2.0;
+        s = sx - sw / s;
+        ex = ex + s;
+        for (int i = 0; i <= hi; i++) H[i * n + i] = H[i * n + i] - s;
+        sx = 0.964;
+        sy = 0.964;
+        sw = 0.964;
+      }
+    }
+    iter++;
+    total++;
+    double h00 = H[lo * n + lo];
+    double r = sx - h00, q = sy - h00;
+    x[0] = (r * q - sw) / H[(lo + 1) * n + lo] + H[lo * n + lo + 1];
+    x[1] = H[(lo + 1) * n + lo + 1] - h00 - r - q;
+    x[2] = H[(lo + 2) * n + lo + 1];
+    for (int k = lo; k <= hi - 2; k++) {
+      if (k > lo) {
+        x[0] = H[k * n + k - 1];
+        x[1] = H[(k + 1) * n + k - 1];
+        x[2] = H[(k + 2) * n + k - 1];
+      }
+      double beta = house(x, 3, v);
+      if (beta != 0.0) {
+        if (k > lo) {
+          H[k * n + k - 1] = sqrt(x[0] * x[0] + (x[1] * x[1] + x[2] * x[2]));
+          H[(k + 1) * n + k - 1] = 0.0;
+          H[(k + 2) * n + k - 1] = 0.0;
+        }
+        house_left(H, n, v, 3, beta, k, k, n - 1);
+        int rr = (k + 3 < hi) ? k + 3 : hi;
+        house_right(H, n, v, 3, beta, 0, rr, k);
+        house_right(Z, n, v, 3, beta, 0, n - 1, k);
+      }
+    }
+    x[0] = H[(hi - 1) * n + hi - 2];
+    x[1] = H[hi * n + hi - 2];
+    double beta = house(x, 2, v);
+    if (beta != 0.0) {
+      H[(hi - 1) * n + hi - 2] = sqrt(x[0] * x[0] + x[1] * x[1]);
+      H[hi * n + hi - 2] = 0.0;
+      house_left(H, n, v, 2, beta, hi - 1, hi - 1, n - 1);
+      house_right(H, n, v, 2, beta, 0, hi, hi - 1);
+      house_right(Z, n, v, 2, beta, 0, n - 1, hi - 1);
+    }
+  }
+  return 0;
+}
+
+/* Divisor guard of the back substitution (as Eigen's EigenSolver): a zero
+ * pivot of T - lambda becomes eps ||T||. */
+static double guard_div(double w, double tn) {
+  return (w == 0.0) ? DBL_EPSILON * tn : w;
+}
+
+/* y = |A| x = R |Lambda| R^-1 x for a real n x n matrix A, R its eigenvectors
+ * (fluxes.cpp:20-70 forms exactly this from Eigen's EigenSolver).  Computed in
+ * real arithmetic: from the real Schur form T = Z^T A Z, a basis Y of each
+ * invariant subspace of T (the eigenvector of a real eigenvalue; real and
+ * imaginary parts of the eigenvector of a complex pair, on which |A| acts as
+ * |lambda| I), W = Z Y, then y = W D W^-1 x with D = |lambda| per column.
+ * A is scaled by the power of two at or above its largest |entry| first. */
+void oracle_abs_apply(const double *A, const double *xin, int n, double *y) {
+  double T[MAXV * MAXV], Z[MAXV * MAXV], Y[MAXV * MAXV], W[MAXV * MAXV], D[MAXV];
+  double scale = 0.0;
+  for (int i = 0; i < n * n; i++) scale = fmax(scale, fabs(A[i]));
+  if (!(scale >= DBL_MIN)) {
+    for (int i = 0; i < n; i++) y[i] = (scale != scale) ? scale : 0.0;
+    return;
+  }
+  int e2;
+  (void)frexp(scale, &e2);
+  double pw = ldexp(1.0, -e2);
+  for (int i = 0; i < n * n; i++) T[i] = A[i] * pw;
+  (void)real_schur(T, Z, n);
+  double tn = 0.0;
+  for (int i = 0; i < n * n; i++) tn = tn + fabs(T[i]);
+  for (int i = 0; i < n * n; i++) Y[i] = 0.0;
+  /* invariant-subspace bases, from the bottom block up */
+  int en = n - 1;
+  while (en >= 0) {
+    if (en >= 1 && T[en * n + en - 1] != 0.0) {
+      /* complex pair on rows en-1, en: eigenvector for lambda = a + i b */
+      int na = en - 1;
+      double a = 0.5 * (T[na * n + na] + T[en * n + en]);
+      double pp = 0.5 * (T[na * n + na] - T[en * n + en]);
+      double bb = sqrt(-(pp * pp + T[na * n + en] * T[en * n + na]));
+      D[na] = ldexp(sqrt(a * a + bb * bb), e2);
+      D[en] = D[na];
+      /* vector (yr + i yi): rows na, en from (T_blk - lambda) y = 0 with y[en] = 1:
+       * y[na] = (lambda - t_en,en) / t_en,na */
+      double yr[MAXV], yi[MAXV];
+      for (int i = 0; i < n; i++) {
+        yr[i] = 0.0;
+        yi[i] = 0.0;
+      }
+      yr[en] = 1.0;
+      yr[na] = (a - T[en * n + en]) / T[en * n + na];
+      yi[na] = bb / T[en * n + na];
+      int i = na - 1;
+      while (i >= 0) {
+        if (i >= 1 && T[i * n + i - 1] != 0.0) {
+          /* 2 x 2 block rows i-1, i: complex 2 x 2 solve (Cramer) */
+          int im = i - 1;
+          double s1r = 0.0, s1i = 0.0, s2r = 0.0, s2i = 0.0;
+          for (int j = i + 1; j <= en; j++) {
+            s1r = fma(T[im * n + j], yr[j], s1r);
+            s1i = fma(T[im * n + j], yi[j], s1i);
+            s2r = fma(T[i * n + j], yr[j], s2r);
+            s2i = fma(T[i * n + j], yi[j], s2i);
+          }
+          /* [[t11 - lam, t12], [t21, t22 - lam]] [u1; u2] = -[s1; s2] */
+          double m11r = T[im * n + im] - a, m11i = -bb, m12 = T[im * n + i];
+          double m21 = T[i * n + im], m22r = T[i * n + i] - a, m22i = -bb;
+          double detr = (m11r * m22r - m11i * m22i) - m12 * m21;
+          double deti = m11r * m22i + m11i * m22r;
+          if (detr == 0.0 && deti == 0.0) detr = DBL_EPSILON * tn;
+          /* u1 = (-s1 m22 + s2 m12) / det, u2 = (-s2 m11 + s1 m21) / det */
+          double n1r = -(s1r * m22r - s1i * m22i) + s2r * m12;
+          double n1i = -(s1r * m22i + s1i * m22r) + s2i * m12;
+          double n2r = -(s2r * m11r - s2i * m11i) + s1r * m21;
+          double n2i = -(s2r * m11i + s2i * m11r) + s1i * m21;
+          double dd = detr * detr + deti * deti;
+          yr[im] = (n1r * detr + n1i * deti) / dd;
+          yi[im] = (n1i * detr - n1r * deti) / dd;
+          yr[i] = (n2r * detr + n2i * deti) / dd;
+          yi[i] = (n2i * detr - n2r * deti) / dd;
+          i -= 2;
+        } else {
+          double sr = 0.0, si = 0.0;
+          for (int j = i + 1; j <= en; j++) {
+            sr = fma(T[i * n + j], yr[j], sr);
+            si = fma(T[i * n + j], yi[j], si);
+          }
+          /* (t_ii - a - i b) y_i = -s */
+          double wr = guard_div(T[i * n + i] - a, tn), wi = -bb;
+          double dd = wr * wr + wi * wi;
+          yr[i] = -(sr * wr + si * wi) / dd;
+          yi[i] = -(si * wr - sr * wi) / dd;
+          i -= 1;
+        }
+      }
+      for (int r = 0; r < n; r++) {
+        Y[r * n + na] = yr[r];
+        Y[r * n + en] = yi[r];
+      }
+      en -= 2;
+    } else {
+      double lam = T[en * n + en];
+      D[en] = ldexp(fabs(lam), e2);
+      double yv[MAXV];
+      for (int i = 0; i < n; i++) yv[i] = 0.0;
+      yv[en] = 1.0;
+      int i = en - 1;
+      while (i >= 0) {
+        if (i >= 1 && T[i * n + i - 1] != 0.0) {
+          int im = i - 1;
+          double s1 = 0.0, s2 = 0.0;
+          for (int j = i + 1; j <= en; j++) {
+            s1 = fma(T[im * n + j], yv[j], s1);
+            s2 = fma(T[i * n + j], yv[j], s2);
+          }
+          double m11 = T[im * n + im] - lam, m12 = T[im * n + i];
+          double m21 = T[i * n + im], m22 = T[i * n + i] - lam;
+          double det = guard_div(m11 * m22 - m12 * m21, tn);
+          yv[im] = (-s1 * m22 + s2 * m12) / det;
+          yv[i] = (-s2 * m11 + s1 * m21) / det;
+          i -= 2;
+        } else {
+          double sacc = 0.0;
+          for (int j = i + 1; j <= en; j++) sacc = fma(T[i * n + j], yv[j], sacc);
+          yv[i] = -sacc / guard_div(T[i * n + i] - lam, tn);
+          i -= 1;
+        }
+      }
+      for (int r = 0; r < n; r++) Y[r * n + en] = yv[r];
+      en -= 1;
+    }
+  }
+  /* W = Z Y */
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      double acc = Z[i * n] * Y[j];
+      for (int k = 1; k < n; k++) acc = fma(Z[i * n + k], Y[k * n + j], acc);
+      W[i * n + j] = acc;
+    }
+  /* b = W^-1 x (LU, partial pivoting), b *= D, y = W b */
+  double LU[MAXV * MAXV], b[MAXV];
+  int piv[MAXV];
+  memcpy(LU, W, sizeof(double) * n * n);
+  for (int i = 0; i < n; i++) b[i] = xin[i];
+  for (int k = 0; k < n; k++) {
+    int p = k;
+    for (int i = k + 1; i < n; i++)
+      if (fabs(LU[i * n + k]) > fabs(LU[p * n + k])) p = i;
+    piv[k] = p;
+    if (p != k) {
+      for (int j = 0; j < n; j++) {
+        double t = LU[k * n + j];
+        LU[k * n + j] = LU[p * n + j];
+        LU[p * n + j] = t;
+      }
+      double t = b[k];
+      b[k] = b[p];
+      b[p] = t;
+    }
+    double piv_k = guard_div(LU[k * n + k], 1.0);
+    for (int i = k + 1; i < n; i++) {
+      double l = LU[i * n + k] / piv_k;
+      LU[i * n + k] = l;
+      for (int j = k + 1; j < n; j++) LU[i * n + j] = fma(-l, LU[k * n + j], LU[i * n + j]);
+      b[i] = fma(-l, b[k], b[i]);
+    }
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    double acc = b[i];
+    for (int j = i + 1; j < n; j++) acc = fma(-LU[i * n + j], b[j], acc);
+    b[i] = acc / guard_div(LU[i * n + i], 1.0);
+  }
+  (void)piv;
+  for (int i = 0; i < n; i++) b[i] = b[i] * D[i];
+  for (int i = 0; i < n; i++) {
+    double acc = W[i * n] * b[0];
+    for (int k = 1; k < n; k++) acc = fma(W[i * n + k], b[k], acc);
+    y[i] = acc;
+  }
+}
+
 double oracle_exp(double x) {
   /* Same method as pde_exp in pypde_amd/systems/pde_prelude.h (which system
    * sources call); duplicated here so the checker is self-contained. */
@@ -1363,21 +1693,41 @@ static void face_trace(const Tab *T, const double *qc, int d, int e, int t,
   }
 }
 
-/* Rusanov: f = max(lam(qL), lam(qR)) (qL - qR) + (F(qL) + F(qR))
- * (fluxes.cpp:72-101); Bint = (sum_i W_i B(qL + x_i (qR - qL))) (qR - qL)
- * (fluxes.cpp:103-116). */
+/* f = D(qL, qR) + (F(qL) + F(qR)) (fluxes.cpp:79-101) with the dissipation
+ *   Rusanov (0): max(lam(qL), lam(qR)) (qL - qR)           (fluxes.cpp:72-77)
+ *   Roe (1):     |sum_i W_i A(q_i)| (qL - qR)               (fluxes.cpp:47-70)
+ *   Osher (2):   sum_i W_i |A(q_i)| (qL - qR)               (fluxes.cpp:20-45)
+ * on the path q_i = qR + x_i (qL - qR), A = dF/dQ + B; Bint =
+ * (sum_i W_i B(qL + x_i (qR - qL))) (qR - qL) (fluxes.cpp:103-116). */
 static void face_flux(const Tab *T, oracle_F F, oracle_B B, int flux,
                       const double *qL, const double *qR, int d, double *f,
                       double *bint) {
   int V = T->V, N = T->N;
   double dq0[MAXD * MAXV] = {0};
-  if (F) {
+  if (F && flux == 0) {
     double fl[MAXV], fr[MAXV];
     double l1 = max_abs_eigs(F, B, qL, dq0, d, V, fl);
     double l2 = max_abs_eigs(F, B, qR, dq0, d, V, fr);
     double s = fmax(l1, l2);
-    (void)flux;
     for (int v = 0; v < V; v++) f[v] = s * (qL[v] - qR[v]) + (fl[v] + fr[v]);
+  } else if (F) {
+    double fl[MAXV], fr[MAXV], Dq[MAXV], q[MAXV], A[MAXV * MAXV], Mx[MAXV * MAXV], y[MAXV],
+        Dsum[MAXV], ftmp[MAXV];
+    F(fl, (double *)qL, dq0, d);
+    F(fr, (double *)qR, dq0, d);
+    for (int v = 0; v < V; v++) Dq[v] = qL[v] - qR[v];
+    for (int i = 0; i < N; i++) {
+      for (int v = 0; v < V; v++) q[v] = fma(T->x[i], Dq[v], qR[v]);
+      system_matrix(F, B, q, dq0, d, V, A, ftmp);
+      if (flux == 1) {
+        for (int k = 0; k < V * V; k++) Mx[k] = (i == 0) ? T->w[0] * A[k] : fma(T->w[i], A[k], Mx[k]);
+      } else {
+        oracle_abs_apply(A, Dq, V, y);
+        for (int v = 0; v < V; v++) Dsum[v] = (i == 0) ? T->w[0] * y[v] : fma(T->w[i], y[v], Dsum[v]);
+      }
+    }
+    if (flux == 1) oracle_abs_apply(Mx, Dq, V, Dsum);
+    for (int v = 0; v < V; v++) f[v] = Dsum[v] + (fl[v] + fr[v]);
   } else {
     for (int v = 0; v < V; v++) f[v] = 0.0;
   }
@@ -1573,7 +1923,7 @@ int oracle_pde_solver(oracle_F F, oracle_B B, oracle_S S, oracle_M M, int useF,
   if (!useS) S = NULL;
   if (!S) M = NULL;
   if (secondOrder) return 10; /* viscous F(Q, dQ) path not restated */
-  if (flux != 0) return 11;   /* Rusanov only */
+  if (flux < 0 || flux > 2) return 11; /* Rusanov, Roe, Osher */
   if (ndt < 1) return 12;
   Tab T;
   if (tab_init(&T, N, ndim, V)) return 13;

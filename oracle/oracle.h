@@ -54,6 +54,8 @@ int oracle_tables(int N, double *nodes, double *wghts, double *endvals,
 
 double oracle_exp(double x);
 double oracle_max_abs_eig(const double *A, int V);
+/* y = |A| x = R |Lambda| R^-1 x (Roe and Osher dissipation, fluxes.cpp:20-70) */
+void oracle_abs_apply(const double *A, const double *x, int V, double *y);
 void oracle_system_matrix(oracle_F F, oracle_B B, const double *q,
                           const double *dq, int d, int ndim, int V, double *out);
 

@@ -615,12 +615,11 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
 
 // Static schedule: one wavefront per group of CPW consecutive cells, which
 // stays until its slowest cell has converged.
-extern "C" __global__ void __launch_bounds__(64)
-k_predictor_static(const double* __restrict__ wh, const double* __restrict__ guess,
-            double* __restrict__ tr, double* __restrict__ vol, double* __restrict__ qh,
-            StepState* st, Geom g, double newton_tol, int newton_maxit) {
+DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
+                               const double* __restrict__ guess, double* __restrict__ tr,
+                               double* __restrict__ vol, double* __restrict__ qh, StepState* st,
+                               const Geom& g, double newton_tol, int newton_maxit) {
   if (st->done) return;
-  __shared__ PredLDS sm;
   const double dt = st->dt;
   const int lane = threadIdx.x;
   NodeCtx c;
@@ -837,6 +836,24 @@ k_predictor_static(const double* __restrict__ wh, const double* __restrict__ gue
     if (fb) atomicAdd(&st->pred_fallbacks, fb);
     atomicMax(&st->pred_maxit, mx);
   }
+}
+
+extern "C" __global__ void __launch_bounds__(64)
+k_predictor_static(const double* __restrict__ wh, const double* __restrict__ guess,
+                   double* __restrict__ tr, double* __restrict__ vol, double* __restrict__ qh,
+                   StepState* st, Geom g, double newton_tol, int newton_maxit) {
+  __shared__ PredLDS sm;
+  predictor_static_body(sm, wh, guess, tr, vol, qh, st, g, newton_tol, newton_maxit);
+}
+
+// The same with the register budget of two wavefronts per SIMD (spilling the
+// rest), for latency hiding where the single-wavefront kernel stalls.
+extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_predictor_static2(const double* __restrict__ wh, const double* __restrict__ guess,
+                    double* __restrict__ tr, double* __restrict__ vol, double* __restrict__ qh,
+                    StepState* st, Geom g, double newton_tol, int newton_maxit) {
+  __shared__ PredLDS sm;
+  predictor_static_body(sm, wh, guess, tr, vol, qh, st, g, newton_tol, newton_maxit);
 }
 
 // ------------------------------------------------------------ face flux

@@ -229,11 +229,11 @@ int check_spec(const Spec& s, bool secondOrder, int FLUX) {
 }
 
 enum { K_WENO, K_DT, K_DTFIN, K_PRED, K_FLUX0, K_FLUX1, K_UPDATE, K_ADVANCE, K_A2S, K_S2A,
-       K_HARVEST, K_MLPGUESS, K_PRED_STATIC, K_COUNT };
+       K_HARVEST, K_MLPGUESS, K_PRED_STATIC, K_PRED_STATIC2, K_COUNT };
 const char* kernel_names[K_COUNT] = {"k_weno",   "k_dt",    "k_dt_finalize", "k_predictor",
                                      "k_flux0",  "k_flux1", "k_update",      "k_advance",
                                      "k_aos_to_soa", "k_soa_to_aos", "k_harvest", "k_mlp_guess",
-                                     "k_predictor_static"};
+                                     "k_predictor_static", "k_predictor_static2"};
 
 struct MLPW {
   const float *W1t, *b1, *W2t, *b2, *W3t, *b3;
@@ -269,6 +269,7 @@ struct pypde_ctx {
   hipFunction_t pred_fn = nullptr;
   long pred_blocks = 0;     // persistent predictor workgroups for pred_fn
   bool pred_static = true;  // one wavefront per cell group instead of the work queue
+  bool pred_w2 = false;     // the two-wavefronts-per-SIMD register budget
 };
 
 namespace {
@@ -373,7 +374,8 @@ int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
     int maxit = c->opts.newton_maxit;
     std::vector<void*> a = {&w, &gs, &t, &v, &qh, &st, &g, &tol, &maxit};
     if (c->pred_static) {
-      if ((rc = launch(c, K_PRED_STATIC, dim3((unsigned)cdiv(g.nwc, c->CPW)), dim3(64), a)))
+      if ((rc = launch(c, c->pred_w2 ? K_PRED_STATIC2 : K_PRED_STATIC,
+                       dim3((unsigned)cdiv(g.nwc, c->CPW)), dim3(64), a)))
         return rc;
     } else if ((rc = launch(c, K_PRED, dim3((unsigned)predictor_blocks(c)), dim3(64), a))) {
       return rc;
@@ -559,7 +561,10 @@ pypde_ctx* pypde_create(const char* src, bool useF, bool useB, bool useS, int* n
   // the 1024 x 256 reactive channel (order 3, 3.76 Newton iterations per cell)
   // the static kernel takes 1.50 ms per step and the work queue 2.62 ms.
   c->pred_static = true;
-  if (const char* e = getenv("PYPDE_PRED_SCHEDULE")) c->pred_static = strcmp(e, "queue") != 0;
+  if (const char* e = getenv("PYPDE_PRED_SCHEDULE")) {
+    c->pred_static = strcmp(e, "queue") != 0;
+    c->pred_w2 = strcmp(e, "static2") == 0;
+  }
   Geom& g = c->g;
   memset(&g, 0, sizeof g);
   g.ncell = 1;

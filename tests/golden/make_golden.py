@@ -5,9 +5,9 @@ oracle/Makefile).  Run from the repository root:  python tests/golden/make_golde
 tables.npz  basis/WENO/DG tables for N = 1..5 (poly/basis.cpp, weno_matrices.cpp,
             dg_matrices.cpp)
 weno.npz    weno_solver (api.cpp:32-48) on seeded random data
-solves.npz  pde_solver end states: C0 Sod (order 3) plus the reference's own
-            sensitivity on it to 1-ulp input changes (one entry; all entries),
-            and 24x24 periodic 2D Euler, orders 2-4
+solves.npz  pde_solver end states: C0 Sod (order 3; Rusanov, Roe and Osher
+            fluxes) plus the reference's own sensitivity on each to 1-ulp input
+            changes, and 24x24 periodic 2D Euler, orders 2-4
 """
 import os
 import sys
@@ -60,6 +60,17 @@ def main():
         noises.append(to.relerr(ur2, ur))
     s["sod_final"] = ur
     s["sod_noise"] = np.array(max(noises))
+    # Roe (1) and Osher (2) fluxes on the same problem, with their own noise
+    for flux, name in ((1, "roe"), (2, "osher")):
+        ur = to.ref_sod_flux(u, flux)
+        fl_noises = []
+        for _ in range(2):
+            u2 = u.copy()
+            m = rng.random(u.shape) < 0.5
+            u2[m] = np.nextafter(u2[m], np.inf)
+            fl_noises.append(to.relerr(to.ref_sod_flux(u2, flux), ur))
+        s[f"sod_{name}_final"] = ur
+        s[f"sod_{name}_noise"] = np.array(max(fl_noises))
     for N in [2, 3, 4]:
         ur, _ = cc.ref_solve("euler2d", to.euler2d_field(24, 24, noise=0.0), 0.05, [1, 1], 1, N,
                              nthreads=8)

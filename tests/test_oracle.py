@@ -547,3 +547,98 @@ def test_newton_iterations_against_guess_accuracy():
     scale = np.max(np.abs(qh), axis=(0, 1))
     for q in (q4, q2):
         assert np.max(np.abs(q - qh) / scale) <= 1e-10
+
+
+def oracle_sod_flux(u, flux):
+    """Oracle pde_solver on 1D Sod (200 cells, order 3, tf 0.2) with a flux code."""
+    s = cc.system("euler1d")
+    uo = np.ascontiguousarray(u).copy()
+    nX = np.array([200], dtype=np.int32)
+    dX = np.array([1 / 200])
+    ret = np.zeros(u.size)
+    o = cc.OracleOpts()
+    st = cc.OracleStats()
+    rc = cc.oracle().oracle_pde_solver(s.F, None, None, None, 1, 0, 0, cc.dp(uo), 0.2,
+                                       cc.ip(nX), 1, cc.dp(dX), 0.9,
+                                       cc.ip(np.array([0], dtype=np.int32)), 0, flux, 3, 3, 1, 0,
+                                       cc.dp(ret), C.byref(o), C.byref(st))
+    assert rc == 0
+    return uo
+
+
+def ref_sod_flux(u, flux):
+    """The reference's pde_solver on the same problem (stdout suppressed)."""
+    s = cc.system("euler1d")
+    ur = np.ascontiguousarray(u).copy()
+    nX = np.array([200], dtype=np.int32)
+    dX = np.array([1 / 200])
+    ret = np.zeros(u.size)
+    with cc.quiet_stdout():
+        cc.ref().pde_solver(s.F, None, None, True, False, False, cc.dp(ur), 0.2, cc.ip(nX), 1,
+                            cc.dp(dX), 0.9, cc.ip(np.array([0], dtype=np.int32)), False, flux, 3,
+                            3, 1, False, cc.dp(ret), 4)
+    return ur
+
+
+def abs_apply_numpy(A, x):
+    w, R = np.linalg.eig(A)
+    b = np.linalg.solve(R, x.astype(complex)) * np.abs(w)
+    return (R @ b).real
+
+
+def test_abs_apply_against_numpy():
+    """|A| x = R |Lambda| R^-1 x (the Roe and Osher dissipation,
+    fluxes.cpp:20-70) against numpy's complex eigendecomposition: real, complex,
+    clustered and badly scaled spectra, n = 1..8; error relative to cond(R)."""
+    O = cc.oracle()
+    O.oracle_abs_apply.argtypes = [cc.PD, cc.PD, C.c_int, cc.PD]
+    rng = np.random.default_rng(0)
+    for trial in range(800):
+        n = int(rng.integers(1, 9))
+        kind = trial % 4
+        if kind == 0:
+            A = rng.standard_normal((n, n))
+        elif kind == 1:
+            R = rng.standard_normal((n, n))
+            A = R @ np.diag(rng.uniform(-2, 2, n)) @ np.linalg.inv(R)
+        elif kind == 2:
+            R = rng.standard_normal((n, n))
+            lam = np.full(n, 0.3) + 1e-8 * rng.standard_normal(n)
+            lam[0] = 2.0
+            if n > 1:
+                lam[-1] = -1.5
+            A = R @ np.diag(lam) @ np.linalg.inv(R)
+        else:
+            A = rng.standard_normal((n, n)) * rng.uniform(1e-3, 1e3)
+        x = rng.standard_normal(n)
+        y = np.zeros(n)
+        O.oracle_abs_apply(cc.dp(np.ascontiguousarray(A.ravel())), cc.dp(x), n, cc.dp(y))
+        want = abs_apply_numpy(A, x)
+        cond = max(np.linalg.cond(np.linalg.eig(A)[1]), 1.0)
+        assert np.max(np.abs(y - want)) <= 1e-13 * cond * max(np.max(np.abs(want)), 1e-300), trial
+
+
+@pytest.mark.ref
+@pytest.mark.parametrize("flux", [1, 2])
+def test_roe_osher_reference_within_its_own_noise(flux):
+    """Roe (flux 1) and Osher (flux 2) on 1D Sod against the reference, within
+    3x the reference's own movement under all-entry 1-ulp input changes."""
+    u = sod1d()
+    uo = oracle_sod_flux(u, flux)
+    ur = ref_sod_flux(u, flux)
+    rng = np.random.default_rng(1)
+    noises = []
+    for _ in range(2):
+        u2 = u.copy()
+        m = rng.random(u.shape) < 0.5
+        u2[m] = np.nextafter(u2[m], np.inf)
+        noises.append(relerr(ref_sod_flux(u2, flux), ur))
+    print("flux", flux, "oracle vs reference", relerr(uo, ur), "reference noise", noises)
+    assert relerr(uo, ur) <= 3 * max(noises)
+
+
+@pytest.mark.parametrize("flux,name", [(1, "roe"), (2, "osher")])
+def test_roe_osher_golden(flux, name):
+    g = load_gold("solves")
+    uo = oracle_sod_flux(sod1d(), flux)
+    assert relerr(uo, g[f"sod_{name}_final"]) <= 3 * float(g[f"sod_{name}_noise"])
