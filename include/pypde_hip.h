@@ -65,7 +65,8 @@ void pypde_default_options(pypde_options *opts);
  * with the three callbacks replaced by `src`, followed by options/statistics.
  * u (prod(nX) x V, row-major, variables fastest) is overwritten with the final
  * state, as in the reference (api.cpp:18, iterator.cpp:129); ret receives ndt
- * snapshot rows (iterator.cpp:136-150).  FLUX: 0 Rusanov (only).
+ * snapshot rows (iterator.cpp:136-150).  FLUX: 0 Rusanov, 1 Roe, 2 Osher
+ * (fluxes.h:6-8).
  * secondOrder (viscous F(Q, dQ)) is not supported. */
 int pde_solver_hip(const char *src, bool useF, bool useB, bool useS, double *u, double tf,
                    int *nX, int ndim, double *dX, double CFL, int *boundaryTypes, bool STIFF,
@@ -158,9 +159,10 @@ int pypde_dev_memset(void *p, int value, size_t bytes);
 int pypde_dev_sync(void);
 int pypde_device_count(void);
 
-/* Compile (or fetch from the code-object cache) without touching a GPU. */
+/* Compile (or fetch from the code-object cache) without touching a GPU; FLUX as
+ * in pde_solver_hip. */
 int pypde_hip_compile(const char *src, bool useF, bool useB, bool useS, int ndim, int N, int V,
-                      bool STIFF, const pypde_options *opts);
+                      bool STIFF, int FLUX, const pypde_options *opts);
 /* Tables exactly as the kernels see them (for checks). */
 int pypde_hip_tables(int N, double *nodes, double *wghts, double *endvals, double *dervals,
                      double *weno_inv, double *sig, double *dgmat, double *dginv, double *fl_c,

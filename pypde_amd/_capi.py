@@ -75,7 +75,7 @@ def lib():
         L.pypde_stream.argtypes = [C.c_void_p]
         L.pypde_destroy.argtypes = [C.c_void_p]
         L.pypde_hip_compile.argtypes = [C.c_char_p, C.c_bool, C.c_bool, C.c_bool, C.c_int,
-                                        C.c_int, C.c_int, C.c_bool, C.POINTER(Options)]
+                                        C.c_int, C.c_int, C.c_bool, C.c_int, C.POINTER(Options)]
         L.pypde_hip_tables.argtypes = [C.c_int] + [PD] * 10
         L.pypde_ensemble_create.restype = C.c_void_p
         L.pypde_ensemble_create.argtypes = [C.c_int, C.c_char_p, C.c_bool, C.c_bool, C.c_bool, PI,
@@ -134,9 +134,10 @@ def make_options(newton_tol=1e-12, newton_maxit=100, max_steps=0, reflect=None, 
     return o
 
 
-def compile_only(src, useF, useB, useS, ndim, N, V, stiff=False, use_M=False, b_quirk=0):
+def compile_only(src, useF, useB, useS, ndim, N, V, stiff=False, use_M=False, b_quirk=0, flux=0):
     o = make_options(use_M=use_M, b_quirk=b_quirk)
-    rc = lib().pypde_hip_compile(src.encode(), useF, useB, useS, ndim, N, V, stiff, C.byref(o))
+    rc = lib().pypde_hip_compile(src.encode(), useF, useB, useS, ndim, N, V, stiff, int(flux),
+                                 C.byref(o))
     check(rc, f"hipRTC compile (ndim={ndim}, N={N}, V={V})")
 
 

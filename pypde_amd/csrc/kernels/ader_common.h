@@ -549,3 +549,412 @@ DEV void b_times(const double* b, const double* dqd, double* out) {
     }
   }
 }
+
+// ------------------------------------------------ |A| x for Roe and Osher
+#ifndef PDE_FLUX
+#define PDE_FLUX 0
+#endif
+#if PDE_FLUX != 0
+DEV double guard_div(double w, double tn) { return (w == 0.0) ? 2.220446049250313e-16 * tn : w; }
+
+// Real Schur form T = Z^T H Z (H overwritten by T): the QR of max_abs_eig
+// applied to the whole matrix with the transformations accumulated in Z,
+// exceptional shifts restored on the diagonal as blocks deflate, and 2 x 2
+// blocks with real eigenvalues split by a rotation (EISPACK hqr2 arrangement).
+DEV void real_schur(double* H, double* Z) {
+#pragma unroll
+  for (int i = 0; i < V * V; i++) Z[i] = (i % (V + 1) == 0) ? 1.0 : 0.0;
+  static_for<0, (V >= 3 ? V - 2 : 0)>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    constexpr int m = V - k - 1;
+    double x[m], vv[m];
+#pragma unroll
+    for (int i = 0; i < m; i++) x[i] = H[(k + 1 + i) * V + k];
+    double beta = house<m>(x, vv);
+    if (beta != 0.0) {
+      house_left<m, k + 1>(H, vv, beta, k, V - 1);
+      house_right<m, k + 1>(H, vv, beta, 0, V - 1);
+      house_right<m, k + 1>(Z, vv, beta, 0, V - 1);
+    }
+#pragma unroll
+    for (int i = k + 2; i < V; i++) H[i * V + k] = 0.0;
+  });
+  double anorm = 0.0;
+#pragma unroll
+  for (int i = 0; i < V * V; i++) anorm = anorm + dabs(H[i]);
+  double ex = 0.0;
+  int hi = V - 1, iter = 0, total = 0;
+  while (hi >= 0) {
+    int lo = 0;
+    bool found = false;
+#pragma unroll
+    for (int l = V - 1; l >= 1; l--) {
+      if (!found && l <= hi) {
+        double s = dabs(H[(l - 1) * V + l - 1]) + dabs(H[l * V + l]);
+        if (s == 0.0) s = anorm;
+        if (dabs(H[l * V + l - 1]) <= 2.220446049250313e-16 * s) {
+          H[l * V + l - 1] = 0.0;
+          lo = l;
+          found = true;
+        }
+      }
+    }
+    if (lo == hi) {
+#pragma unroll
+      for (int i = 0; i < V; i++)
+        if (i == hi) H[i * V + i] = H[i * V + i] + ex;
+      hi -= 1;
+      iter = 0;
+      continue;
+    }
+    if (lo == hi - 1) {
+      static_for<1, V>([&](auto hc) {
+        constexpr int hh = decltype(hc)::value;
+        constexpr int na = hh - 1;
+        if (hh == hi) {
+          H[na * V + na] = H[na * V + na] + ex;
+          H[hh * V + hh] = H[hh * V + hh] + ex;
+          double a = H[na * V + na], b = H[na * V + hh], c = H[hh * V + na], d = H[hh * V + hh];
+          double p = 0.5 * (a - d);
+          double disc = p * p + b * c;
+          if (disc >= 0.0) {
+            double z = dsqrt(disc);
+            z = (p >= 0.0) ? p + z : p - z;
+            double xr = c, sr = dabs(xr) + dabs(z);
+            double pr = xr / sr, qr = z / sr;
+            double rr = dsqrt(pr * pr + qr * qr);
+            pr = pr / rr;
+            qr = qr / rr;
+#pragma unroll
+            for (int j = na; j < V; j++) {
+              double t = H[na * V + j];
+              H[na * V + j] = qr * t + pr * H[hh * V + j];
+              H[hh * V + j] = qr * H[hh * V + j] - pr * t;
+            }
+#pragma unroll
+            for (int i = 0; i <= hh; i++) {
+              double t = H[i * V + na];
+              H[i * V + na] = qr * t + pr * H[i * V + hh];
+              H[i * V + hh] = qr * H[i * V + hh] - pr * t;
+            }
+#pragma unroll
+            for (int i = 0; i < V; i++) {
+              double t = Z[i * V + na];
+              Z[i * V + na] = qr * t + pr * Z[i * V + hh];
+              Z[i * V + hh] = qr * Z[i * V + hh] - pr * t;
+            }
+            H[hh * V + na] = 0.0;
+          }
+        }
+      });
+      hi -= 2;
+      iter = 0;
+      continue;
+    }
+    if constexpr (V >= 3) {
+      if (total >= 40 * V) {
+#pragma unroll
+        for (int i = 0; i < V; i++)
+          if (i <= hi) H[i * V + i] = H[i * V + i] + ex;
+        break;
+      }
+      double sx = hband<0>(H, hi), sy = hband<0>(H, hi - 1);
+      double sw = hband<-1>(H, hi) * hband<1>(H, hi - 1);
+      if (iter == 10) {
+        ex = ex + sx;
+#pragma unroll
+        for (int i = 0; i < V; i++)
+          if (i <= hi) H[i * V + i] = H[i * V + i] - sx;
+        double s = dabs(hband<-1>(H, hi)) + dabs(hband<-1>(H, hi - 1));
+        sx = 0.75 * s;
+        sy = sx;
+        sw = -0.4375 * s * s;
+      } else if (iter == 30) {
+        double s = (sy - sx) / 2.0;
+        s = s * s + sw;
+        if (s > 0.0) {
+          s = dsqrt(s);
+          if (sy < sx) s = -s;
+          s = s + (sy - sx) / 2.0;
+          s = sx - sw / s;
+          ex = ex + s;
+#pragma unroll
+          for (int i = 0; i < V; i++)
+            if (i <= hi) H[i * V + i] = H[i * V + i] - s;
+          sx = 0.964;
+          sy = 0.964;
+          sw = 0.964;
+        }
+      }
+      iter++;
+      total++;
+      double h00 = hband<0>(H, lo);
+      double r = sx - h00, q = sy - h00;
+      double x[3], v[3];
+      x[0] = (r * q - sw) / hband<-1>(H, lo + 1) + hband<1>(H, lo);
+      x[1] = hband<0>(H, lo + 1) - h00 - r - q;
+      x[2] = hband<-1>(H, lo + 2);
+      static_for<0, (V >= 3 ? V - 2 : 0)>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if (k >= lo && k <= hi - 2) {
+          if constexpr (k >= 1) {
+            if (k > lo) {
+              x[0] = H[k * V + k - 1];
+              x[1] = H[(k + 1) * V + k - 1];
+              x[2] = H[(k + 2) * V + k - 1];
+            }
+          }
+          double beta = house<3>(x, v);
+          if (beta != 0.0) {
+            if constexpr (k >= 1) {
+              if (k > lo) {
+                H[k * V + k - 1] = dsqrt(x[0] * x[0] + (x[1] * x[1] + x[2] * x[2]));
+                H[(k + 1) * V + k - 1] = 0.0;
+                H[(k + 2) * V + k - 1] = 0.0;
+              }
+            }
+            house_left<3, k>(H, v, beta, k, V - 1);
+            int rr = (k + 3 < hi) ? k + 3 : hi;
+            house_right<3, k>(H, v, beta, 0, rr);
+            house_right<3, k>(Z, v, beta, 0, V - 1);
+          }
+        }
+      });
+      static_for<2, (V >= 3 ? V : 2)>([&](auto hc) {
+        constexpr int hh = decltype(hc)::value;
+        if (hh == hi) {
+          x[0] = H[(hh - 1) * V + hh - 2];
+          x[1] = H[hh * V + hh - 2];
+          double beta = house<2>(x, v);
+          if (beta != 0.0) {
+            H[(hh - 1) * V + hh - 2] = dsqrt(x[0] * x[0] + x[1] * x[1]);
+            H[hh * V + hh - 2] = 0.0;
+            house_left<2, hh - 1>(H, v, beta, hh - 1, V - 1);
+            house_right<2, hh - 1>(H, v, beta, 0, hh);
+            house_right<2, hh - 1>(Z, v, beta, 0, V - 1);
+          }
+        }
+      });
+    }
+  }
+}
+
+// y = |A| x = R |Lambda| R^-1 x (fluxes.cpp:20-70), in real arithmetic: from
+// the real Schur form, a basis Y of each invariant subspace of T (eigenvector
+// of a real eigenvalue; real and imaginary parts of the eigenvector of a
+// complex pair, on which |A| is |lambda| I), W = Z Y, y = W D W^-1 x.
+DEV void abs_apply(const double* A, const double* xin, double* y) {
+  double T[V * V], Zm[V * V], Y[V * V], D[V];
+  double scale = 0.0;
+#pragma unroll
+  for (int i = 0; i < V * V; i++) scale = dmax(scale, dabs(A[i]));
+  if (!(scale >= 2.2250738585072014e-308)) {
+#pragma unroll
+    for (int i = 0; i < V; i++) y[i] = (scale != scale) ? scale : 0.0;
+    return;
+  }
+  int e2;
+  (void)__builtin_frexp(scale, &e2);
+  const double pw = __builtin_ldexp(1.0, -e2);
+#pragma unroll
+  for (int i = 0; i < V * V; i++) T[i] = A[i] * pw;
+  real_schur(T, Zm);
+  double tn = 0.0;
+#pragma unroll
+  for (int i = 0; i < V * V; i++) tn = tn + dabs(T[i]);
+#pragma unroll
+  for (int i = 0; i < V * V; i++) Y[i] = 0.0;
+  bool skip = false;
+  static_for<0, V>([&](auto ic) {
+    constexpr int en = V - 1 - decltype(ic)::value;
+    if (skip) {
+      skip = false;
+      return;
+    }
+    bool cpx = false;
+    if constexpr (en >= 1) cpx = T[en * V + en - 1] != 0.0;
+    if (cpx) {
+      if constexpr (en >= 1) {
+        constexpr int na = en - 1;
+        double a = 0.5 * (T[na * V + na] + T[en * V + en]);
+        double pp = 0.5 * (T[na * V + na] - T[en * V + en]);
+        double bb = dsqrt(-(pp * pp + T[na * V + en] * T[en * V + na]));
+        D[na] = __builtin_ldexp(dsqrt(a * a + bb * bb), e2);
+        D[en] = D[na];
+        double yr[V], yi[V];
+#pragma unroll
+        for (int i = 0; i < V; i++) {
+          yr[i] = 0.0;
+          yi[i] = 0.0;
+        }
+        yr[en] = 1.0;
+        yr[na] = (a - T[en * V + en]) / T[en * V + na];
+        yi[na] = bb / T[en * V + na];
+        bool skip2 = false;
+        static_for<0, na>([&](auto jc) {
+          constexpr int i = na - 1 - decltype(jc)::value;
+          if (skip2) {
+            skip2 = false;
+            return;
+          }
+          bool blk = false;
+          if constexpr (i >= 1) blk = T[i * V + i - 1] != 0.0;
+          if (blk) {
+            if constexpr (i >= 1) {
+              constexpr int im = i - 1;
+              double s1r = 0.0, s1i = 0.0, s2r = 0.0, s2i = 0.0;
+#pragma unroll
+              for (int j = i + 1; j <= en; j++) {
+                s1r = dfma(T[im * V + j], yr[j], s1r);
+                s1i = dfma(T[im * V + j], yi[j], s1i);
+                s2r = dfma(T[i * V + j], yr[j], s2r);
+                s2i = dfma(T[i * V + j], yi[j], s2i);
+              }
+              double m11r = T[im * V + im] - a, m11i = -bb, m12 = T[im * V + i];
+              double m21 = T[i * V + im], m22r = T[i * V + i] - a, m22i = -bb;
+              double detr = (m11r * m22r - m11i * m22i) - m12 * m21;
+              double deti = m11r * m22i + m11i * m22r;
+              if (detr == 0.0 && deti == 0.0) detr = 2.220446049250313e-16 * tn;
+              double n1r = -(s1r * m22r - s1i * m22i) + s2r * m12;
+              double n1i = -(s1r * m22i + s1i * m22r) + s2i * m12;
+              double n2r = -(s2r * m11r - s2i * m11i) + s1r * m21;
+              double n2i = -(s2r * m11i + s2i * m11r) + s1i * m21;
+              double dd = detr * detr + deti * deti;
+              yr[im] = (n1r * detr + n1i * deti) / dd;
+              yi[im] = (n1i * detr - n1r * deti) / dd;
+              yr[i] = (n2r * detr + n2i * deti) / dd;
+              yi[i] = (n2i * detr - n2r * deti) / dd;
+              skip2 = true;
+            }
+          } else {
+            double sr = 0.0, si = 0.0;
+#pragma unroll
+            for (int j = i + 1; j <= en; j++) {
+              sr = dfma(T[i * V + j], yr[j], sr);
+              si = dfma(T[i * V + j], yi[j], si);
+            }
+            double wr = guard_div(T[i * V + i] - a, tn), wi = -bb;
+            double dd = wr * wr + wi * wi;
+            yr[i] = -(sr * wr + si * wi) / dd;
+            yi[i] = -(si * wr - sr * wi) / dd;
+          }
+        });
+#pragma unroll
+        for (int r = 0; r < V; r++) {
+          Y[r * V + na] = yr[r];
+          Y[r * V + en] = yi[r];
+        }
+        skip = true;
+      }
+    } else {
+      const double lam = T[en * V + en];
+      D[en] = __builtin_ldexp(dabs(lam), e2);
+      double yv[V];
+#pragma unroll
+      for (int i = 0; i < V; i++) yv[i] = 0.0;
+      yv[en] = 1.0;
+      bool skip2 = false;
+      static_for<0, en>([&](auto jc) {
+        constexpr int i = en - 1 - decltype(jc)::value;
+        if (skip2) {
+          skip2 = false;
+          return;
+        }
+        bool blk = false;
+        if constexpr (i >= 1) blk = T[i * V + i - 1] != 0.0;
+        if (blk) {
+          if constexpr (i >= 1) {
+            constexpr int im = i - 1;
+            double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+            for (int j = i + 1; j <= en; j++) {
+              s1 = dfma(T[im * V + j], yv[j], s1);
+              s2 = dfma(T[i * V + j], yv[j], s2);
+            }
+            double m11 = T[im * V + im] - lam, m12 = T[im * V + i];
+            double m21 = T[i * V + im], m22 = T[i * V + i] - lam;
+            double det = guard_div(m11 * m22 - m12 * m21, tn);
+            yv[im] = (-s1 * m22 + s2 * m12) / det;
+            yv[i] = (-s2 * m11 + s1 * m21) / det;
+            skip2 = true;
+          }
+        } else {
+          double sacc = 0.0;
+#pragma unroll
+          for (int j = i + 1; j <= en; j++) sacc = dfma(T[i * V + j], yv[j], sacc);
+          yv[i] = -sacc / guard_div(T[i * V + i] - lam, tn);
+        }
+      });
+#pragma unroll
+      for (int r = 0; r < V; r++) Y[r * V + en] = yv[r];
+    }
+  });
+  // W = Z Y; b = W^-1 x (partial pivoting); b *= D; y = W b
+  double W[V * V], LU[V * V], b[V];
+#pragma unroll
+  for (int i = 0; i < V; i++)
+#pragma unroll
+    for (int j = 0; j < V; j++) {
+      double acc = Zm[i * V] * Y[j];
+#pragma unroll
+      for (int k = 1; k < V; k++) acc = dfma(Zm[i * V + k], Y[k * V + j], acc);
+      W[i * V + j] = acc;
+    }
+#pragma unroll
+  for (int i = 0; i < V * V; i++) LU[i] = W[i];
+#pragma unroll
+  for (int i = 0; i < V; i++) b[i] = xin[i];
+#pragma unroll
+  for (int k = 0; k < V; k++) {
+    int p = k;
+    double best = dabs(LU[k * V + k]);
+#pragma unroll
+    for (int i = k + 1; i < V; i++) {
+      double a = dabs(LU[i * V + k]);
+      if (a > best) {
+        best = a;
+        p = i;
+      }
+    }
+#pragma unroll
+    for (int i = k + 1; i < V; i++) {
+      if (p == i) {
+#pragma unroll
+        for (int j = 0; j < V; j++) {
+          double t = LU[k * V + j];
+          LU[k * V + j] = LU[i * V + j];
+          LU[i * V + j] = t;
+        }
+        double t = b[k];
+        b[k] = b[i];
+        b[i] = t;
+      }
+    }
+    const double piv_k = guard_div(LU[k * V + k], 1.0);
+#pragma unroll
+    for (int i = k + 1; i < V; i++) {
+      double l = LU[i * V + k] / piv_k;
+      LU[i * V + k] = l;
+#pragma unroll
+      for (int j = k + 1; j < V; j++) LU[i * V + j] = dfma(-l, LU[k * V + j], LU[i * V + j]);
+      b[i] = dfma(-l, b[k], b[i]);
+    }
+  }
+#pragma unroll
+  for (int i = V - 1; i >= 0; i--) {
+    double acc = b[i];
+#pragma unroll
+    for (int j = i + 1; j < V; j++) acc = dfma(-LU[i * V + j], b[j], acc);
+    b[i] = acc / guard_div(LU[i * V + i], 1.0);
+  }
+#pragma unroll
+  for (int i = 0; i < V; i++) b[i] = b[i] * D[i];
+#pragma unroll
+  for (int i = 0; i < V; i++) {
+    double acc = W[i * V] * b[0];
+#pragma unroll
+    for (int k = 1; k < V; k++) acc = dfma(W[i * V + k], b[k], acc);
+    y[i] = acc;
+  }
+}
+#endif

@@ -899,11 +899,45 @@ DEV void flux_dir(const double* __restrict__ tr, double* __restrict__ ff, double
 #pragma unroll
     for (int i = 0; i < D * V; i++) dq0[i] = 0.0;
     double fl[V], fr[V];
+#if PDE_FLUX == 0
+    // Rusanov: max(lam(qL), lam(qR)) (qL - qR) + F(qL) + F(qR)  (fluxes.cpp:72-101)
     double l1 = max_abs_eigs(qL, dq0, d, fl);
     double l2 = max_abs_eigs(qR, dq0, d, fr);
     double s = dmax(l1, l2);
 #pragma unroll
     for (int v = 0; v < V; v++) ff[o + (long)v * nf] = s * (qL[v] - qR[v]) + (fl[v] + fr[v]);
+#else
+    // Roe: |sum_i W_i A(q_i)| (qL - qR); Osher: sum_i W_i |A(q_i)| (qL - qR), on
+    // the path q_i = qR + x_i (qL - qR)  (fluxes.cpp:20-70)
+    F(fl, qL, dq0, d);
+    F(fr, qR, dq0, d);
+    double Dq[V], q[V], A[V * V], ftmp[V], Dsum[V];
+#if PDE_FLUX == 1
+    double Mx[V * V];
+#endif
+#pragma unroll
+    for (int v = 0; v < V; v++) Dq[v] = qL[v] - qR[v];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+#pragma unroll
+      for (int v = 0; v < V; v++) q[v] = dfma(T_X[i], Dq[v], qR[v]);
+      system_matrix(q, dq0, d, A, ftmp);
+#if PDE_FLUX == 1
+#pragma unroll
+      for (int k = 0; k < V * V; k++) Mx[k] = (i == 0) ? T_W[0] * A[k] : dfma(T_W[i], A[k], Mx[k]);
+#else
+      double yv[V];
+      abs_apply(A, Dq, yv);
+#pragma unroll
+      for (int v = 0; v < V; v++) Dsum[v] = (i == 0) ? T_W[0] * yv[v] : dfma(T_W[i], yv[v], Dsum[v]);
+#endif
+    }
+#if PDE_FLUX == 1
+    abs_apply(Mx, Dq, Dsum);
+#endif
+#pragma unroll
+    for (int v = 0; v < V; v++) ff[o + (long)v * nf] = Dsum[v] + (fl[v] + fr[v]);
+#endif
   }
 #endif
 #if PDE_USE_B

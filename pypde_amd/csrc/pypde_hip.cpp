@@ -108,6 +108,7 @@ struct Spec {
   std::string src;
   bool useF, useB, useS, useM, stiff, guess;
   int D, N, V, b_quirk;
+  int flux = 0;  // 0 Rusanov, 1 Roe, 2 Osher (fluxes.h:6-8)
 };
 
 std::string full_source(const Spec& s, const pypde::Tables& T) {
@@ -118,7 +119,7 @@ std::string full_source(const Spec& s, const pypde::Tables& T) {
   o << "#define PDE_USE_S " << (int)s.useS << "\n#define PDE_USE_M " << (int)(s.useS && s.useM)
     << "\n";
   o << "#define PDE_STIFF " << (int)s.stiff << "\n#define PDE_B_QUIRK " << s.b_quirk << "\n";
-  o << "#define PDE_GUESS " << (int)s.guess << "\n";
+  o << "#define PDE_GUESS " << (int)s.guess << "\n#define PDE_FLUX " << s.flux << "\n";
   o << "constexpr int ipow_c(int b, int e) { return e <= 0 ? 1 : b * ipow_c(b, e - 1); }\n";
   o << pypde::tables_source(T, s.D);
   o << PRELUDE_SRC << "\n";
@@ -222,7 +223,8 @@ int check_spec(const Spec& s, bool secondOrder, int FLUX) {
   if (nd > 64) return fail(PYPDE_ERR_UNSUPPORTED, "N^ndim=%d > 64 nodes per cell", nd);
   if (s.V < 1 || s.V > MAXV) return fail(PYPDE_ERR_ARG, "V=%d outside 1..%d", s.V, MAXV);
   if (secondOrder) return fail(PYPDE_ERR_UNSUPPORTED, "second-order (viscous) F not supported");
-  if (FLUX != 0) return fail(PYPDE_ERR_UNSUPPORTED, "FLUX=%d: only Rusanov (0) is supported", FLUX);
+  if (FLUX < 0 || FLUX > 2)
+    return fail(PYPDE_ERR_ARG, "FLUX=%d: 0 Rusanov, 1 Roe or 2 Osher (fluxes.h:6-8)", FLUX);
   if (!s.src.size() && (s.useF || s.useB || s.useS))
     return fail(PYPDE_ERR_ARG, "empty system source");
   return 0;
@@ -470,10 +472,11 @@ int pypde_hip_tables(int N, double* nodes, double* wghts, double* endvals, doubl
 }
 
 int pypde_hip_compile(const char* src, bool useF, bool useB, bool useS, int ndim, int N, int V,
-                      bool STIFF, const pypde_options* opts) {
+                      bool STIFF, int FLUX, const pypde_options* opts) {
   Spec s;
   make_spec(&s, src, useF, useB, useS, ndim, N, V, STIFF, opts);
-  int rc = check_spec(s, false, 0);
+  s.flux = FLUX;
+  int rc = check_spec(s, false, FLUX);
   if (rc) return rc;
   std::vector<char> code;
   if ((rc = compile_code(s, &code))) return rc;
@@ -519,6 +522,7 @@ pypde_ctx* pypde_create(const char* src, bool useF, bool useB, bool useS, int* n
   Spec s;
   make_spec(&s, src, useF, useB, useS, ndim, N, V, STIFF, &opts);
   s.guess = opts.guess != nullptr;
+  s.flux = FLUX;
   if ((rc = check_spec(s, secondOrder, FLUX))) return nullptr;
   if (!nX || !dX || !boundaryTypes) {
     rc = fail(PYPDE_ERR_ARG, "null grid argument");

@@ -113,7 +113,7 @@ class Solver:
 
     def __init__(self, u0, L, system, ndim=None, order=3, boundaryTypes="transitive", cfl=0.9,
                  stiff=False, reflect=None, newton_tol=1e-12, newton_maxit=100, b_quirk=False,
-                 profile=False, device=0, guess=None, use_graph=False):
+                 profile=False, device=0, guess=None, use_graph=False, flux="rusanov"):
         u0 = np.ascontiguousarray(u0, dtype=np.float64)
         self.shape = u0.shape
         self.nX = np.array(u0.shape[:-1], dtype=np.int32)
@@ -131,7 +131,7 @@ class Solver:
         rc = C.c_int(0)
         self.h = _capi.lib().pypde_create(src.encode(), useF, useB, useS, _capi.ip(self.nX), nd,
                                           _capi.dp(self.dX), float(cfl), _capi.ip(self.bcs),
-                                          bool(stiff), 0, int(order), V, False,
+                                          bool(stiff), FLUXES[flux], int(order), V, False,
                                           C.byref(self.opts), C.byref(rc))
         if not self.h:
             _capi.check(rc.value or 3, "pypde_create")
@@ -185,7 +185,8 @@ class Ensemble:
     each of shape (n_x, [n_y,] V); ``L``: list of domain lengths per member."""
 
     def __init__(self, grids, L, system, order=3, boundaryTypes="transitive", cfl=0.9,
-                 stiff=False, reflect=None, newton_tol=1e-12, newton_maxit=100, device=0):
+                 stiff=False, reflect=None, newton_tol=1e-12, newton_maxit=100, device=0,
+                 flux="rusanov"):
         grids = [np.ascontiguousarray(g, dtype=np.float64) for g in grids]
         self.shapes = [g.shape for g in grids]
         nd = len(self.shapes[0]) - 1
@@ -204,7 +205,8 @@ class Ensemble:
         rc = C.c_int(0)
         self.h = _capi.lib().pypde_ensemble_create(len(grids), src.encode(), useF, useB, useS,
                                                    _capi.ip(nX), nd, _capi.dp(dX), float(cfl),
-                                                   _capi.ip(bcs), bool(stiff), 0, int(order), V,
+                                                   _capi.ip(bcs), bool(stiff), FLUXES[flux],
+                                                   int(order), V,
                                                    C.byref(self.opts), C.byref(rc))
         if not self.h:
             _capi.check(rc.value or 3, "pypde_ensemble_create")

@@ -41,7 +41,13 @@ def standard_configs(full=False):
     for name, ndim, N, stiff in combos:
         s = SYSTEMS[name]
         out.append(dict(src=source(name), useF=s["F"], useB=s["B"], useS=s["S"], ndim=ndim, N=N,
-                        V=s["V"], stiff=stiff, use_M=s["M"]))
+                        V=s["V"], stiff=stiff, use_M=s["M"], flux=0))
+    # Roe (1) and Osher (2) fluxes
+    for name, ndim, N in ([("euler1d", 1, 3), ("euler2d", 2, 3)] if full else []):
+        s = SYSTEMS[name]
+        for flux in (1, 2):
+            out.append(dict(src=source(name), useF=s["F"], useB=s["B"], useS=s["S"], ndim=ndim,
+                            N=N, V=s["V"], stiff=False, use_M=s["M"], flux=flux))
     return out
 
 

@@ -53,10 +53,10 @@ def test_argument_errors_are_reported():
                    boundaryTypes="bogus")
     o = _capi.make_options()
     rc = _capi.lib().pypde_hip_compile(systems.source("euler2d").encode(), True, False, False,
-                                       3, 3, 4, False, C.byref(o))
+                                       3, 3, 4, False, 0, C.byref(o))
     assert rc == 4 and "ndim=3" in _capi.last_error()
     rc = _capi.lib().pypde_hip_compile(b"this is not C", True, False, False, 1, 3, 3, False,
-                                       C.byref(o))
+                                       0, C.byref(o))
     assert rc == 2 and "error" in _capi.last_error()
 
 
@@ -325,3 +325,23 @@ def test_predictor_schedules_agree():
     assert np.array_equal(ua, ub)
     for k in ("steps", "pred_iters", "pred_cells", "pred_maxit", "pred_capped"):
         assert sa[k] == sb[k], k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("flux", ["roe", "osher"])
+def test_roe_osher_match_oracle(flux):
+    """Roe and Osher fluxes (fluxes.cpp:20-70): 1D Sod (C0) and a 2D periodic
+    Euler field, GPU against the oracle."""
+    code = {"roe": 1, "osher": 2}[flux]
+    u0 = systems.sod1d()
+    uo, _, st, rc = oracle_run("euler1d", u0, 0.2, [1.0], 0, 3, False, flux=code)
+    ug, _, sg = gpu_run("euler1d", u0, 0.2, [1.0], "transitive", 3, False, flux=flux)
+    print(f"{flux} sod: steps {sg['steps']} err {relerr(ug, uo):.3e}")
+    assert rc == 0 and sg["rc"] == 0 and sg["steps"] == st.steps
+    assert relerr(ug, uo) <= TOL
+    u0 = systems.euler2d_modes(24, seed=4)
+    uo, _, st, rc = oracle_run("euler2d", u0, 0.03, [1.0, 1.0], 1, 3, False, flux=code)
+    ug, _, sg = gpu_run("euler2d", u0, 0.03, [1.0, 1.0], "periodic", 3, False, flux=flux)
+    print(f"{flux} euler2d: steps {sg['steps']} err {relerr(ug, uo):.3e}")
+    assert rc == 0 and sg["steps"] == st.steps
+    assert relerr(ug, uo) <= TOL
