@@ -727,6 +727,27 @@ static double max_abs_eigs(oracle_F F, oracle_B B, const double *q,
   return oracle_max_abs_eig(A, V);
 }
 
+/* Viscous wave speed (system.cpp:53-61, NumericalDiff.h:6-44 with
+ * secondOrder): 2 (N + 1) / dX_d max|lambda(dF_d / d(dQ_d))|, the Jacobian by
+ * forward differences in the gradient components dQ[d V + i]. */
+static double max_abs_eigs_second_order(oracle_F F, const double *q, const double *dq,
+                                        int d, int V, int N, const double *dX) {
+  double qq[MAXV], dqq[MAXD * MAXV], v1[MAXV], v2[MAXV], A[MAXV * MAXV];
+  memcpy(qq, q, V * sizeof(double));
+  memcpy(dqq, dq, MAXD * V * sizeof(double));
+  F(v1, qq, dqq, d);
+  for (int i = 0; i < V; i++) {
+    double qi0 = dqq[d * V + i];
+    double h = fmax(FD_EPS * fabs(qi0), FD_EPS);
+    dqq[d * V + i] = qi0 + h;
+    F(v2, qq, dqq, d);
+    dqq[d * V + i] = qi0;
+    double rh = 1.0 / h;
+    for (int r = 0; r < V; r++) A[r * V + i] = (v2[r] - v1[r]) * rh;
+  }
+  return (double)(2 * (N + 1)) / dX[d] * oracle_max_abs_eig(A, V);
+}
+
 /* ------------------------------------------- |A| x for Roe and Osher fluxes */
 
 /* Real Schur form T = Z^T H Z of an n x n matrix (H is overwritten by T, Z
@@ -1225,7 +1246,7 @@ int oracle_weno(double *wh, const double *u, const int *nX, int ndim, int N,
  * weights, MAX = max_cells sum_d lambda_d / dX_d, dt = CFL / MAX, x0.2 while
  * count <= 5, clamped to tf - t. */
 static double cell_speed(const Tab *T, oracle_F F, oracle_B B,
-                         const double *wc, const double *dX) {
+                         const double *wc, const double *dX, int second) {
   int V = T->V, D = T->D, Nd = T->Nd;
   double q[MAXV], dq[MAXD * MAXV] = {0}, tmp[MAXV];
   for (int v = 0; v < V; v++) {
@@ -1245,26 +1266,33 @@ static double cell_speed(const Tab *T, oracle_F F, oracle_B B,
   double sum = 0.0;
   for (int d = 0; d < D; d++) {
     double lam = max_abs_eigs(F, B, q, dq, d, V, NULL);
+    if (second) lam = lam + max_abs_eigs_second_order(F, q, dq, d, V, T->N, dX);
     sum = sum + lam / dX[d];
   }
   return sum;
 }
 
-double oracle_timestep(oracle_F F, oracle_B B, const double *wh, long nw,
-                       const double *dX, int ndim, int N, int V, double CFL,
-                       double tf, double t, long count) {
+static double timestep_impl(oracle_F F, oracle_B B, const double *wh, long nw,
+                            const double *dX, int ndim, int N, int V, double CFL,
+                            double tf, double t, long count, int second) {
   Tab T;
   if (tab_init(&T, N, ndim, V)) return NAN;
   double MAX = 0.0;
 #pragma omp parallel for reduction(max : MAX) schedule(static)
   for (long c = 0; c < nw; c++) {
-    double s = cell_speed(&T, F, B, wh + c * T.Nd * V, dX);
+    double s = cell_speed(&T, F, B, wh + c * T.Nd * V, dX, second);
     if (s > MAX) MAX = s;
   }
   double dt = CFL / MAX;
   if (count <= 5) dt = dt * 0.2;
   if (t + dt > tf) return tf - t;
   return dt;
+}
+
+double oracle_timestep(oracle_F F, oracle_B B, const double *wh, long nw,
+                       const double *dX, int ndim, int N, int V, double CFL,
+                       double tf, double t, long count) {
+  return timestep_impl(F, B, wh, nw, dX, ndim, N, V, CFL, tf, t, count, 0);
 }
 
 /* ------------------------------------------------------------ predictor */
@@ -1693,32 +1721,61 @@ static void face_trace(const Tab *T, const double *qc, int d, int e, int t,
   }
 }
 
+/* Gradient at the face point: the derivatives of qh(t) along every d_ at the
+ * nodes of the line through transverse node j, extrapolated to side e
+ * (fv.cpp:87-122: qh_derivatives, then endpts); out[d_ V + v]. */
+static void face_grad_trace(const Tab *T, const double *qc, int d, int e, int t, int j,
+                            const double *dX, double *out) {
+  int N = T->N, V = T->V, D = T->D;
+  int stride = ipow(N, D - 1 - d);
+  int hi = j / stride, lo = j % stride;
+  int base = hi * stride * N + lo;
+  const double *ev = e ? T->end1 : T->end0;
+  const double *qt = qc + (long)t * T->Nd * V;
+  for (int d_ = 0; d_ < D; d_++) {
+    double g[MAXN][MAXV];
+    for (int m = 0; m < N; m++) node_deriv(T, qt, base + m * stride, d_, 1.0 / dX[d_], g[m]);
+    for (int v = 0; v < V; v++) {
+      double acc = ev[0] * g[0][v];
+      for (int m = 1; m < N; m++) acc = fma(ev[m], g[m][v], acc);
+      out[d_ * V + v] = acc;
+    }
+  }
+}
+
 /* f = D(qL, qR) + (F(qL) + F(qR)) (fluxes.cpp:79-101) with the dissipation
  *   Rusanov (0): max(lam(qL), lam(qR)) (qL - qR)           (fluxes.cpp:72-77)
  *   Roe (1):     |sum_i W_i A(q_i)| (qL - qR)               (fluxes.cpp:47-70)
  *   Osher (2):   sum_i W_i |A(q_i)| (qL - qR)               (fluxes.cpp:20-45)
- * on the path q_i = qR + x_i (qL - qR), A = dF/dQ + B; Bint =
- * (sum_i W_i B(qL + x_i (qR - qL))) (qR - qL) (fluxes.cpp:103-116). */
+ * on the path q_i = qR + x_i (qL - qR), dq_i = dqR + x_i (dqL - dqR),
+ * A = dF/dQ + B, and with a viscous F (second) + max(lam2(qL), lam2(qR))
+ * (qL - qR) (fluxes.cpp:96-100); Bint = (sum_i W_i B(qL + x_i (qR - qL)))
+ * (qR - qL) (fluxes.cpp:103-116). */
 static void face_flux(const Tab *T, oracle_F F, oracle_B B, int flux,
-                      const double *qL, const double *qR, int d, double *f,
+                      const double *qL, const double *qR, const double *dqL,
+                      const double *dqR, int second, const double *dX, int d, double *f,
                       double *bint) {
-  int V = T->V, N = T->N;
-  double dq0[MAXD * MAXV] = {0};
+  int V = T->V, N = T->N, D = T->D;
   if (F && flux == 0) {
     double fl[MAXV], fr[MAXV];
-    double l1 = max_abs_eigs(F, B, qL, dq0, d, V, fl);
-    double l2 = max_abs_eigs(F, B, qR, dq0, d, V, fr);
+    double l1 = max_abs_eigs(F, B, qL, dqL, d, V, fl);
+    double l2 = max_abs_eigs(F, B, qR, dqR, d, V, fr);
     double s = fmax(l1, l2);
     for (int v = 0; v < V; v++) f[v] = s * (qL[v] - qR[v]) + (fl[v] + fr[v]);
   } else if (F) {
     double fl[MAXV], fr[MAXV], Dq[MAXV], q[MAXV], A[MAXV * MAXV], Mx[MAXV * MAXV], y[MAXV],
-        Dsum[MAXV], ftmp[MAXV];
-    F(fl, (double *)qL, dq0, d);
-    F(fr, (double *)qR, dq0, d);
+        Dsum[MAXV], ftmp[MAXV], dq[MAXD * MAXV], qq[MAXV], dqq[MAXD * MAXV];
+    memcpy(qq, qL, V * sizeof(double));
+    memcpy(dqq, dqL, D * V * sizeof(double));
+    F(fl, qq, dqq, d);
+    memcpy(qq, qR, V * sizeof(double));
+    memcpy(dqq, dqR, D * V * sizeof(double));
+    F(fr, qq, dqq, d);
     for (int v = 0; v < V; v++) Dq[v] = qL[v] - qR[v];
     for (int i = 0; i < N; i++) {
       for (int v = 0; v < V; v++) q[v] = fma(T->x[i], Dq[v], qR[v]);
-      system_matrix(F, B, q, dq0, d, V, A, ftmp);
+      for (int k = 0; k < D * V; k++) dq[k] = fma(T->x[i], dqL[k] - dqR[k], dqR[k]);
+      system_matrix(F, B, q, dq, d, V, A, ftmp);
       if (flux == 1) {
         for (int k = 0; k < V * V; k++) Mx[k] = (i == 0) ? T->w[0] * A[k] : fma(T->w[i], A[k], Mx[k]);
       } else {
@@ -1730,6 +1787,12 @@ static void face_flux(const Tab *T, oracle_F F, oracle_B B, int flux,
     for (int v = 0; v < V; v++) f[v] = Dsum[v] + (fl[v] + fr[v]);
   } else {
     for (int v = 0; v < V; v++) f[v] = 0.0;
+  }
+  if (F && second) {
+    double m1 = max_abs_eigs_second_order(F, qL, dqL, d, V, N, dX);
+    double m2 = max_abs_eigs_second_order(F, qR, dqR, d, V, N, dX);
+    double s2 = fmax(m1, m2);
+    for (int v = 0; v < V; v++) f[v] = f[v] + s2 * (qL[v] - qR[v]);
   }
   if (B) {
     double Dq[MAXV], Mx[MAXV * MAXV], b[MAXV * MAXV], q[MAXV];
@@ -1789,9 +1852,9 @@ static void cell_volume(const Tab *T, oracle_F F, oracle_B B, oracle_S S,
  * node) and then the face-cell c+1 (c on the left, over t, then d, then
  * transverse node).  Face fluxes are evaluated once per face point and then
  * gathered per cell in exactly that order. */
-int oracle_fv(oracle_F F, oracle_B B, oracle_S S, double *u, const double *qh,
-              const int *nX, int ndim, const double *dX, int flux, int N, int V,
-              double dt, int b_quirk) {
+static int fv_impl(oracle_F F, oracle_B B, oracle_S S, double *u, const double *qh,
+                   const int *nX, int ndim, const double *dX, int flux, int N, int V,
+                   double dt, int b_quirk, int second) {
   Tab T;
   if (tab_init(&T, N, ndim, V)) return 1;
   int D = ndim, Nd1 = T.Nd1;
@@ -1830,11 +1893,15 @@ int oracle_fv(oracle_F F, oracle_B B, oracle_S S, double *u, const double *qh,
       }
       for (int t = 0; t < N; t++)
         for (int j = 0; j < Nd1; j++) {
-          double qL[MAXV], qR[MAXV];
+          double qL[MAXV], qR[MAXV], dqL[MAXD * MAXV] = {0}, dqR[MAXD * MAXV] = {0};
           face_trace(&T, qh + wl * NTV, d, 1, t, j, qL);
           face_trace(&T, qh + wr * NTV, d, 0, t, j, qR);
+          if (second) {
+            face_grad_trace(&T, qh + wl * NTV, d, 1, t, j, dX, dqL);
+            face_grad_trace(&T, qh + wr * NTV, d, 0, t, j, dX, dqR);
+          }
           long o = ((f * N + t) * Nd1 + j) * V;
-          face_flux(&T, F, B, flux, qL, qR, d, ff[d] + o, fb[d] + o);
+          face_flux(&T, F, B, flux, qL, qR, dqL, dqR, second, dX, d, ff[d] + o, fb[d] + o);
         }
     }
   }
@@ -1908,6 +1975,12 @@ int oracle_fv(oracle_F F, oracle_B B, oracle_S S, double *u, const double *qh,
   return 0;
 }
 
+int oracle_fv(oracle_F F, oracle_B B, oracle_S S, double *u, const double *qh,
+              const int *nX, int ndim, const double *dX, int flux, int N, int V,
+              double dt, int b_quirk) {
+  return fv_impl(F, B, S, u, qh, nX, ndim, dX, flux, N, V, dt, b_quirk, 0);
+}
+
 /* ---------------------------------------------------------------- driver */
 
 /* Time loop of iterator.cpp:38-151 (serial over the whole grid; the reference's
@@ -1922,7 +1995,6 @@ int oracle_pde_solver(oracle_F F, oracle_B B, oracle_S S, oracle_M M, int useF,
   if (!useB) B = NULL;
   if (!useS) S = NULL;
   if (!S) M = NULL;
-  if (secondOrder) return 10; /* viscous F(Q, dQ) path not restated */
   if (flux < 0 || flux > 2) return 11; /* Rusanov, Roe, Osher */
   if (ndt < 1) return 12;
   Tab T;
@@ -1949,10 +2021,10 @@ int oracle_pde_solver(oracle_F F, oracle_B B, oracle_S S, oracle_M M, int useF,
   while (t < tf && (max_steps <= 0 || count < max_steps)) {
     oracle_boundaries(ub, u, nX, ndim, bc, reflect, N, V);
     double *wh = weno_sweeps(&T, ub, nb);
-    double dt = oracle_timestep(F, B, wh, nw, dX, ndim, N, V, CFL, tf, t, count);
+    double dt = timestep_impl(F, B, wh, nw, dX, ndim, N, V, CFL, tf, t, count, secondOrder);
     predictor_impl(&T, F, B, S, M, wh, nw, dX, dt, stiff, opts, qh, NULL, &ps);
     free(wh);
-    oracle_fv(F, B, S, u, qh, nX, ndim, dX, flux, N, V, dt, opts ? opts->b_quirk : 0);
+    fv_impl(F, B, S, u, qh, nX, ndim, dX, flux, N, V, dt, opts ? opts->b_quirk : 0, secondOrder);
     t += dt;
     count += 1;
     if (push < ndt && t >= (double)(push + 1) / (double)ndt * tf) {

@@ -17,6 +17,8 @@ SYSTEMS = {
     "reactive2d": dict(V=5, F=True, B=False, S=True, M=True),
     "advect": dict(V=1, F=True, B=False, S=False, M=False),
     "ncadvect": dict(V=2, F=False, B=True, S=True, M=False),
+    "advdiff": dict(V=1, F=True, B=False, S=False, M=False, second=True),
+    "ns2d": dict(V=4, F=True, B=False, S=False, M=False, second=True),
 }
 
 

@@ -7,7 +7,8 @@ tables.npz  basis/WENO/DG tables for N = 1..5 (poly/basis.cpp, weno_matrices.cpp
 weno.npz    weno_solver (api.cpp:32-48) on seeded random data
 solves.npz  pde_solver end states: C0 Sod (order 3; Rusanov, Roe and Osher
             fluxes) plus the reference's own sensitivity on each to 1-ulp input
-            changes, and 24x24 periodic 2D Euler, orders 2-4
+            changes, 24x24 periodic 2D Euler, orders 2-4, and the viscous
+            path (advection-diffusion 1D, Navier-Stokes 2D)
 """
 import os
 import sys
@@ -75,6 +76,10 @@ def main():
         ur, _ = cc.ref_solve("euler2d", to.euler2d_field(24, 24, noise=0.0), 0.05, [1, 1], 1, N,
                              nthreads=8)
         s[f"e2d_final_{N}"] = ur
+    # viscous (second-order) path: advection-diffusion and 2D Navier-Stokes
+    s["advdiff_final"] = to.ref_viscous("advdiff", to.advdiff_init(64), 0.2, [1.0], 1, 3)
+    s["ns2d_final"] = to.ref_viscous("ns2d", to.systems.euler2d_modes(16, seed=2), 0.02,
+                                     [1.0, 1.0], 1, 3)
     np.savez_compressed(os.path.join(HERE, "solves.npz"), **s)
     print("golden vectors written:", sorted(os.listdir(HERE)))
 

@@ -642,3 +642,85 @@ def test_roe_osher_golden(flux, name):
     g = load_gold("solves")
     uo = oracle_sod_flux(sod1d(), flux)
     assert relerr(uo, g[f"sod_{name}_final"]) <= 3 * float(g[f"sod_{name}_noise"])
+
+
+def advdiff_init(n):
+    x = (np.arange(n) + 0.5) / n
+    return (1 + 0.5 * np.sin(2 * np.pi * x))[:, None]
+
+
+def oracle_viscous(name, u0, tf, L, bc, N, flux=0):
+    """Oracle pde_solver with secondOrder (F depends on dQ)."""
+    s = cc.system(name)
+    u = np.ascontiguousarray(u0).copy()
+    nX = np.array(u.shape[:-1], dtype=np.int32)
+    nd, V = len(nX), u.shape[-1]
+    dX = np.array([L[i] / nX[i] for i in range(nd)])
+    ret = np.zeros(u.size)
+    o = cc.OracleOpts()
+    st = cc.OracleStats()
+    rc = cc.oracle().oracle_pde_solver(s.F, None, None, None, 1, 0, 0, cc.dp(u), tf, cc.ip(nX),
+                                       nd, cc.dp(dX), 0.9,
+                                       cc.ip(np.array([bc] * nd, dtype=np.int32)), 0, flux, N, V,
+                                       1, 1, cc.dp(ret), C.byref(o), C.byref(st))
+    assert rc == 0
+    return u, st
+
+
+def ref_viscous(name, u0, tf, L, bc, N, flux=0):
+    """The reference's pde_solver with secondOrder = true."""
+    s = cc.system(name)
+    u = np.ascontiguousarray(u0).copy()
+    nX = np.array(u.shape[:-1], dtype=np.int32)
+    nd, V = len(nX), u.shape[-1]
+    dX = np.array([L[i] / nX[i] for i in range(nd)])
+    ret = np.zeros(u.size)
+    with cc.quiet_stdout():
+        cc.ref().pde_solver(s.F, None, None, True, False, False, cc.dp(u), tf, cc.ip(nX), nd,
+                            cc.dp(dX), 0.9, cc.ip(np.array([bc] * nd, dtype=np.int32)), False,
+                            flux, N, V, 1, True, cc.dp(ret), 4)
+    return u
+
+
+@pytest.mark.ref
+def test_viscous_path_reference():
+    """Second-order (viscous) F: the gradient traces at faces (fv.cpp:87-122),
+    the viscous wave speed in dt and in the flux (system.cpp:53-61,
+    stepper.cpp:58-59, fluxes.cpp:96-100) against the reference."""
+    uo, _ = oracle_viscous("advdiff", advdiff_init(64), 0.2, [1.0], 1, 3)
+    ur = ref_viscous("advdiff", advdiff_init(64), 0.2, [1.0], 1, 3)
+    assert relerr(uo, ur) <= 1e-10
+    u0 = systems.euler2d_modes(16, seed=2)
+    uo, _ = oracle_viscous("ns2d", u0, 0.02, [1.0, 1.0], 1, 3)
+    ur = ref_viscous("ns2d", u0, 0.02, [1.0, 1.0], 1, 3)
+    assert relerr(uo, ur) <= 1e-8
+
+
+def test_viscous_golden():
+    g = load_gold("solves")
+    uo, _ = oracle_viscous("advdiff", advdiff_init(64), 0.2, [1.0], 1, 3)
+    assert relerr(uo, g["advdiff_final"]) <= 1e-10
+    uo, _ = oracle_viscous("ns2d", systems.euler2d_modes(16, seed=2), 0.02, [1.0, 1.0], 1, 3)
+    assert relerr(uo, g["ns2d_final"]) <= 1e-8
+
+
+def test_advection_diffusion_converges_to_exact():
+    """Viscous path against the exact solution of advection-diffusion, u = 1 +
+    0.5 sin(2 pi (x - t)) exp(-nu (2 pi)^2 t) (cell averages).  At order 3 the
+    reference's viscous treatment (gradient traces plus the 2 (N + 1) nu / dX
+    penalty, fluxes.cpp:96-100) converges at about second order; the reference
+    gives the same errors (L1 4.04e-3, 8.63e-4, 1.95e-4 at 16, 32, 64 cells)."""
+    nu, tf = 0.02, 0.2
+    errs = []
+    for n in (16, 32, 64):
+        e = np.arange(n + 1) / n - tf
+        k = 2 * np.pi
+        avg = lambda ee: 1 + 0.5 * (-(np.cos(k * ee[1:]) - np.cos(k * ee[:-1])) / k * n)  # noqa
+        u0 = avg(np.arange(n + 1) / n)[:, None]
+        exact = 1 + (avg(e) - 1) * np.exp(-nu * k * k * tf)
+        uo, _ = oracle_viscous("advdiff", u0, tf, [1.0], 1, 3)
+        errs.append(np.mean(np.abs(uo[:, 0] - exact)))
+    orders = [np.log2(errs[i] / errs[i + 1]) for i in range(2)]
+    print("advection-diffusion L1 errors", errs, "orders", orders)
+    assert np.allclose(errs, [4.039684724e-3, 8.632007961e-4, 1.949539833e-4], rtol=1e-6)
+    assert min(orders) >= 2.0
