@@ -66,8 +66,8 @@ void pypde_default_options(pypde_options *opts);
  * u (prod(nX) x V, row-major, variables fastest) is overwritten with the final
  * state, as in the reference (api.cpp:18, iterator.cpp:129); ret receives ndt
  * snapshot rows (iterator.cpp:136-150).  FLUX: 0 Rusanov, 1 Roe, 2 Osher
- * (fluxes.h:6-8).
- * secondOrder (viscous F(Q, dQ)) is not supported. */
+ * (fluxes.h:6-8).  secondOrder: F depends on the gradient dQ (viscous terms;
+ * face gradients and viscous wave speeds as fv.cpp:87-122, system.cpp:53-61). */
 int pde_solver_hip(const char *src, bool useF, bool useB, bool useS, double *u, double tf,
                    int *nX, int ndim, double *dX, double CFL, int *boundaryTypes, bool STIFF,
                    int FLUX, int N, int V, int ndt, bool secondOrder, double *ret,
@@ -106,7 +106,7 @@ typedef struct pypde_ensemble pypde_ensemble;
 pypde_ensemble *pypde_ensemble_create(int n, const char *src, bool useF, bool useB, bool useS,
                                       const int *nX, int ndim, const double *dX, double CFL,
                                       const int *boundaryTypes, bool STIFF, int FLUX, int N, int V,
-                                      const pypde_options *opts, int *rc);
+                                      bool secondOrder, const pypde_options *opts, int *rc);
 int pypde_ensemble_set_state(pypde_ensemble *e, int i, const double *u);
 int pypde_ensemble_get_state(pypde_ensemble *e, int i, double *u);
 /* Advance every member from t = 0 to tf (or max_steps); stats: n entries or NULL. */
@@ -159,10 +159,10 @@ int pypde_dev_memset(void *p, int value, size_t bytes);
 int pypde_dev_sync(void);
 int pypde_device_count(void);
 
-/* Compile (or fetch from the code-object cache) without touching a GPU; FLUX as
- * in pde_solver_hip. */
+/* Compile (or fetch from the code-object cache) without touching a GPU; FLUX and
+ * secondOrder as in pde_solver_hip. */
 int pypde_hip_compile(const char *src, bool useF, bool useB, bool useS, int ndim, int N, int V,
-                      bool STIFF, int FLUX, const pypde_options *opts);
+                      bool STIFF, int FLUX, bool secondOrder, const pypde_options *opts);
 /* Tables exactly as the kernels see them (for checks). */
 int pypde_hip_tables(int N, double *nodes, double *wghts, double *endvals, double *dervals,
                      double *weno_inv, double *sig, double *dgmat, double *dginv, double *fl_c,

@@ -1764,7 +1764,7 @@ static void face_flux(const Tab *T, oracle_F F, oracle_B B, int flux,
     for (int v = 0; v < V; v++) f[v] = s * (qL[v] - qR[v]) + (fl[v] + fr[v]);
   } else if (F) {
     double fl[MAXV], fr[MAXV], Dq[MAXV], q[MAXV], A[MAXV * MAXV], Mx[MAXV * MAXV], y[MAXV],
-        Dsum[MAXV], ftmp[MAXV], dq[MAXD * MAXV], qq[MAXV], dqq[MAXD * MAXV];
+        Dsum[MAXV], ftmp[MAXV], dq[MAXD * MAXV] = {0}, qq[MAXV], dqq[MAXD * MAXV] = {0};
     memcpy(qq, qL, V * sizeof(double));
     memcpy(dqq, dqL, D * V * sizeof(double));
     F(fl, qq, dqq, d);

@@ -75,12 +75,13 @@ def lib():
         L.pypde_stream.argtypes = [C.c_void_p]
         L.pypde_destroy.argtypes = [C.c_void_p]
         L.pypde_hip_compile.argtypes = [C.c_char_p, C.c_bool, C.c_bool, C.c_bool, C.c_int,
-                                        C.c_int, C.c_int, C.c_bool, C.c_int, C.POINTER(Options)]
+                                        C.c_int, C.c_int, C.c_bool, C.c_int, C.c_bool,
+                                        C.POINTER(Options)]
         L.pypde_hip_tables.argtypes = [C.c_int] + [PD] * 10
         L.pypde_ensemble_create.restype = C.c_void_p
         L.pypde_ensemble_create.argtypes = [C.c_int, C.c_char_p, C.c_bool, C.c_bool, C.c_bool, PI,
                                             C.c_int, PD, C.c_double, PI, C.c_bool, C.c_int,
-                                            C.c_int, C.c_int, C.POINTER(Options), PI]
+                                            C.c_int, C.c_int, C.c_bool, C.POINTER(Options), PI]
         L.pypde_ensemble_set_state.argtypes = [C.c_void_p, C.c_int, PD]
         L.pypde_ensemble_get_state.argtypes = [C.c_void_p, C.c_int, PD]
         L.pypde_ensemble_run.argtypes = [C.c_void_p, C.c_double, C.c_long, C.POINTER(Stats), PD]
@@ -134,10 +135,11 @@ def make_options(newton_tol=1e-12, newton_maxit=100, max_steps=0, reflect=None, 
     return o
 
 
-def compile_only(src, useF, useB, useS, ndim, N, V, stiff=False, use_M=False, b_quirk=0, flux=0):
+def compile_only(src, useF, useB, useS, ndim, N, V, stiff=False, use_M=False, b_quirk=0, flux=0,
+                 second=False):
     o = make_options(use_M=use_M, b_quirk=b_quirk)
     rc = lib().pypde_hip_compile(src.encode(), useF, useB, useS, ndim, N, V, stiff, int(flux),
-                                 C.byref(o))
+                                 bool(second), C.byref(o))
     check(rc, f"hipRTC compile (ndim={ndim}, N={N}, V={V})")
 
 

@@ -530,6 +530,35 @@ DEV double max_abs_eigs(const double* q, const double* dq, int d, double* f0) {
   return max_abs_eig(A);
 }
 
+#ifndef PDE_SECOND
+#define PDE_SECOND 0
+#endif
+#if PDE_SECOND
+// Viscous wave speed (system.cpp:53-61): 2 (N + 1) / dX_d max|lambda(dF_d /
+// d(dQ_d))|, the Jacobian by forward differences in dQ[d V + i].
+template <int d>
+DEV double max_abs_eigs_second_order(const double* q_in, const double* dq_in, double dxd) {
+  double q[V], dq[D * V], v1[V], v2[V], A[V * V];
+#pragma unroll
+  for (int i = 0; i < V; i++) q[i] = q_in[i];
+#pragma unroll
+  for (int i = 0; i < D * V; i++) dq[i] = dq_in[i];
+  F(v1, q, dq, d);
+#pragma unroll
+  for (int i = 0; i < V; i++) {
+    double qi0 = dq[d * V + i];
+    double h = dmax(FD_EPS * dabs(qi0), FD_EPS);
+    dq[d * V + i] = qi0 + h;
+    F(v2, q, dq, d);
+    dq[d * V + i] = qi0;
+    double rh = 1.0 / h;
+#pragma unroll
+    for (int r = 0; r < V; r++) A[r * V + i] = (v2[r] - v1[r]) * rh;
+  }
+  return (double)(2 * (N + 1)) / dxd * max_abs_eig(A);
+}
+#endif
+
 // B(q) applied to a gradient: matrix-vector product, or (PDE_B_QUIRK) the
 // reference's depth-1 outer-product evaluation (quirk 1: dg.cpp:110, b(0,0) dq[v];
 // quirk 2: fv.cpp:234, b(v,0) dq[0]).

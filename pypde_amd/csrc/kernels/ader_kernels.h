@@ -201,12 +201,15 @@ k_dt(const double* __restrict__ wh, StepState* st, Geom g) {
 #pragma unroll
       for (int v = 0; v < V; v++) dq[d * V + v] = acc[v];
     });
-#pragma unroll
-    for (int d = 0; d < D; d++) {
+    static_for<0, D>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
       double f0[V];
       double lam = max_abs_eigs(q, dq, d, f0);
+#if PDE_SECOND
+      lam = lam + max_abs_eigs_second_order<d>(q, dq, g.dX[d]);
+#endif
       sum = sum + lam / g.dX[d];
-    }
+    });
     if (!(sum > 0.0)) sum = 0.0;
   }
   double m = wave_max(sum);

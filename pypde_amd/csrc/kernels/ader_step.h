@@ -101,6 +101,56 @@ struct PredLDS {
   double red[64];
 };
 
+#if PDE_SECOND
+// Gradient traces for the viscous flux (fv.cpp:87-122): the derivatives of
+// the converged q(t) along every d_ at this lane's node (staged in sm.f), then
+// extrapolated along each face direction d to both cell ends; planes
+// [d][side][t][transverse node][d_][v] of trd.  Call with every lane of the
+// wavefront (it synchronises); `on` marks lanes whose cell is done, and
+// sm.q must hold the converged q.
+DEV void grad_traces(const NodeCtx& c, PredLDS& sm, const Geom& g, long cell, bool on,
+                     double* __restrict__ trd) {
+  if (on) {
+    static_for<0, D>([&](auto dc_) {
+      constexpr int d_ = decltype(dc_)::value;
+#pragma unroll
+      for (int t = 0; t < N; t++) {
+        double tmp[V];
+        lds_deriv<d_>(c, sm.q, t, 1.0 / g.dX[d_], tmp);
+#pragma unroll
+        for (int v = 0; v < V; v++) sm.f[d_][c.slot][t][c.idx][v] = tmp[v];
+      }
+    });
+  }
+  __syncthreads();
+  if (on) {
+    static_for<0, D>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      constexpr int stride = ipow_c(N, D - 1 - d);
+      if (c.kd[d] == 0) {
+        const int j = (c.idx / (stride * N)) * stride + c.idx % stride;
+#pragma unroll
+        for (int e = 0; e < 2; e++)
+#pragma unroll
+          for (int t = 0; t < N; t++)
+#pragma unroll
+            for (int d_ = 0; d_ < D; d_++)
+#pragma unroll
+              for (int v = 0; v < V; v++) {
+                double acc = (e ? T_END1[0] : T_END0[0]) * sm.f[d_][c.slot][t][c.idx][v];
+#pragma unroll
+                for (int m = 1; m < N; m++)
+                  acc = dfma(e ? T_END1[m] : T_END0[m], sm.f[d_][c.slot][t][c.idx + m * stride][v], acc);
+                const int plane = ((((d * 2 + e) * N + t) * ND1 + j) * D + d_) * V + v;
+                trd[(long)plane * g.nwc + cell] = acc;
+              }
+      }
+    });
+  }
+  __syncthreads();
+}
+#endif
+
 // rhs of the space-time system at this lane's node for all t (dg.cpp:50-126):
 // r = W_t prod W (S(q) - sum_d (B_d dq_d + d_d F_d)),  rhs = r dt + Ww.
 // Assumes sm.q holds the current iterate; leaves the lane's dq in dqs.
@@ -352,7 +402,8 @@ DEV void newton_node(const NodeCtx& c, const double (&q)[N][V], double (&G)[N][V
 extern "C" __global__ void __launch_bounds__(64)
 k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
             double* __restrict__ tr, double* __restrict__ vol, double* __restrict__ qh,
-            StepState* st, Geom g, double newton_tol, int newton_maxit) {
+            StepState* st, Geom g, double newton_tol, int newton_maxit,
+            double* __restrict__ trd) {
   if (st->done) return;
   __shared__ PredLDS sm;
   const double dt = st->dt;
@@ -537,6 +588,9 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
           for (int v = 0; v < V; v++) qh[(long)((t * ND + c.idx) * V + v) * g.nwc + cell] = q[t][v];
       }
     }
+#if PDE_SECOND
+    grad_traces(c, sm, g, cell, fin, trd);
+#endif
 #if PDE_USE_S || PDE_USE_B
     // volume term (fv.cpp:33-85): per node sum over t, then over nodes in order
     double part[V];
@@ -618,7 +672,8 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
 DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
                                const double* __restrict__ guess, double* __restrict__ tr,
                                double* __restrict__ vol, double* __restrict__ qh, StepState* st,
-                               const Geom& g, double newton_tol, int newton_maxit) {
+                               const Geom& g, double newton_tol, int newton_maxit,
+                               double* __restrict__ trd) {
   if (st->done) return;
   const double dt = st->dt;
   const int lane = threadIdx.x;
@@ -776,6 +831,9 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
         for (int v = 0; v < V; v++) qh[(long)((t * ND + c.idx) * V + v) * g.nwc + cell] = q[t][v];
     }
   }
+#if PDE_SECOND
+  grad_traces(c, sm, g, cell, valid, trd);
+#endif
 #if PDE_USE_S || PDE_USE_B
   // volume term (fv.cpp:33-85): per node sum over t, then over nodes in order
   double part[V];
@@ -841,9 +899,10 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
 extern "C" __global__ void __launch_bounds__(64)
 k_predictor_static(const double* __restrict__ wh, const double* __restrict__ guess,
                    double* __restrict__ tr, double* __restrict__ vol, double* __restrict__ qh,
-                   StepState* st, Geom g, double newton_tol, int newton_maxit) {
+                   StepState* st, Geom g, double newton_tol, int newton_maxit,
+                   double* __restrict__ trd) {
   __shared__ PredLDS sm;
-  predictor_static_body(sm, wh, guess, tr, vol, qh, st, g, newton_tol, newton_maxit);
+  predictor_static_body(sm, wh, guess, tr, vol, qh, st, g, newton_tol, newton_maxit, trd);
 }
 
 // The same with the register budget of two wavefronts per SIMD (spilling the
@@ -851,9 +910,10 @@ k_predictor_static(const double* __restrict__ wh, const double* __restrict__ gue
 extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_predictor_static2(const double* __restrict__ wh, const double* __restrict__ guess,
                     double* __restrict__ tr, double* __restrict__ vol, double* __restrict__ qh,
-                    StepState* st, Geom g, double newton_tol, int newton_maxit) {
+                    StepState* st, Geom g, double newton_tol, int newton_maxit,
+                   double* __restrict__ trd) {
   __shared__ PredLDS sm;
-  predictor_static_body(sm, wh, guess, tr, vol, qh, st, g, newton_tol, newton_maxit);
+  predictor_static_body(sm, wh, guess, tr, vol, qh, st, g, newton_tol, newton_maxit, trd);
 }
 
 // ------------------------------------------------------------ face flux
@@ -877,7 +937,7 @@ DEV void face_cells(const Geom& g, long f, long* wl, long* wr) {
 
 template <int d>
 DEV void flux_dir(const double* __restrict__ tr, double* __restrict__ ff, double* __restrict__ fb,
-                  const StepState* st, const Geom& g) {
+                  const StepState* st, const Geom& g, const double* __restrict__ trd) {
   const long nf = g.nf[d];
   const long item = (long)blockIdx.x * 256 + threadIdx.x;
   if (item >= nf * N * ND1) return;
@@ -895,23 +955,33 @@ DEV void flux_dir(const double* __restrict__ tr, double* __restrict__ ff, double
   const long o = (long)((t * ND1 + j) * V) * nf + f;
 #if PDE_USE_F
   {
-    double dq0[D * V];
+    // face gradients (viscous F only; zero otherwise, as the reference passes
+    // unused dQ)
+    double dqL[D * V], dqR[D * V];
 #pragma unroll
-    for (int i = 0; i < D * V; i++) dq0[i] = 0.0;
-    double fl[V], fr[V];
+    for (int i = 0; i < D * V; i++) {
+#if PDE_SECOND
+      dqL[i] = trd[(long)((((d * 2 + 1) * N + t) * ND1 + j) * D * V + i) * g.nwc + wl];
+      dqR[i] = trd[(long)((((d * 2 + 0) * N + t) * ND1 + j) * D * V + i) * g.nwc + wr];
+#else
+      dqL[i] = 0.0;
+      dqR[i] = 0.0;
+#endif
+    }
+    double fl[V], fr[V], fv[V];
 #if PDE_FLUX == 0
     // Rusanov: max(lam(qL), lam(qR)) (qL - qR) + F(qL) + F(qR)  (fluxes.cpp:72-101)
-    double l1 = max_abs_eigs(qL, dq0, d, fl);
-    double l2 = max_abs_eigs(qR, dq0, d, fr);
+    double l1 = max_abs_eigs(qL, dqL, d, fl);
+    double l2 = max_abs_eigs(qR, dqR, d, fr);
     double s = dmax(l1, l2);
 #pragma unroll
-    for (int v = 0; v < V; v++) ff[o + (long)v * nf] = s * (qL[v] - qR[v]) + (fl[v] + fr[v]);
+    for (int v = 0; v < V; v++) fv[v] = s * (qL[v] - qR[v]) + (fl[v] + fr[v]);
 #else
     // Roe: |sum_i W_i A(q_i)| (qL - qR); Osher: sum_i W_i |A(q_i)| (qL - qR), on
     // the path q_i = qR + x_i (qL - qR)  (fluxes.cpp:20-70)
-    F(fl, qL, dq0, d);
-    F(fr, qR, dq0, d);
-    double Dq[V], q[V], A[V * V], ftmp[V], Dsum[V];
+    F(fl, qL, dqL, d);
+    F(fr, qR, dqR, d);
+    double Dq[V], q[V], A[V * V], ftmp[V], Dsum[V], dqp[D * V];
 #if PDE_FLUX == 1
     double Mx[V * V];
 #endif
@@ -921,7 +991,9 @@ DEV void flux_dir(const double* __restrict__ tr, double* __restrict__ ff, double
     for (int i = 0; i < N; i++) {
 #pragma unroll
       for (int v = 0; v < V; v++) q[v] = dfma(T_X[i], Dq[v], qR[v]);
-      system_matrix(q, dq0, d, A, ftmp);
+#pragma unroll
+      for (int k = 0; k < D * V; k++) dqp[k] = dfma(T_X[i], dqL[k] - dqR[k], dqR[k]);
+      system_matrix(q, dqp, d, A, ftmp);
 #if PDE_FLUX == 1
 #pragma unroll
       for (int k = 0; k < V * V; k++) Mx[k] = (i == 0) ? T_W[0] * A[k] : dfma(T_W[i], A[k], Mx[k]);
@@ -936,8 +1008,18 @@ DEV void flux_dir(const double* __restrict__ tr, double* __restrict__ ff, double
     abs_apply(Mx, Dq, Dsum);
 #endif
 #pragma unroll
-    for (int v = 0; v < V; v++) ff[o + (long)v * nf] = Dsum[v] + (fl[v] + fr[v]);
+    for (int v = 0; v < V; v++) fv[v] = Dsum[v] + (fl[v] + fr[v]);
 #endif
+#if PDE_SECOND
+    // viscous dissipation (fluxes.cpp:96-100)
+    const double m1 = max_abs_eigs_second_order<d>(qL, dqL, g.dX[d]);
+    const double m2 = max_abs_eigs_second_order<d>(qR, dqR, g.dX[d]);
+    const double s2 = dmax(m1, m2);
+#pragma unroll
+    for (int v = 0; v < V; v++) fv[v] = fv[v] + s2 * (qL[v] - qR[v]);
+#endif
+#pragma unroll
+    for (int v = 0; v < V; v++) ff[o + (long)v * nf] = fv[v];
   }
 #endif
 #if PDE_USE_B
@@ -966,16 +1048,16 @@ DEV void flux_dir(const double* __restrict__ tr, double* __restrict__ ff, double
 
 extern "C" __global__ void __launch_bounds__(256)
 k_flux0(const double* __restrict__ tr, double* __restrict__ ff, double* __restrict__ fb,
-        const StepState* st, Geom g) {
+        const StepState* st, Geom g, const double* __restrict__ trd) {
   if (st->done) return;
-  flux_dir<0>(tr, ff, fb, st, g);
+  flux_dir<0>(tr, ff, fb, st, g, trd);
 }
 #if PDE_D >= 2
 extern "C" __global__ void __launch_bounds__(256)
 k_flux1(const double* __restrict__ tr, double* __restrict__ ff, double* __restrict__ fb,
-        const StepState* st, Geom g) {
+        const StepState* st, Geom g, const double* __restrict__ trd) {
   if (st->done) return;
-  flux_dir<1>(tr, ff, fb, st, g);
+  flux_dir<1>(tr, ff, fb, st, g, trd);
 }
 #endif
 

@@ -109,6 +109,7 @@ struct Spec {
   bool useF, useB, useS, useM, stiff, guess;
   int D, N, V, b_quirk;
   int flux = 0;  // 0 Rusanov, 1 Roe, 2 Osher (fluxes.h:6-8)
+  bool second = false;  // F depends on the gradient (viscous, secondOrder)
 };
 
 std::string full_source(const Spec& s, const pypde::Tables& T) {
@@ -120,6 +121,7 @@ std::string full_source(const Spec& s, const pypde::Tables& T) {
     << "\n";
   o << "#define PDE_STIFF " << (int)s.stiff << "\n#define PDE_B_QUIRK " << s.b_quirk << "\n";
   o << "#define PDE_GUESS " << (int)s.guess << "\n#define PDE_FLUX " << s.flux << "\n";
+  o << "#define PDE_SECOND " << (int)s.second << "\n";
   o << "constexpr int ipow_c(int b, int e) { return e <= 0 ? 1 : b * ipow_c(b, e - 1); }\n";
   o << pypde::tables_source(T, s.D);
   o << PRELUDE_SRC << "\n";
@@ -222,7 +224,8 @@ int check_spec(const Spec& s, bool secondOrder, int FLUX) {
   for (int d = 0; d < s.D; d++) nd *= s.N;
   if (nd > 64) return fail(PYPDE_ERR_UNSUPPORTED, "N^ndim=%d > 64 nodes per cell", nd);
   if (s.V < 1 || s.V > MAXV) return fail(PYPDE_ERR_ARG, "V=%d outside 1..%d", s.V, MAXV);
-  if (secondOrder) return fail(PYPDE_ERR_UNSUPPORTED, "second-order (viscous) F not supported");
+  if (secondOrder && !s.useF)
+    return fail(PYPDE_ERR_ARG, "secondOrder needs F (the viscous flux depends on the gradient)");
   if (FLUX < 0 || FLUX > 2)
     return fail(PYPDE_ERR_ARG, "FLUX=%d: 0 Rusanov, 1 Roe or 2 Osher (fluxes.h:6-8)", FLUX);
   if (!s.src.size() && (s.useF || s.useB || s.useS))
@@ -255,6 +258,7 @@ struct pypde_ctx {
   hipStream_t stream = nullptr;
   double *u = nullptr, *wh = nullptr, *tr = nullptr, *vol = nullptr, *ff[2] = {}, *fb[2] = {},
          *ret = nullptr, *guess = nullptr, *tmp = nullptr;
+  double* trd = nullptr;    // face gradient traces (viscous F only)
   StepState* st = nullptr;
   StepState host_st;
   size_t ret_rows = 0;
@@ -374,7 +378,8 @@ int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
     StepState* st = c->st;
     double tol = c->opts.newton_tol;
     int maxit = c->opts.newton_maxit;
-    std::vector<void*> a = {&w, &gs, &t, &v, &qh, &st, &g, &tol, &maxit};
+    double* td = c->trd;
+    std::vector<void*> a = {&w, &gs, &t, &v, &qh, &st, &g, &tol, &maxit, &td};
     if (c->pred_static) {
       if ((rc = launch(c, c->pred_w2 ? K_PRED_STATIC2 : K_PRED_STATIC,
                        dim3((unsigned)cdiv(g.nwc, c->CPW)), dim3(64), a)))
@@ -388,7 +393,8 @@ int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
     const double* t = c->tr;
     double *f = c->ff[d], *b = c->fb[d];
     const StepState* st = c->st;
-    std::vector<void*> a = {&t, &f, &b, &st, &g};
+    const double* td = c->trd;
+    std::vector<void*> a = {&t, &f, &b, &st, &g, &td};
     long items = g.nf[d] * c->N * (c->ND / c->N);
     if ((rc = launch(c, d == 0 ? K_FLUX0 : K_FLUX1, dim3((unsigned)cdiv(items, 256)), dim3(256), a)))
       return rc;
@@ -472,11 +478,12 @@ int pypde_hip_tables(int N, double* nodes, double* wghts, double* endvals, doubl
 }
 
 int pypde_hip_compile(const char* src, bool useF, bool useB, bool useS, int ndim, int N, int V,
-                      bool STIFF, int FLUX, const pypde_options* opts) {
+                      bool STIFF, int FLUX, bool secondOrder, const pypde_options* opts) {
   Spec s;
   make_spec(&s, src, useF, useB, useS, ndim, N, V, STIFF, opts);
   s.flux = FLUX;
-  int rc = check_spec(s, false, FLUX);
+  s.second = secondOrder;
+  int rc = check_spec(s, secondOrder, FLUX);
   if (rc) return rc;
   std::vector<char> code;
   if ((rc = compile_code(s, &code))) return rc;
@@ -492,7 +499,7 @@ void pypde_destroy(pypde_ctx* c) {
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   for (double* p : {c->u, c->wh, c->tr, c->vol, c->ff[0], c->ff[1], c->fb[0], c->fb[1], c->ret,
-                    c->guess, c->tmp})
+                    c->guess, c->tmp, c->trd})
     if (p) hipFree(p);
   if (c->st) hipFree(c->st);
   if (c->gbuf) hipFree(c->gbuf);
@@ -523,6 +530,7 @@ pypde_ctx* pypde_create(const char* src, bool useF, bool useB, bool useS, int* n
   make_spec(&s, src, useF, useB, useS, ndim, N, V, STIFF, &opts);
   s.guess = opts.guess != nullptr;
   s.flux = FLUX;
+  s.second = secondOrder;
   if ((rc = check_spec(s, secondOrder, FLUX))) return nullptr;
   if (!nX || !dX || !boundaryTypes) {
     rc = fail(PYPDE_ERR_ARG, "null grid argument");
@@ -608,6 +616,7 @@ pypde_ctx* pypde_create(const char* src, bool useF, bool useB, bool useS, int* n
   };
   bool ok = alloc(&c->u, g.ncell * V) && alloc(&c->wh, g.nwc * c->ND * V) &&
             alloc(&c->tr, g.nwc * NTR) && alloc(&c->tmp, g.ncell * V) &&
+            (!secondOrder || alloc(&c->trd, g.nwc * NTR * ndim)) &&
             ((!useS && !useB) || alloc(&c->vol, g.nwc * V));
   for (int d = 0; d < ndim && ok; d++) {
     size_t fs = (size_t)g.nf[d] * N * (c->ND / N) * V;
@@ -817,7 +826,7 @@ void pypde_ensemble_destroy(pypde_ensemble* e) {
 pypde_ensemble* pypde_ensemble_create(int n, const char* src, bool useF, bool useB, bool useS,
                                       const int* nX, int ndim, const double* dX, double CFL,
                                       const int* boundaryTypes, bool STIFF, int FLUX, int N, int V,
-                                      const pypde_options* opts, int* rc_out) {
+                                      bool secondOrder, const pypde_options* opts, int* rc_out) {
   int dummy;
   int& rc = rc_out ? *rc_out : dummy;
   if (n < 1 || !nX || !dX || !boundaryTypes || ndim < 1 || ndim > 2) {
@@ -834,7 +843,7 @@ pypde_ensemble* pypde_ensemble_create(int n, const char* src, bool useF, bool us
     double dx[2] = {dX[i * ndim], ndim > 1 ? dX[i * ndim + 1] : 1.0};
     int bc[2] = {boundaryTypes[0], ndim > 1 ? boundaryTypes[1] : 0};
     pypde_ctx* c = pypde_create(src, useF, useB, useS, nx, ndim, dx, CFL, bc, STIFF, FLUX, N, V,
-                                false, opts, &rc);
+                                secondOrder, opts, &rc);
     if (!c) {
       pypde_ensemble_destroy(e);
       return nullptr;

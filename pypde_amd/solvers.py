@@ -48,6 +48,13 @@ def _source(F, B, S, M, system):
     return "\n".join(parts), F is not None, B is not None, S is not None, M is not None
 
 
+def _second_order(src, useF):
+    """The reference picks its viscous path when F takes the gradient
+    (three-argument F, pypde/cfuncs.py:11-27, solvers.py:196); a system
+    source says so with ``#define PDE_SECOND_ORDER 1``."""
+    return bool(useF and "#define PDE_SECOND_ORDER 1" in src)
+
+
 def pde_solver(Q0, tf, L, F=None, B=None, S=None, boundaryTypes="transitive", cfl=0.9,
                order=2, ndt=100, flux="rusanov", stiff=True, nThreads=-1, M=None,
                system=None, reflect=None, newton_tol=1e-12, newton_maxit=100, max_steps=0,
@@ -69,9 +76,7 @@ def pde_solver(Q0, tf, L, F=None, B=None, S=None, boundaryTypes="transitive", cf
     if flux not in FLUXES:
         raise ValueError(f"flux must be one of {sorted(FLUXES)}")
     src, useF, useB, useS, useM = _source(F, B, S, M, system)
-    secondOrder = False
-    if useF and "dQ[" in src and "PDE_SECOND_ORDER" in src:
-        secondOrder = True
+    secondOrder = _second_order(src, useF)
     inplace = Q0.dtype == np.float64 and Q0.flags.c_contiguous
     u = Q0.ravel() if inplace else np.ascontiguousarray(Q0, dtype=np.float64).ravel().copy()
     ret = np.zeros(ndt * u.size)
@@ -131,7 +136,8 @@ class Solver:
         rc = C.c_int(0)
         self.h = _capi.lib().pypde_create(src.encode(), useF, useB, useS, _capi.ip(self.nX), nd,
                                           _capi.dp(self.dX), float(cfl), _capi.ip(self.bcs),
-                                          bool(stiff), FLUXES[flux], int(order), V, False,
+                                          bool(stiff), FLUXES[flux], int(order), V,
+                                          _second_order(src, useF),
                                           C.byref(self.opts), C.byref(rc))
         if not self.h:
             _capi.check(rc.value or 3, "pypde_create")
@@ -206,7 +212,7 @@ class Ensemble:
         self.h = _capi.lib().pypde_ensemble_create(len(grids), src.encode(), useF, useB, useS,
                                                    _capi.ip(nX), nd, _capi.dp(dX), float(cfl),
                                                    _capi.ip(bcs), bool(stiff), FLUXES[flux],
-                                                   int(order), V,
+                                                   int(order), V, _second_order(src, useF),
                                                    C.byref(self.opts), C.byref(rc))
         if not self.h:
             _capi.check(rc.value or 3, "pypde_ensemble_create")

@@ -50,6 +50,11 @@ def standard_configs(full=False):
         for flux in (1, 2):
             out.append(dict(src=source(name), useF=s["F"], useB=s["B"], useS=s["S"], ndim=ndim,
                             N=N, V=s["V"], stiff=False, use_M=s["M"], flux=flux))
+    # viscous (second-order) systems
+    for name, ndim, N in ([("advdiff", 1, 3), ("ns2d", 2, 3)] if full else []):
+        s = SYSTEMS[name]
+        out.append(dict(src=source(name), useF=s["F"], useB=s["B"], useS=s["S"], ndim=ndim, N=N,
+                        V=s["V"], stiff=False, use_M=s["M"], flux=0, second=True))
     return out
 
 

@@ -173,7 +173,7 @@ def ref_tables(N):
 
 def oracle_solve(sysname, u0, tf, L, bc, N, stiff=False, cfl=0.9, ndt=1, max_steps=0,
                  reflect=None, newton_tol=1e-12, newton_maxit=100, useF=True, useB=None,
-                 useS=None, b_quirk=0, flux=0):
+                 useS=None, b_quirk=0, flux=0, second=False):
     """Run oracle_pde_solver on a copy of u0 (shape nX + (V,)); returns (final, ret, stats)."""
     s = system(sysname)
     u = np.ascontiguousarray(u0, dtype=np.float64).copy()
@@ -196,7 +196,8 @@ def oracle_solve(sysname, u0, tf, L, bc, N, stiff=False, cfl=0.9, ndt=1, max_ste
     useS = (s.S is not None) if useS is None else useS
     rc = oracle().oracle_pde_solver(s.F, s.B, s.S, s.M, int(useF and s.F is not None),
                                     int(useB), int(useS), dp(u), tf, ip(nX), ndim, dp(dX),
-                                    cfl, ip(bcs), int(stiff), int(flux), N, V, ndt, 0, dp(ret),
+                                    cfl, ip(bcs), int(stiff), int(flux), N, V, ndt, int(second),
+                                    dp(ret),
                                     C.byref(opts), C.byref(st))
     return u, ret.reshape((ndt,) + u.shape), st, rc
 

@@ -53,10 +53,10 @@ def test_argument_errors_are_reported():
                    boundaryTypes="bogus")
     o = _capi.make_options()
     rc = _capi.lib().pypde_hip_compile(systems.source("euler2d").encode(), True, False, False,
-                                       3, 3, 4, False, 0, C.byref(o))
+                                       3, 3, 4, False, 0, False, C.byref(o))
     assert rc == 4 and "ndim=3" in _capi.last_error()
     rc = _capi.lib().pypde_hip_compile(b"this is not C", True, False, False, 1, 3, 3, False,
-                                       0, C.byref(o))
+                                       0, False, C.byref(o))
     assert rc == 2 and "error" in _capi.last_error()
 
 
@@ -345,3 +345,25 @@ def test_roe_osher_match_oracle(flux):
     print(f"{flux} euler2d: steps {sg['steps']} err {relerr(ug, uo):.3e}")
     assert rc == 0 and sg["steps"] == st.steps
     assert relerr(ug, uo) <= TOL
+
+
+@pytest.mark.gpu
+def test_viscous_path_matches_oracle():
+    """Second-order (viscous) F: face gradients from the predictor, viscous
+    wave speeds in dt and in the flux (fv.cpp:87-122, system.cpp:53-61,
+    fluxes.cpp:96-100); advection-diffusion in 1D and Navier-Stokes in 2D,
+    GPU against the oracle (Rusanov; and Osher for Navier-Stokes)."""
+    n = 64
+    x = (np.arange(n) + 0.5) / n
+    u0 = (1 + 0.5 * np.sin(2 * np.pi * x))[:, None]
+    uo, _, st, rc = oracle_run("advdiff", u0, 0.2, [1.0], 1, 3, False, second=True)
+    ug, _, sg = gpu_run("advdiff", u0, 0.2, [1.0], "periodic", 3, False)
+    print(f"advdiff: steps {sg['steps']} err {relerr(ug, uo):.3e}")
+    assert rc == 0 and sg["steps"] == st.steps and relerr(ug, uo) <= TOL
+    u0 = systems.euler2d_modes(16, seed=2)
+    for flux, code in (("rusanov", 0), ("osher", 2)):
+        uo, _, st, rc = oracle_run("ns2d", u0, 0.02, [1.0, 1.0], 1, 3, False, second=True,
+                                   flux=code)
+        ug, _, sg = gpu_run("ns2d", u0, 0.02, [1.0, 1.0], "periodic", 3, False, flux=flux)
+        print(f"ns2d {flux}: steps {sg['steps']} err {relerr(ug, uo):.3e}")
+        assert rc == 0 and sg["steps"] == st.steps and relerr(ug, uo) <= TOL
