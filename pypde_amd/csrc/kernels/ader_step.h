@@ -659,11 +659,12 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
   s_fb = wave_sum_u64(s_fb);
   s_max = wave_max_u32(s_max);
   if (lane == 0 && s_cells) {
-    atomicAdd(&st->pred_iters, s_iters);
-    atomicAdd(&st->pred_cells, s_cells);
-    if (s_cap) atomicAdd(&st->pred_capped, s_cap);
-    if (s_fb) atomicAdd(&st->pred_fallbacks, s_fb);
-    atomicMax(&st->pred_maxit, s_max);
+    unsigned long long* ps = st->pstat[blockIdx.x & 63];
+    atomicAdd(&ps[0], s_iters);
+    atomicAdd(&ps[1], s_cells);
+    if (s_cap) atomicAdd(&ps[2], s_cap);
+    if (s_fb) atomicAdd(&ps[3], s_fb);
+    atomicMax(&ps[4], (unsigned long long)s_max);
   }
 }
 
@@ -888,11 +889,12 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
   fb = wave_sum_u64(fb);
   mx = wave_max_u32(mx);
   if (lane == 0) {
-    atomicAdd(&st->pred_iters, mine);
-    atomicAdd(&st->pred_cells, cnt);
-    if (cap) atomicAdd(&st->pred_capped, cap);
-    if (fb) atomicAdd(&st->pred_fallbacks, fb);
-    atomicMax(&st->pred_maxit, mx);
+    unsigned long long* ps = st->pstat[blockIdx.x & 63];
+    atomicAdd(&ps[0], mine);
+    atomicAdd(&ps[1], cnt);
+    if (cap) atomicAdd(&ps[2], cap);
+    if (fb) atomicAdd(&ps[3], fb);
+    atomicMax(&ps[4], (unsigned long long)mx);
   }
 }
 
@@ -1156,6 +1158,19 @@ k_update(double* __restrict__ u, const double* __restrict__ vol, FaceBufs fbufs,
 
 // t += dt; count++; snapshot row; NaN and end-of-run checks (iterator.cpp:129-147).
 extern "C" __global__ void k_advance(StepState* st, int ndt) {
+  {
+    // fold the predictor's spread statistics into the totals (64 threads)
+    unsigned long long* ps = st->pstat[threadIdx.x & 63];
+    if (ps[1] != 0ull) {
+      atomicAdd(&st->pred_iters, ps[0]);
+      atomicAdd(&st->pred_cells, ps[1]);
+      atomicAdd(&st->pred_capped, ps[2]);
+      atomicAdd(&st->pred_fallbacks, ps[3]);
+      atomicMax(&st->pred_maxit, (unsigned int)ps[4]);
+#pragma unroll
+      for (int k = 0; k < 5; k++) ps[k] = 0ull;
+    }
+  }
   if (threadIdx.x != 0 || st->done) return;
   st->t = st->t + st->dt;
   st->count += 1;

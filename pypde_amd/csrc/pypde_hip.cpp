@@ -76,6 +76,11 @@ struct StepState {
   unsigned long long pred_iters, pred_cells, pred_capped, pred_fallbacks;
   unsigned int pred_maxit, pad3;
   unsigned long long qhead;
+  // predictor statistics spread over 64 slots, one 128-byte line each, so the
+  // wavefronts' atomics do not queue on one address; k_advance folds them into
+  // the totals above and clears them ([0] iterations, [1] cells, [2] capped,
+  // [3] fallbacks, [4] max iterations)
+  unsigned long long pstat[64][16];
 };
 
 struct FaceBufs {
