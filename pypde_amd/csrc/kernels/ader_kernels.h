@@ -28,7 +28,7 @@ struct StepState {
   double t, dt, tf, cfl;
   long long count, max_steps;
   int push, done, nan_step, pad;
-  unsigned long long maxbits;   // max wave-speed sum as IEEE bits (values >= 0)
+  unsigned long long maxbits;   // unused (the wave-speed maximum lives in pstat[.][8])
   unsigned int nan_flag, pad2;
   unsigned long long pred_iters, pred_cells, pred_capped, pred_fallbacks;
   unsigned int pred_maxit, pad3;
@@ -36,7 +36,8 @@ struct StepState {
   // predictor statistics spread over 64 slots, one 128-byte line each, so the
   // wavefronts' atomics do not queue on one address; k_advance folds them into
   // the totals above and clears them ([0] iterations, [1] cells, [2] capped,
-  // [3] fallbacks, [4] max iterations)
+  // [3] fallbacks, [4] max iterations; [8] k_dt's partial maximum, folded by
+  // k_dt_finalize)
   unsigned long long pstat[64][16];
 };
 
@@ -219,17 +220,25 @@ k_dt(const double* __restrict__ wh, StepState* st, Geom g) {
   }
   double m = wave_max(sum);
   if ((threadIdx.x & 63) == 0 && m > 0.0)
-    atomicMax(&st->maxbits, (unsigned long long)__double_as_longlong(m));
+    atomicMax(&st->pstat[blockIdx.x & 63][8], (unsigned long long)__double_as_longlong(m));
 }
 
 // dt = CFL / MAX, x0.2 while count <= 5, clamped to tf - t (stepper.cpp:67-75).
 extern "C" __global__ void k_dt_finalize(StepState* st) {
+  // max over the 64 slots k_dt spreads its atomics over (IEEE bits of values
+  // >= 0 order as integers); each slot is cleared for the next step
+  unsigned long long b = st->pstat[threadIdx.x & 63][8];
+  st->pstat[threadIdx.x & 63][8] = 0ull;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    unsigned long long y = __shfl_xor(b, o, 64);
+    b = b > y ? b : y;
+  }
   if (threadIdx.x != 0 || st->done) return;
-  double MAX = __longlong_as_double((long long)st->maxbits);
+  double MAX = __longlong_as_double((long long)b);
   double dt = st->cfl / MAX;
   if (st->count <= 5) dt = dt * 0.2;
   if (st->t + dt > st->tf) dt = st->tf - st->t;
   st->dt = dt;
-  st->maxbits = 0ull;
   st->qhead = 0ull;
 }

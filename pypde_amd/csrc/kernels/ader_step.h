@@ -1159,16 +1159,25 @@ k_update(double* __restrict__ u, const double* __restrict__ vol, FaceBufs fbufs,
 // t += dt; count++; snapshot row; NaN and end-of-run checks (iterator.cpp:129-147).
 extern "C" __global__ void k_advance(StepState* st, int ndt) {
   {
-    // fold the predictor's spread statistics into the totals (64 threads)
+    // fold the predictor's spread statistics into the totals (one wavefront:
+    // lane i takes slot i, then sums and a max across the wavefront)
     unsigned long long* ps = st->pstat[threadIdx.x & 63];
-    if (ps[1] != 0ull) {
-      atomicAdd(&st->pred_iters, ps[0]);
-      atomicAdd(&st->pred_cells, ps[1]);
-      atomicAdd(&st->pred_capped, ps[2]);
-      atomicAdd(&st->pred_fallbacks, ps[3]);
-      atomicMax(&st->pred_maxit, (unsigned int)ps[4]);
+    unsigned long long a0 = wave_sum_u64(ps[0]), a1 = wave_sum_u64(ps[1]);
+    unsigned long long a2 = wave_sum_u64(ps[2]), a3 = wave_sum_u64(ps[3]);
+    unsigned long long m4 = ps[4];
 #pragma unroll
-      for (int k = 0; k < 5; k++) ps[k] = 0ull;
+    for (int o = 32; o >= 1; o >>= 1) {
+      unsigned long long y = __shfl_xor(m4, o, 64);
+      m4 = m4 > y ? m4 : y;
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++) ps[k] = 0ull;
+    if (threadIdx.x == 0) {
+      st->pred_iters += a0;
+      st->pred_cells += a1;
+      st->pred_capped += a2;
+      st->pred_fallbacks += a3;
+      if ((unsigned int)m4 > st->pred_maxit) st->pred_maxit = (unsigned int)m4;
     }
   }
   if (threadIdx.x != 0 || st->done) return;

@@ -79,7 +79,8 @@ struct StepState {
   // predictor statistics spread over 64 slots, one 128-byte line each, so the
   // wavefronts' atomics do not queue on one address; k_advance folds them into
   // the totals above and clears them ([0] iterations, [1] cells, [2] capped,
-  // [3] fallbacks, [4] max iterations)
+  // [3] fallbacks, [4] max iterations; [8] k_dt's partial maximum, folded by
+  // k_dt_finalize)
   unsigned long long pstat[64][16];
 };
 
