@@ -125,6 +125,38 @@ def gpu_run(steps, warmup, profile, net=None):
     return st, t1 - t0, w, state
 
 
+def ensemble_leg(steps):
+    """C4: 48 independent 2D Euler grids, sides drawn from {64,128,256,512},
+    seeded four-quadrant Riemann data, order 3, transmissive walls, each grid
+    with its own dt; every timestep of all grids is one HIP graph launch.
+    Timed against the same grids run one after another."""
+    from pypde_amd.solvers import Ensemble
+    shapes = systems.ensemble_sizes(48)
+    grids = [systems.riemann4(nx, ny, seed=i) for i, (nx, ny) in enumerate(shapes)]
+    Ls = [[1.0, 1.0]] * len(grids)
+    src = systems.source("euler2d")
+    ens = Ensemble(grids, Ls, src, order=3)
+    ens.run(1.0, max_steps=2)                      # warm-up (code objects, graph)
+    ens.close()
+    ens = Ensemble(grids, Ls, src, order=3)
+    t0 = time.perf_counter()
+    stats, _ = ens.run(1.0, max_steps=steps)
+    t1 = time.perf_counter()
+    ens.close()
+    cells = sum(nx * ny for nx, ny in shapes)
+    done = sum(st["steps"] * nx * ny for st, (nx, ny) in zip(stats, shapes))
+    seq = 0.0
+    for g, L_ in zip(grids, Ls):
+        sol = Solver(g, L_, src, order=3)
+        a = time.perf_counter()
+        sol.run(1.0, max_steps=steps)
+        seq += time.perf_counter() - a
+        sol.close()
+    return dict(grids=len(grids), cells=cells, steps=steps, value=done / (t1 - t0),
+                unit="cell-updates/s", ms_per_step=(t1 - t0) * 1e3 / steps,
+                sequential_value=done / seq, graph_speedup=seq / (t1 - t0))
+
+
 def rel_linf(a, b):
     """max over cells and variables of |a - b| / max(|b[var]|_inf, 1e-300)."""
     scale = np.maximum(np.max(np.abs(b.reshape(-1, V)), 0), 1e-300)
@@ -187,6 +219,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--train-steps", type=int, default=12)
     ap.add_argument("--train-pairs", type=int, default=2_000_000)
@@ -220,6 +253,10 @@ def main():
                        linf_vs_plain=rel_linf(u_l, u_p), training=info,
                        roofline=predictor_roofline(st_l, K))
     head = learned if learned is not None else plain
+    ens = None
+    if not args.no_ensemble:
+        ens = ensemble_leg(20)
+        log(f"ensemble: {ens}")
     cpu = None
     if not args.no_cpu:
         # t after args.cpu_steps steps from t = 0, from a short GPU run
@@ -249,6 +286,7 @@ def main():
         "speedup_vs_plain": (learned["value"] / plain["value"]) if learned else None,
         "plain": plain,
         "learned": learned,
+        "ensemble": ens,
         "t_end": st_p["t"],
         "finite": bool(np.isfinite(u_p).all()),
     }
