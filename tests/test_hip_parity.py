@@ -277,7 +277,7 @@ def test_graph_replay_equals_eager():
 
 @pytest.mark.gpu
 def test_ensemble_members_equal_single_runs():
-    """C4 shape: independent grids of different sizes with seeded 4-quadrant
+    """Ensemble: independent grids of different sizes with seeded 4-quadrant
     Riemann data, advanced together (one graph launch per step, each grid
     with its own dt); every member equals its own single-grid run bit for
     bit, and the smallest ones equal the oracle."""
