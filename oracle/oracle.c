@@ -429,61 +429,102 @@ static void hess_charpoly(const double *H, int n, double *c) {
   for (int i = 0; i <= n; i++) c[i] = D[n][i];
 }
 
-/* One real root of the monic degree-n polynomial c by Laguerre's method from
- * x0 (cubic convergence at simple roots; from above the largest real root of
- * a real-rooted polynomial it decreases monotonically onto it).  Returns 0
+/* One real root of the monic degree-n polynomial c (n >= 2) by Laguerre's
+ * method from x0 (cubic convergence at simple roots; from above the largest
+ * real root of a real-rooted polynomial it decreases monotonically onto it).
+ * The step n / (G +- sqrt((n-1)(n H - G^2))), G = p'/p, H = G^2 - p''/p, is
+ * evaluated multiplied through by p: n p / (p' +- sqrt((n-1)((n-1) p'^2 -
+ * n p p''))), with f = p''/2 from the Horner recurrence, whose first two
+ * steps are written out (exact for c[n] = 1 and finite x).  Converged:
+ * |step| <= 4 eps |x|, or |step| <= 1e-8 |x| with |step| |x| <= |previous
+ * step|^2 (superlinear convergence seen: the remaining error is below
+ * |step|^2.5).  *dp receives p' where p was last evaluated.  Returns 0
  * when it does not converge in LAG_IT steps. */
 #define LAG_IT 40
-static int laguerre_real(const double *c, int n, double x0, double *root) {
-  double x = x0;
+static int laguerre_real(const double *c, int n, double x0, double *root, double *dp) {
+  double x = x0, prev = 0.0;
   for (int it = 0; it < LAG_IT; it++) {
-    double b = c[n], d = 0.0, f = 0.0;
-    for (int j = n - 1; j >= 0; j--) {
+    double b1 = x + c[n - 1];
+    double f = 1.0, d = x + b1, b = x * b1 + c[n - 2];
+    for (int j = n - 3; j >= 0; j--) {
       f = x * f + d;
       d = x * d + b;
       b = x * b + c[j];
     }
+    *dp = d;
     if (b == 0.0) {
       *root = x;
       return 1;
     }
-    double G = d / b;
-    double Hh = G * G - 2.0 * (f / b);
-    double r = (double)(n - 1) * ((double)n * Hh - G * G);
+    double r = (double)(n - 1) * ((double)(n - 1) * (d * d) - (double)(2 * n) * (f * b));
     double sq = r > 0.0 ? sqrt(r) : 0.0;
-    double den = (G >= 0.0) ? G + sq : G - sq;
+    double den = (d >= 0.0) ? d + sq : d - sq;
     if (!(den != 0.0)) return 0;
-    double step = (double)n / den;
+    double step = ((double)n * b) / den;
     x = x - step;
-    if (!(fabs(step) > 4.0 * DBL_EPSILON * fabs(x)) || fabs(step) <= DBL_MIN) {
+    double as = fabs(step), ax = fabs(x);
+    if (!(as > 4.0 * DBL_EPSILON * ax) || as <= DBL_MIN ||
+        (as <= 1e-8 * ax && as * ax <= prev * prev)) {
       *root = x;
-      return fabs(x) <= 1e300; /* finite */
+      return ax <= 1e300; /* finite */
     }
+    prev = as;
   }
   return 0;
 }
 
 /* max |lambda| of an upper Hessenberg H (n >= 3, entries scaled to <= 1) from
  * its characteristic polynomial: the largest and smallest real roots by
- * Laguerre from +-||H||_inf, then a certificate that every other root is
- * smaller in modulus -- the remaining factor q = p / ((x - l1)(x - l2)) is
+ * Laguerre, checked for accuracy against the rounding error of evaluating
+ * p, then a certificate that every other root is smaller in modulus -- the remaining factor q = p / ((x - l1)(x - l2)) is
  * centred on the mean of its roots, alpha, and its roots bounded by
- * |z - alpha| <= R, with R at or above the Cauchy radius of q(y + alpha)
- * (the positive root of y^m - sum |t_k| y^k).  Returns 0 (caller falls back
- * to QR) when a root does not converge or the certificate fails: complex or
- * repeated extreme eigenvalues. */
+ * |z - alpha| <= R, R = 2^(E+1) >= 2 max_k |t_k|^(1/(m-k)) (Fujiwara's bound
+ * on the roots of t(y) = q(y + alpha), in powers of two: |t_k| < 2^e_k gives
+ * |t_k|^(1/j) < 2^ceil(e_k/j)).  Returns 0 (caller falls back to QR) when a
+ * root does not converge or the certificate fails: complex or repeated
+ * extreme eigenvalues. */
 static int charpoly_max_abs(const double *H, int n, double *out) {
   double c[MAXV + 1], q[MAXV + 1], t[MAXV + 1];
   hess_charpoly(H, n, c);
-  double U = 0.0;
-  for (int i = 0; i < n; i++) {
-    double rs = 0.0;
-    for (int j = 0; j < n; j++) rs = rs + fabs(H[i * n + j]);
-    U = fmax(U, rs);
+  /* starting points mu +- 1.001 sigma sqrt(n - 1), mu and sigma^2 the mean and
+   * variance of the roots from c[n-1] and c[n-2] (Laguerre-Samuelson: every
+   * root of a real-rooted polynomial lies in mu +- sigma sqrt(n - 1)), or
+   * +-||H||_inf when sigma^2 is not positive */
+  double xhi, xlo;
+  double mu = -(c[n - 1] / (double)n);
+  double s2 = (c[n - 1] * c[n - 1] - 2.0 * c[n - 2]) / (double)n - mu * mu;
+  if (s2 > 0.0 && s2 <= 1e300) {
+    double w = 1.001 * sqrt(s2 * (double)(n - 1));
+    xhi = mu + w;
+    xlo = mu - w;
+  } else {
+    double U = 0.0;
+    for (int i = 0; i < n; i++) {
+      double rs = 0.0;
+      for (int j = 0; j < n; j++) rs = rs + fabs(H[i * n + j]);
+      U = fmax(U, rs);
+    }
+    xhi = U;
+    xlo = -U;
   }
-  double l1, l2;
-  if (!laguerre_real(c, n, U, &l1) || !laguerre_real(c, n, -U, &l2)) return 0;
-  if (!(l1 > l2)) return 0;
+  double l1, l2, d1, d2;
+  if (!laguerre_real(c, n, xhi, &l1, &d1) || !laguerre_real(c, n, xlo, &l2, &d2)) return 0;
+  double M = fmax(fabs(l1), fabs(l2));
+  /* two distinct roots: from both ends the iterations can reach the one real
+   * root, and deflating p by it twice would leave no certificate */
+  if (!(l1 - l2 > 1e-9 * M)) return 0;
+  /* accuracy: evaluating p rounds by up to 2 n eps sum_k |c_k| |x|^k, which
+   * moves a root by that over |p'|; both must stay below 1e-11 M (a simple
+   * root next to a cluster is ill-conditioned as a polynomial root) */
+  {
+    double a1 = fabs(l1), a2 = fabs(l2), E1 = 1.0, E2 = 1.0;
+    for (int j = n - 1; j >= 0; j--) {
+      E1 = a1 * E1 + fabs(c[j]);
+      E2 = a2 * E2 + fabs(c[j]);
+    }
+    if (!((double)(2 * n) * DBL_EPSILON * E1 <= 1e-11 * M * fabs(d1))) return 0;
+    if (!((double)(2 * n) * DBL_EPSILON * E2 <= 1e-11 * M * fabs(d2))) return 0;
+  }
   /* q = p / (x - l1) / (x - l2), degree m = n - 2, monic */
   for (int i = 0; i <= n; i++) q[i] = c[i];
   int deg = n;
@@ -496,39 +537,25 @@ static int charpoly_max_abs(const double *H, int n, double *out) {
     for (int j = 0; j <= deg; j++) q[j] = nq[j];
   }
   int m = deg;
-  double M = fmax(fabs(l1), fabs(l2));
   double alpha = -(q[m - 1] / (double)m);
   /* Taylor shift t(y) = q(y + alpha) */
   for (int i = 0; i <= m; i++) t[i] = q[i];
   for (int k = 0; k < m; k++)
     for (int j = m - 1; j >= k; j--) t[j] = t[j] + alpha * t[j + 1];
-  /* Cauchy radius rho of t: the positive root of f(y) = y^m - sum_k |t_k| y^k
-   * (f < 0 below rho, >= 0 above).  R: the smallest power-of-two fraction of
-   * 1 + max |t_k| (>= rho) with f(R) >= 0, then Newton steps from above, each
-   * kept only while f stays >= 0, so R >= rho >= |z - alpha| for every root z. */
-  double amax = 0.0;
-  for (int k = 0; k < m; k++) amax = fmax(amax, fabs(t[k]));
-  double R = 1.0 + amax;
-  for (int e = 0; e < 64; e++) {
-    double yh = 0.5 * R;
-    double f = 1.0;
-    for (int k = m - 1; k >= 0; k--) f = f * yh - fabs(t[k]);
-    if (!(f >= 0.0)) break;
-    R = yh;
-  }
-  for (int it = 0; it < 3; it++) {
-    double f = 1.0, fp = 0.0;
-    for (int k = m - 1; k >= 0; k--) {
-      fp = fp * R + f;
-      f = f * R - fabs(t[k]);
+  double tsum = 0.0;
+  for (int k = 0; k < m; k++) tsum = tsum + fabs(t[k]);
+  if (!(tsum <= DBL_MAX)) return 0;
+  int emax = -2000;
+  for (int k = 0; k < m; k++) {
+    if (t[k] != 0.0) {
+      int e;
+      (void)frexp(t[k], &e);                 /* |t_k| = f 2^e, f in [0.5, 1) */
+      int j = m - k;
+      int ce = e >= 0 ? (e + j - 1) / j : -((-e) / j);
+      emax = ce > emax ? ce : emax;
     }
-    if (!(fp > 0.0)) break;
-    double Rn = R - f / fp;
-    double g = 1.0;
-    for (int k = m - 1; k >= 0; k--) g = g * Rn - fabs(t[k]);
-    if (!(g >= 0.0) || !(Rn > 0.0)) break;
-    R = Rn;
   }
+  double R = emax == -2000 ? 0.0 : ldexp(1.0, emax + 1);
   if (!(fabs(alpha) + R < M)) return 0;
   *out = M;
   return 1;

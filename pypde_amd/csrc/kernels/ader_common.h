@@ -199,64 +199,137 @@ DEV void hess_charpoly(const double* H, double* c) {
   for (int i = 0; i <= V; i++) c[i] = D[V][i];
 }
 
-// One real root of the monic degree-V polynomial c by Laguerre's method from
-// x0 (cubic at simple roots; from above the largest real root of a
-// real-rooted polynomial it decreases monotonically onto it).  False when it
-// does not converge in 40 steps.
-DEV bool laguerre_real(const double* c, double x0, double* root) {
-  double x = x0;
-  for (int it = 0; it < 40; it++) {
-    double b = c[V], d = 0.0, f = 0.0;
+// K real roots of monic degree-V polynomials by Laguerre's method (cubic
+// convergence at simple roots; from above the largest real root of a
+// real-rooted polynomial it decreases monotonically onto it), root k of c[k]
+// from x0[k].  The K iterations are independent and advance in one loop;
+// each stops at its own convergence exactly as it would alone.  The step
+// n / (G +- sqrt((n-1)(n H - G^2))), G = p'/p, H = G^2 - p''/p, is evaluated
+// multiplied through by p: n p / (p' +- sqrt((n-1)((n-1) p'^2 - n p p''))),
+// one division.  The Horner recurrence for p, p' and p''/2 starts after its
+// first two steps, which are exact for the monic c (c[V] = 1, finite x).
+// Converged: |step| <= 4 eps |x|, or |step| <= 1e-8 |x| with |step| |x| <=
+// |previous step|^2 (superlinear convergence seen, so the remaining error
+// is below |step|^2.5).  ok[k] is false when root k does not converge in
+// 40 steps; dp[k] is p' where p was last evaluated (at the root, or one step
+// of at most 1e-8 |root| before it).
+template <int K>
+DEV void laguerre_multi(const double (&c)[K][V + 1], const double (&x0)[K], double (&root)[K],
+                        double (&dp)[K], bool (&ok)[K]) {
+  double x[K], prev[K];
+  int state[K];  // 0 running, 1 converged, 2 failed
 #pragma unroll
-    for (int j = V - 1; j >= 0; j--) {
-      f = x * f + d;
-      d = x * d + b;
-      b = x * b + c[j];
-    }
-    if (b == 0.0) {
-      *root = x;
-      return true;
-    }
-    const double G = d / b;
-    const double Hh = G * G - 2.0 * (f / b);
-    const double r = (double)(V - 1) * ((double)V * Hh - G * G);
-    const double sq = r > 0.0 ? dsqrt(r) : 0.0;
-    const double den = (G >= 0.0) ? G + sq : G - sq;
-    if (!(den != 0.0)) return false;
-    const double step = (double)V / den;
-    x = x - step;
-    if (!(dabs(step) > 4.0 * 2.220446049250313e-16 * dabs(x)) ||
-        dabs(step) <= 2.2250738585072014e-308) {
-      *root = x;
-      return dabs(x) <= 1e300;
+  for (int k = 0; k < K; k++) {
+    x[k] = x0[k];
+    prev[k] = 0.0;
+    state[k] = 0;
+    root[k] = 0.0;
+    dp[k] = 0.0;
+  }
+  for (int it = 0; it < 40; it++) {
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < K; k++) any = any || state[k] == 0;
+    if (!any) break;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const double xk = x[k];
+      const double b1 = xk + c[k][V - 1];
+      double f = 1.0, d = xk + b1, b = xk * b1 + c[k][V - 2];
+#pragma unroll
+      for (int j = V - 3; j >= 0; j--) {
+        f = xk * f + d;
+        d = xk * d + b;
+        b = xk * b + c[k][j];
+      }
+      const double r = (double)(V - 1) * ((double)(V - 1) * (d * d) - (double)(2 * V) * (f * b));
+      const double sq = r > 0.0 ? dsqrt(r) : 0.0;
+      const double den = (d >= 0.0) ? d + sq : d - sq;
+      const double step = ((double)V * b) / den;
+      const double xn = xk - step;
+      const double as = dabs(step), ax = dabs(xn);
+      const bool conv = !(as > 4.0 * 2.220446049250313e-16 * ax) ||
+                        as <= 2.2250738585072014e-308 ||
+                        (as <= 1e-8 * ax && as * ax <= prev[k] * prev[k]);
+      if (state[k] == 0) {
+        dp[k] = d;
+        if (b == 0.0) {
+          root[k] = xk;
+          state[k] = 1;
+        } else if (!(den != 0.0)) {
+          state[k] = 2;
+        } else {
+          x[k] = xn;
+          prev[k] = as;
+          if (conv) {
+            root[k] = xn;
+            state[k] = ax <= 1e300 ? 1 : 2;
+          }
+        }
+      }
     }
   }
-  return false;
+#pragma unroll
+  for (int k = 0; k < K; k++) ok[k] = state[k] == 1;
 }
 
-// max |lambda| of the scaled upper Hessenberg H from its characteristic
-// polynomial: the largest and smallest real roots by Laguerre from
-// +-||H||_inf, then a certificate that every other root is smaller in modulus
-// -- q = p / ((x - l1)(x - l2)) centred on the mean of its roots, alpha, and
-// its roots bounded by |z - alpha| <= R, R at or above the Cauchy radius of
-// q(y + alpha) (positive root of y^m - sum |t_k| y^k).  False (the caller
-// falls back to QR) when a root does not converge or the certificate fails:
-// complex or repeated extreme eigenvalues.
-DEV bool charpoly_max_abs(const double* H, double* out) {
-  constexpr int m = V - 2;
-  double c[V + 1];
+// Characteristic polynomial c of the scaled upper Hessenberg H and the
+// starting points of the Laguerre iterations for its largest and smallest
+// real roots: mu +- 1.001 sigma sqrt(V - 1), mu and sigma^2 the mean and
+// variance of the roots from c[V-1] and c[V-2] (Laguerre-Samuelson: every
+// root of a real-rooted polynomial lies in mu +- sigma sqrt(V - 1)), or
+// +-||H||_inf when sigma^2 is not positive.
+DEV void charpoly_setup(const double* H, double* c, double* xhi, double* xlo) {
   hess_charpoly(H, c);
-  double U = 0.0;
+  const double mu = -(c[V - 1] / (double)V);
+  const double s2 = (c[V - 1] * c[V - 1] - 2.0 * c[V - 2]) / (double)V - mu * mu;
+  if (s2 > 0.0 && s2 <= 1e300) {
+    const double w = 1.001 * dsqrt(s2 * (double)(V - 1));
+    *xhi = mu + w;
+    *xlo = mu - w;
+    return;
+  }
+  double u = 0.0;
 #pragma unroll
   for (int i = 0; i < V; i++) {
     double rs = 0.0;
 #pragma unroll
     for (int j = 0; j < V; j++) rs = rs + dabs(H[i * V + j]);
-    U = dmax(U, rs);
+    u = dmax(u, rs);
   }
-  double l1, l2;
-  if (!laguerre_real(c, U, &l1) || !laguerre_real(c, -U, &l2)) return false;
-  if (!(l1 > l2)) return false;
+  *xhi = u;
+  *xlo = -u;
+}
+
+// Given the largest and smallest real roots l1 > l2 of c, with p' = d1, d2
+// there, check that they are two roots (l1 - l2 > 1e-9 M), that both are
+// accurate -- the rounding error of evaluating p,
+// 2 V eps sum_k |c_k| |x|^k, moves a root by that over |p'|, which must stay
+// below 1e-11 M, M = max(|l1|, |l2|) (a simple root next to a cluster is
+// ill-conditioned as a polynomial root) -- and certify that every
+// other root is smaller in modulus: q = p / ((x - l1)(x - l2)) is centred on
+// the mean of its roots, alpha, and its roots bounded by |z - alpha| <= R,
+// R = 2^(E+1) >= 2 max_k |t_k|^(1/(m-k)) (Fujiwara's bound, in powers of
+// two: |t_k| < 2^e_k gives |t_k|^(1/j) < 2^ceil(e_k/j)) for t(y) = q(y +
+// alpha).  False (the caller falls back to QR) when the certificate fails:
+// complex or repeated extreme eigenvalues.
+DEV bool charpoly_certify(const double* c, double l1, double l2, double d1, double d2,
+                          double* out) {
+  constexpr int m = V - 2;
+  const double M = dmax(dabs(l1), dabs(l2));
+  // two distinct roots (both runs can land on the one real root)
+  if (!(l1 - l2 > 1e-9 * M)) return false;
+  {
+    const double a1 = dabs(l1), a2 = dabs(l2);
+    double E1 = 1.0, E2 = 1.0;
+#pragma unroll
+    for (int j = V - 1; j >= 0; j--) {
+      E1 = a1 * E1 + dabs(c[j]);
+      E2 = a2 * E2 + dabs(c[j]);
+    }
+    if (!((double)(2 * V) * 2.220446049250313e-16 * E1 <= 1e-11 * M * dabs(d1))) return false;
+    if (!((double)(2 * V) * 2.220446049250313e-16 * E2 <= 1e-11 * M * dabs(d2))) return false;
+  }
   // q = p / (x - l1) / (x - l2)
   double q1[V], q[V - 1];
   q1[V - 1] = c[V];
@@ -265,7 +338,6 @@ DEV bool charpoly_max_abs(const double* H, double* out) {
   q[V - 2] = q1[V - 1];
 #pragma unroll
   for (int j = V - 3; j >= 0; j--) q[j] = q1[j + 1] + l2 * q[j + 1];
-  const double M = dmax(dabs(l1), dabs(l2));
   const double alpha = -(q[m - 1] / (double)m);
   double t[m + 1];
 #pragma unroll
@@ -274,55 +346,42 @@ DEV bool charpoly_max_abs(const double* H, double* out) {
   for (int k = 0; k < m; k++)
 #pragma unroll
     for (int j = m - 1; j >= k; j--) t[j] = t[j] + alpha * t[j + 1];
-  double amax = 0.0;
+  double tsum = 0.0;
 #pragma unroll
-  for (int k = 0; k < m; k++) amax = dmax(amax, dabs(t[k]));
-  double R = 1.0 + amax;
-  for (int e = 0; e < 64; e++) {
-    const double yh = 0.5 * R;
-    double f = 1.0;
+  for (int k = 0; k < m; k++) tsum = tsum + dabs(t[k]);
+  if (!(tsum <= 1.7976931348623157e308)) return false;
+  int emax = -2000;
 #pragma unroll
-    for (int k = m - 1; k >= 0; k--) f = f * yh - dabs(t[k]);
-    if (!(f >= 0.0)) break;
-    R = yh;
-  }
-  for (int it = 0; it < 3; it++) {
-    double f = 1.0, fp = 0.0;
-#pragma unroll
-    for (int k = m - 1; k >= 0; k--) {
-      fp = fp * R + f;
-      f = f * R - dabs(t[k]);
+  for (int k = 0; k < m; k++) {
+    if (t[k] != 0.0) {
+      int e;
+      (void)__builtin_frexp(t[k], &e);  // |t_k| = f 2^e, f in [0.5, 1)
+      const int jj = m - k;
+      const int ce = e >= 0 ? (e + jj - 1) / jj : -((-e) / jj);
+      emax = ce > emax ? ce : emax;
     }
-    if (!(fp > 0.0)) break;
-    const double Rn = R - f / fp;
-    double g = 1.0;
-#pragma unroll
-    for (int k = m - 1; k >= 0; k--) g = g * Rn - dabs(t[k]);
-    if (!(g >= 0.0) || !(Rn > 0.0)) break;
-    R = Rn;
   }
+  const double R = emax == -2000 ? 0.0 : __builtin_ldexp(1.0, emax + 1);
   if (!(dabs(alpha) + R < M)) return false;
   *out = M;
   return true;
 }
 #endif
 
-// max |lambda| of A (V x V, row-major; destroyed), as the reference takes it from
-// Eigen's EigenSolver (system.cpp:28-43): scale by the power of two at or above
-// the largest |entry| (exact; Eigen's RealSchur divides by the largest |entry|,
-// RealSchur.h:258-276), Householder Hessenberg reduction, then
-// Francis double-shift QR steps (Golub & Van Loan Alg. 7.5.1-7.5.2, EISPACK
-// hqr structure) with deflation on eps-small subdiagonals and origin-moving
-// exceptional shifts after 10 and 30 steps without deflation; after 40 V steps
-// the remaining diagonal stands in for unresolved eigenvalues.  All array
-// indices are compile-time constants; run-time bounds only predicate.
-DEV double max_abs_eig(double* H) {
+// Scale A (destroyed) by the power of two 2^-e2 at or above its largest
+// |entry| and reduce it to upper Hessenberg form in place.  False when A is
+// zero, subnormal or NaN; *early is then max |lambda|.
+DEV bool eig_prepare(double* H, int* e2out, double* early) {
   double scale = 0.0;
 #pragma unroll
   for (int i = 0; i < V * V; i++) scale = dmax(scale, dabs(H[i]));
-  if (!(scale >= 2.2250738585072014e-308)) return (scale != scale) ? scale : 0.0;
+  if (!(scale >= 2.2250738585072014e-308)) {
+    *early = (scale != scale) ? scale : 0.0;
+    return false;
+  }
   int e2;
   (void)__builtin_frexp(scale, &e2);  // scale = m 2^e2, m in [0.5, 1)
+  *e2out = e2;
   const double pw = __builtin_ldexp(1.0, -e2);
 #pragma unroll
   for (int i = 0; i < V * V; i++) H[i] = H[i] * pw;
@@ -338,12 +397,12 @@ DEV double max_abs_eig(double* H) {
       house_right<m, k + 1>(H, vv, beta, 0, V - 1);
     }
   });
-#if PDE_V >= 3
-  {
-    double r;
-    if (charpoly_max_abs(H, &r)) return __builtin_ldexp(r, e2);
-  }
-#endif
+  return true;
+}
+
+// Francis double-shift QR on the scaled Hessenberg H (destroyed): max
+// |lambda| in the scaled units.
+DEV double eig_qr(double* H) {
   double anorm = 0.0;
 #pragma unroll
   for (int i = 0; i < V * V; i++) anorm = anorm + dabs(H[i]);
@@ -474,7 +533,91 @@ DEV double max_abs_eig(double* H) {
       });
     }
   }
-  return __builtin_ldexp(best, e2);
+  return best;
+}
+
+// max |lambda| of A (V x V, row-major; destroyed), as the reference takes it from
+// Eigen's EigenSolver (system.cpp:28-43): scale by the power of two at or above
+// the largest |entry| (exact; Eigen's RealSchur divides by the largest |entry|,
+// RealSchur.h:258-276), Householder Hessenberg reduction, then
+// Francis double-shift QR steps (Golub & Van Loan Alg. 7.5.1-7.5.2, EISPACK
+// hqr structure) with deflation on eps-small subdiagonals and origin-moving
+// exceptional shifts after 10 and 30 steps without deflation; after 40 V steps
+// the remaining diagonal stands in for unresolved eigenvalues.  All array
+// indices are compile-time constants; run-time bounds only predicate.
+DEV double max_abs_eig(double* H) {
+  int e2 = 0;
+  double early;
+  if (!eig_prepare(H, &e2, &early)) return early;
+#if PDE_V >= 3
+  {
+    double c[2][V + 1], x0[2], root[2], dp[2], r;
+    bool ok[2];
+    charpoly_setup(H, c[0], &x0[0], &x0[1]);
+#pragma unroll
+    for (int j = 0; j <= V; j++) c[1][j] = c[0][j];
+    laguerre_multi<2>(c, x0, root, dp, ok);
+    if (ok[0] && ok[1] && charpoly_certify(c[0], root[0], root[1], dp[0], dp[1], &r))
+      return __builtin_ldexp(r, e2);
+  }
+#endif
+  return __builtin_ldexp(eig_qr(H), e2);
+}
+
+// max_abs_eig of two matrices at once (both destroyed): the same arithmetic
+// for each, with the four Laguerre iterations in one loop (used where both
+// matrices fit the register budget: the per-cell wave speeds of k_dt).
+DEV void max_abs_eig2(double* HA, double* HB, double* ra, double* rb) {
+  int ea = 0, eb = 0;
+  double xa = 0.0, xb = 0.0;
+  const bool pa = eig_prepare(HA, &ea, &xa);
+  const bool pb = eig_prepare(HB, &eb, &xb);
+  bool okA = false, okB = false;
+  double mA = 0.0, mB = 0.0;
+#if PDE_V >= 3
+  {
+    // a matrix answered by eig_prepare joins as the zero matrix, whose
+    // Laguerre iterations stop at once
+    if (!pa)
+#pragma unroll
+      for (int i = 0; i < V * V; i++) HA[i] = 0.0;
+    if (!pb)
+#pragma unroll
+      for (int i = 0; i < V * V; i++) HB[i] = 0.0;
+    double c[4][V + 1], x0[4], root[4], dp[4];
+    bool ok[4];
+    charpoly_setup(HA, c[0], &x0[0], &x0[1]);
+    charpoly_setup(HB, c[2], &x0[2], &x0[3]);
+#pragma unroll
+    for (int j = 0; j <= V; j++) {
+      c[1][j] = c[0][j];
+      c[3][j] = c[2][j];
+    }
+    laguerre_multi<4>(c, x0, root, dp, ok);
+    okA = pa && ok[0] && ok[1] && charpoly_certify(c[0], root[0], root[1], dp[0], dp[1], &mA);
+    okB = pb && ok[2] && ok[3] && charpoly_certify(c[2], root[2], root[3], dp[2], dp[3], &mB);
+  }
+#endif
+  const bool needA = pa && !okA, needB = pb && !okB;
+  double qa = 0.0, qb = 0.0;
+  if (needA || needB) {
+    double Hf[V * V];
+#pragma unroll 1
+    for (int pass = 0; pass < 2; pass++) {
+      const bool doit = pass == 0 ? needA : needB;
+      if (doit) {
+#pragma unroll
+        for (int i = 0; i < V * V; i++) Hf[i] = pass == 0 ? HA[i] : HB[i];
+        const double r = eig_qr(Hf);
+        if (pass == 0)
+          qa = r;
+        else
+          qb = r;
+      }
+    }
+  }
+  *ra = !pa ? xa : __builtin_ldexp(okA ? mA : qa, ea);
+  *rb = !pb ? xb : __builtin_ldexp(okB ? mB : qb, eb);
 }
 
 // ------------------------------------------------------ system functions
@@ -528,6 +671,15 @@ DEV double max_abs_eigs(const double* q, const double* dq, int d, double* f0) {
   double A[V * V];
   system_matrix(q, dq, d, A, f0);
   return max_abs_eig(A);
+}
+
+// max_abs_eigs for two states (or directions) at once.
+DEV void max_abs_eigs2(const double* qa, const double* dqa, int da, double* fa, const double* qb,
+                       const double* dqb, int db, double* fb, double* la, double* lb) {
+  double A[V * V], B[V * V];
+  system_matrix(qa, dqa, da, A, fa);
+  system_matrix(qb, dqb, db, B, fb);
+  max_abs_eig2(A, B, la, lb);
 }
 
 #ifndef PDE_SECOND

@@ -207,10 +207,17 @@ k_dt(const double* __restrict__ wh, StepState* st, Geom g) {
 #pragma unroll
       for (int v = 0; v < V; v++) dq[d * V + v] = acc[v];
     });
+    double lams[D];
+    if constexpr (D == 2) {
+      double f0[V], f1[V];
+      max_abs_eigs2(q, dq, 0, f0, q, dq, 1, f1, &lams[0], &lams[1]);
+    } else {
+      double f0[V];
+      lams[0] = max_abs_eigs(q, dq, 0, f0);
+    }
     static_for<0, D>([&](auto dc) {
       constexpr int d = decltype(dc)::value;
-      double f0[V];
-      double lam = max_abs_eigs(q, dq, d, f0);
+      double lam = lams[d];
 #if PDE_SECOND
       lam = lam + max_abs_eigs_second_order<d>(q, dq, g.dX[d]);
 #endif

@@ -143,6 +143,23 @@ def test_max_abs_eig_numpy():
     assert abs(cc.oracle().oracle_max_abs_eig(cc.dp(J), 4) - 2.0) < 1e-3
     R = np.ascontiguousarray([[0.0, -3.0], [3.0, 0.0]])
     assert abs(cc.oracle().oracle_max_abs_eig(cc.dp(R), 2) - 3.0) < 1e-14
+    # one real root and a larger complex pair: the Laguerre runs from both ends
+    # reach the same real root, which must not pass as two roots
+    for re_, im_, lr in [(0.28, 1.47, 0.448), (-0.1, 2.0, 1.0), (0.0, 1.0, -0.999)]:
+        B = np.array([[lr, 0, 0], [0, re_, -im_], [0, im_, re_]])
+        P = np.random.default_rng(5).standard_normal((3, 3))
+        A = np.ascontiguousarray(P @ B @ np.linalg.inv(P))
+        want = np.max(np.abs(np.linalg.eigvals(A)))
+        assert abs(cc.oracle().oracle_max_abs_eig(cc.dp(A), 3) - want) <= 1e-12 * want
+    # a simple extreme eigenvalue 0.036 from a five-fold cluster split by 1e-12
+    # noise: as a polynomial root it is good to ~1e-8 only, so the accuracy
+    # check must send it to QR
+    D = np.diag([-0.90238406] * 5 + [-0.97495421, -0.78669644])
+    P = np.random.default_rng(6).standard_normal((7, 7))
+    A = np.ascontiguousarray(P @ D @ np.linalg.inv(P)
+                             + 1e-12 * np.random.default_rng(7).standard_normal((7, 7)))
+    want = np.max(np.abs(np.linalg.eigvals(A)))
+    assert abs(cc.oracle().oracle_max_abs_eig(cc.dp(A), 7) - want) <= 1e-10 * want
 
 
 @pytest.mark.ref
