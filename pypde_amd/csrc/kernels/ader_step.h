@@ -805,25 +805,29 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
   }
   __syncthreads();
   if (valid) {
+    // traces with every lane busy: the N ND1 = ND lanes of a cell take one
+    // (time node t, transverse node j) each and write both ends e of every
+    // direction d and variable v
+    const int t = c.idx / ND1, j = c.idx % ND1;
     static_for<0, D>([&](auto dc) {
       constexpr int d = decltype(dc)::value;
       constexpr int stride = ipow_c(N, D - 1 - d);
-      if (c.kd[d] == 0) {
-        const int j = (c.idx / (stride * N)) * stride + c.idx % stride;
+      const int idx0 = (j / stride) * stride * N + j % stride;  // node with k_d = 0
+      double qm[N][V];
 #pragma unroll
-        for (int e = 0; e < 2; e++)
+      for (int m = 0; m < N; m++)
 #pragma unroll
-          for (int t = 0; t < N; t++)
+        for (int v = 0; v < V; v++) qm[m][v] = sm.q[c.slot][t][idx0 + m * stride][v];
 #pragma unroll
-            for (int v = 0; v < V; v++) {
-              double acc = (e ? T_END1[0] : T_END0[0]) * sm.q[c.slot][t][c.idx][v];
+      for (int e = 0; e < 2; e++)
 #pragma unroll
-              for (int m = 1; m < N; m++)
-                acc = dfma(e ? T_END1[m] : T_END0[m], sm.q[c.slot][t][c.idx + m * stride][v], acc);
-              const int plane = (((d * 2 + e) * N + t) * ND1 + j) * V + v;
-              tr[(long)plane * g.nwc + cell] = acc;
-            }
-      }
+        for (int v = 0; v < V; v++) {
+          double acc = (e ? T_END1[0] : T_END0[0]) * qm[0][v];
+#pragma unroll
+          for (int m = 1; m < N; m++) acc = dfma(e ? T_END1[m] : T_END0[m], qm[m][v], acc);
+          const int plane = (((d * 2 + e) * N + t) * ND1 + j) * V + v;
+          tr[(long)plane * g.nwc + cell] = acc;
+        }
     });
     if (qh) {
 #pragma unroll
@@ -867,13 +871,17 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
     for (int v = 0; v < V; v++) sm.q[c.slot][0][c.idx][v] = part[v];
   }
   __syncthreads();
-  if (valid && c.idx == 0) {
+  // the cell's lane idx sums variable v = idx (+ ND, ...) over the nodes in order
+  if (valid) {
 #pragma unroll
-    for (int v = 0; v < V; v++) {
-      double acc = sm.q[c.slot][0][0][v];
+    for (int v0 = 0; v0 < V; v0 += ND) {
+      const int v = v0 + c.idx;
+      if (v < V) {
+        double acc = sm.q[c.slot][0][0][v];
 #pragma unroll
-      for (int i = 1; i < ND; i++) acc = acc + sm.q[c.slot][0][i][v];
-      vol[(long)v * g.nwc + cell] = acc;
+        for (int i = 1; i < ND; i++) acc = acc + sm.q[c.slot][0][i][v];
+        vol[(long)v * g.nwc + cell] = acc;
+      }
     }
   }
 #endif
