@@ -2,6 +2,7 @@
 channel, order 3, stiff per-cell Newton predictor, reflective walls, FP64.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-learned]
+                    [--no-ensemble]
 
 Prints one JSON line.  A step is one full ADER-WENO timestep of the whole
 grid (boundaries + WENO, CFL reduction, learned guess, predictor, face fluxes,
@@ -12,8 +13,14 @@ Two legs on the same seeded initial state (seed 99): the plain FP64 method
 (constant-in-time Newton initial guess) and the learned one (an MLP trained
 here, on the GPU, on predictor solutions harvested from seeds 0..15 of the same
 channel at 256 x 64 cells and the same cell size).  `value` is the learned
-leg's rate; both legs' rates, Newton iteration counts and their difference in
-the final state are reported beside it.
+leg's rate where its final state stays within 1e-8 (relative L_inf) of the
+plain one and it is faster, else the plain leg's (`value_leg` says which and
+why); both legs' rates, Newton iteration counts and their difference in the
+final state are reported beside it, and an ensemble leg (48 small 2D Euler
+grids, one graph launch per timestep) after them.
+
+    python bench.py --calibrate-copy   # 3 x 256 MiB device copy, for HBM
+                                       # counter calibration under rocprofv3
 """
 import argparse
 import ctypes as C
@@ -226,7 +233,20 @@ def main():
     ap.add_argument("--train-steps", type=int, default=12)
     ap.add_argument("--train-pairs", type=int, default=2_000_000)
     ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 256 MiB device to device three times (HBM counter "
+                         "calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = 32 << 20
+        a = torch.zeros(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n * 8, "launches": 3}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
@@ -234,27 +254,42 @@ def main():
     K, W = args.steps, args.warmup
     cells = NX * NY
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
-    st_p, el_p, _, u_p = gpu_run(K, W, profile=True)
-    log(f"plain: {el_p * 1e3 / K:.3f} ms/step")
+    # timed run without per-kernel events; a second, profiled run of the same
+    # steps gives the kernel breakdown and the roofline
+    st_p, el_p, _, u_p = gpu_run(K, W, profile=False)
+    st_k, el_k, _, _ = gpu_run(K, W, profile=True)
+    log(f"plain: {el_p * 1e3 / K:.3f} ms/step ({el_k * 1e3 / K:.3f} with kernel events)")
     plain = dict(value=cells * K / el_p, ms_per_step=el_p * 1e3 / K,
                  iters_per_cell=st_p["pred_iters"] / max(1, st_p["pred_cells"]),
                  max_iters=st_p["pred_maxit"], capped=st_p["pred_capped"],
-                 kernel_ms_per_step={k: v / K for k, v in st_p["ms_kernel"].items()},
-                 roofline=predictor_roofline(st_p, K))
+                 kernel_ms_per_step={k: v / K for k, v in st_k["ms_kernel"].items()},
+                 roofline=predictor_roofline(st_k, K))
     learned = None
     if not args.no_learned:
         net, info = train_guess_model(args.train_steps, args.train_pairs, args.epochs, log)
         log(f"trained: {info}")
-        st_l, el_l, _, u_l = gpu_run(K, W, profile=True, net=net)
+        st_l, el_l, _, u_l = gpu_run(K, W, profile=False, net=net)
+        st_lk, _, _, _ = gpu_run(K, W, profile=True, net=net)
         log(f"learned: {el_l * 1e3 / K:.3f} ms/step")
         learned = dict(value=cells * K / el_l, ms_per_step=el_l * 1e3 / K,
                        iters_per_cell=st_l["pred_iters"] / max(1, st_l["pred_cells"]),
                        max_iters=st_l["pred_maxit"], capped=st_l["pred_capped"],
                        guess_fallbacks=st_l["guess_fallbacks"],
-                       kernel_ms_per_step={k: v / K for k, v in st_l["ms_kernel"].items()},
+                       kernel_ms_per_step={k: v / K for k, v in st_lk["ms_kernel"].items()},
                        linf_vs_plain=rel_linf(u_l, u_p), training=info,
-                       roofline=predictor_roofline(st_l, K))
-    head = learned if learned is not None else plain
+                       roofline=predictor_roofline(st_lk, K))
+    # the headline is the learned leg only where it keeps the metric's accuracy
+    # bound (final state within 1e-8 of the plain FP64 one) and is faster
+    use_learned = (learned is not None and learned["linf_vs_plain"] <= 1e-8
+                   and learned["value"] > plain["value"])
+    head = learned if use_learned else plain
+    if learned is None:
+        value_leg = "plain (learned leg not run)"
+    elif use_learned:
+        value_leg = "learned"
+    else:
+        value_leg = ("plain (learned leg: L_inf vs plain %.2e, %.2fx the plain rate)"
+                     % (learned["linf_vs_plain"], learned["value"] / plain["value"]))
     ens = None
     if not args.no_ensemble:
         ens = ensemble_leg(20)
@@ -272,6 +307,7 @@ def main():
         "steps": K,
         "warmup": W,
         "ms_per_step": head["ms_per_step"],
+        "value_leg": value_leg,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -281,7 +317,7 @@ def main():
                                "1024x256 channel, order 3, Rusanov, CFL 0.9, reflective walls, "
                                "stiff per-cell Newton predictor (tol 1e-12), "
                                + ("learned Newton initial guess (BF16 MLP)"
-                                  if learned is not None else "plain FP64"),
+                                  if use_learned else "plain FP64"),
                    "cells": cells, "order": ORDER, "nvar": V},
         "roofline": head["roofline"],
         "cpu_baseline": cpu,

@@ -486,15 +486,16 @@ static int laguerre_real(const double *c, int n, double x0, double *root, double
 static int charpoly_max_abs(const double *H, int n, double *out) {
   double c[MAXV + 1], q[MAXV + 1], t[MAXV + 1];
   hess_charpoly(H, n, c);
-  /* starting points mu +- 1.001 sigma sqrt(n - 1), mu and sigma^2 the mean and
-   * variance of the roots from c[n-1] and c[n-2] (Laguerre-Samuelson: every
-   * root of a real-rooted polynomial lies in mu +- sigma sqrt(n - 1)), or
-   * +-||H||_inf when sigma^2 is not positive */
+  /* starting points mu +- 1.001 sigma sqrt(n / 2), mu and sigma^2 the mean and
+   * variance of the roots from c[n-1] and c[n-2] -- for a spectrum of pairs
+   * u +- a_i around roots at u (the waves of a hyperbolic system) sigma^2 n / 2
+   * = sum a_i^2, at or above the largest a_i^2 -- or +-||H||_inf when sigma^2
+   * is not positive.  Any start is safe: the certificate decides. */
   double xhi, xlo;
   double mu = -(c[n - 1] / (double)n);
   double s2 = (c[n - 1] * c[n - 1] - 2.0 * c[n - 2]) / (double)n - mu * mu;
   if (s2 > 0.0 && s2 <= 1e300) {
-    double w = 1.001 * sqrt(s2 * (double)(n - 1));
+    double w = 1.001 * sqrt(s2 * (0.5 * (double)n));
     xhi = mu + w;
     xlo = mu - w;
   } else {

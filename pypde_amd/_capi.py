@@ -86,6 +86,7 @@ def lib():
         L.pypde_ensemble_get_state.argtypes = [C.c_void_p, C.c_int, PD]
         L.pypde_ensemble_run.argtypes = [C.c_void_p, C.c_double, C.c_long, C.POINTER(Stats), PD]
         L.pypde_ensemble_destroy.argtypes = [C.c_void_p]
+        L.pypde_eig_batch.argtypes = [C.c_void_p, C.c_int, PD, PD, C.c_long, PD]
         _lib = L
     return _lib
 

@@ -275,16 +275,17 @@ DEV void laguerre_multi(const double (&c)[K][V + 1], const double (&x0)[K], doub
 
 // Characteristic polynomial c of the scaled upper Hessenberg H and the
 // starting points of the Laguerre iterations for its largest and smallest
-// real roots: mu +- 1.001 sigma sqrt(V - 1), mu and sigma^2 the mean and
-// variance of the roots from c[V-1] and c[V-2] (Laguerre-Samuelson: every
-// root of a real-rooted polynomial lies in mu +- sigma sqrt(V - 1)), or
-// +-||H||_inf when sigma^2 is not positive.
+// real roots: mu +- 1.001 sigma sqrt(V / 2), mu and sigma^2 the mean and
+// variance of the roots from c[V-1] and c[V-2] -- for a spectrum of pairs
+// u +- a_i around roots at u (the waves of a hyperbolic system) sigma^2 V / 2
+// = sum a_i^2, at or above the largest a_i^2 -- or +-||H||_inf when sigma^2
+// is not positive.  Any start is safe: the certificate decides.
 DEV void charpoly_setup(const double* H, double* c, double* xhi, double* xlo) {
   hess_charpoly(H, c);
   const double mu = -(c[V - 1] / (double)V);
   const double s2 = (c[V - 1] * c[V - 1] - 2.0 * c[V - 2]) / (double)V - mu * mu;
   if (s2 > 0.0 && s2 <= 1e300) {
-    const double w = 1.001 * dsqrt(s2 * (double)(V - 1));
+    const double w = 1.001 * dsqrt(s2 * (0.5 * (double)V));
     *xhi = mu + w;
     *xlo = mu - w;
     return;

@@ -249,3 +249,43 @@ extern "C" __global__ void k_dt_finalize(StepState* st) {
   st->dt = dt;
   st->qhead = 0ull;
 }
+
+// ------------------------------------------------------- eigen diagnostics
+// The wave-speed and |A| routines on a batch of V x V matrices (row-major,
+// one per thread), for checking them against the CPU restatement:
+// which 0: max_abs_eig; 1: max_abs_eig2 on the pairs (A_2i, A_2i+1), out of
+// n entries; 2: abs_apply(A_i, x_i) (Roe and Osher modules), out of n V.
+extern "C" __global__ void __launch_bounds__(256)
+k_eig_batch(const double* __restrict__ A, const double* __restrict__ x, double* __restrict__ out,
+            long n, int which) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (which == 1) {
+    if (2 * i + 1 >= n) return;
+    double HA[V * V], HB[V * V];
+#pragma unroll
+    for (int k = 0; k < V * V; k++) {
+      HA[k] = A[(2 * i) * V * V + k];
+      HB[k] = A[(2 * i + 1) * V * V + k];
+    }
+    max_abs_eig2(HA, HB, &out[2 * i], &out[2 * i + 1]);
+    return;
+  }
+  if (i >= n) return;
+  double H[V * V];
+#pragma unroll
+  for (int k = 0; k < V * V; k++) H[k] = A[i * V * V + k];
+  if (which == 0) {
+    out[i] = max_abs_eig(H);
+    return;
+  }
+#if PDE_FLUX != 0
+  double xv[V], y[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) xv[v] = x[i * V + v];
+  abs_apply(H, xv, y);
+#pragma unroll
+  for (int v = 0; v < V; v++) out[i * V + v] = y[v];
+#else
+  (void)x;
+#endif
+}

@@ -240,11 +240,11 @@ int check_spec(const Spec& s, bool secondOrder, int FLUX) {
 }
 
 enum { K_WENO, K_DT, K_DTFIN, K_PRED, K_FLUX0, K_FLUX1, K_UPDATE, K_ADVANCE, K_A2S, K_S2A,
-       K_HARVEST, K_MLPGUESS, K_PRED_STATIC, K_PRED_STATIC2, K_COUNT };
+       K_HARVEST, K_MLPGUESS, K_PRED_STATIC, K_PRED_STATIC2, K_EIG, K_COUNT };
 const char* kernel_names[K_COUNT] = {"k_weno",   "k_dt",    "k_dt_finalize", "k_predictor",
                                      "k_flux0",  "k_flux1", "k_update",      "k_advance",
                                      "k_aos_to_soa", "k_soa_to_aos", "k_harvest", "k_mlp_guess",
-                                     "k_predictor_static", "k_predictor_static2"};
+                                     "k_predictor_static", "k_predictor_static2", "k_eig_batch"};
 
 struct MLPW {
   const float *W1t, *b1, *W2t, *b2, *W3t, *b3;
@@ -979,6 +979,45 @@ int pypde_set_guess_model(pypde_ctx* c, const pypde_mlp* m) {
               m->out_std};
   c->use_model = true;
   return 0;
+}
+
+int pypde_eig_batch(pypde_ctx* c, int which, const double* A, const double* x, long n,
+                    double* out) {
+  if (!c || !A || !out || n < 0 || which < 0 || which > 2)
+    return fail(PYPDE_ERR_ARG, "pypde_eig_batch: bad arguments");
+  if (which == 2 && (c->spec.flux == 0 || !x))
+    return fail(PYPDE_ERR_ARG, "pypde_eig_batch: abs_apply needs a Roe or Osher context and x");
+  if (which == 1 && (n & 1)) return fail(PYPDE_ERR_ARG, "pypde_eig_batch: pairs need an even n");
+  if (n == 0) return 0;
+  HIPCHK(hipSetDevice(c->device));
+  const long V = c->V, nout = which == 2 ? n * V : n;
+  double *dA = nullptr, *dx = nullptr, *dout = nullptr;
+  int rc = 0;
+  auto done = [&](int r) {
+    if (dA) hipFree(dA);
+    if (dx) hipFree(dx);
+    if (dout) hipFree(dout);
+    return r;
+  };
+  if (hipMalloc((void**)&dA, n * V * V * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&dout, nout * sizeof(double)) != hipSuccess ||
+      (which == 2 && hipMalloc((void**)&dx, n * V * sizeof(double)) != hipSuccess))
+    return done(fail(PYPDE_ERR_HIP, "pypde_eig_batch: hipMalloc failed"));
+  if (hipMemcpyAsync(dA, A, n * V * V * sizeof(double), hipMemcpyHostToDevice, c->stream) !=
+          hipSuccess ||
+      (which == 2 && hipMemcpyAsync(dx, x, n * V * sizeof(double), hipMemcpyHostToDevice,
+                                    c->stream) != hipSuccess))
+    return done(fail(PYPDE_ERR_HIP, "pypde_eig_batch: copy in failed"));
+  const double* a0 = dA;
+  const double* x0 = dx;
+  std::vector<void*> a = {&a0, &x0, &dout, &n, &which};
+  const long threads = which == 1 ? n / 2 : n;
+  if ((rc = launch(c, K_EIG, dim3((unsigned)cdiv(threads, 256)), dim3(256), a))) return done(rc);
+  if (hipMemcpyAsync(out, dout, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return done(fail(PYPDE_ERR_HIP, "pypde_eig_batch: copy out failed"));
+  return done(0);
 }
 
 int pypde_harvest_step(pypde_ctx* c, float* X, float* Y, long ldx, long col0, pypde_stats* stats) {

@@ -367,3 +367,102 @@ def test_viscous_path_matches_oracle():
         ug, _, sg = gpu_run("ns2d", u0, 0.02, [1.0, 1.0], "periodic", 3, False, flux=flux)
         print(f"ns2d {flux}: steps {sg['steps']} err {relerr(ug, uo):.3e}")
         assert rc == 0 and sg["steps"] == st.steps and relerr(ug, uo) <= TOL
+
+
+# ------------------------------------------------ eigen routines, matrix by matrix
+
+def _eig_cases(V, seed):
+    """Matrices that reach every branch of the wave-speed routine: random,
+    symmetric, Hessenberg, triangular, near-multiple clusters (as
+    finite-difference Jacobians of multi-wave systems give), a lone real root
+    beside a larger complex pair, a cluster next to the extreme root, Jordan
+    blocks, zero, tiny, huge and NaN matrices."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(400):
+        A = rng.standard_normal((V, V))
+        if k % 4 == 1:
+            A = A + A.T
+        elif k % 4 == 2:
+            A[np.tril_indices(V, -2)] = 0
+        elif k % 4 == 3:
+            A = np.triu(A)
+        out.append(A)
+    for k in range(200):
+        lam = rng.uniform(-2, 2)
+        D = np.diag(np.r_[np.full(V - 2, lam), rng.uniform(-4, 4, 2)])
+        P = rng.standard_normal((V, V))
+        out.append(P @ D @ np.linalg.inv(P) + 1e-12 * rng.standard_normal((V, V)))
+    for k in range(100):
+        re_, im_ = rng.uniform(-1, 1), rng.uniform(0.5, 3)
+        B = np.diag(rng.uniform(-0.5, 0.5, V))
+        B[V - 2:, V - 2:] = [[re_, -im_], [im_, re_]]
+        P = rng.standard_normal((V, V))
+        out.append(P @ B @ np.linalg.inv(P))
+    for k in range(50):
+        d = np.r_[np.full(V - 2, -0.9), -0.975, 0.5][:V]
+        P = rng.standard_normal((V, V))
+        out.append(P @ np.diag(d) @ np.linalg.inv(P) + 1e-12 * rng.standard_normal((V, V)))
+    J = 2.0 * np.eye(V) + np.eye(V, k=1)
+    out += [J, -J, np.zeros((V, V)), 1e-310 * rng.standard_normal((V, V)),
+            1e300 * rng.standard_normal((V, V)), np.full((V, V), np.nan)]
+    return np.ascontiguousarray(np.stack(out))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["euler1d", "euler2d", "reactive2d"])
+def test_max_abs_eig_batch_bitwise_equal_to_oracle(name):
+    """The GPU max |lambda| (single and two-at-a-time) equals the oracle's bit
+    for bit on every case of _eig_cases, V = nvar of the system."""
+    s = systems.SYSTEMS[name]
+    V = s["V"]
+    ndim = 1 if name == "euler1d" else 2
+    u0 = np.ones((8,) * ndim + (V,))
+    sol = Solver(u0, [1.0] * ndim, systems.source(name), order=3, stiff=name == "reactive2d")
+    try:
+        A = _eig_cases(V, 11 + V)
+        n = A.shape[0] - (A.shape[0] & 1)
+        A = np.ascontiguousarray(A[:n])
+        want = np.array([cc.oracle().oracle_max_abs_eig(cc.dp(np.ascontiguousarray(a)), V)
+                         for a in A])
+        for which in (0, 1):
+            got = np.zeros(n)
+            _capi.check(_capi.lib().pypde_eig_batch(sol.h, which, _capi.dp(A), None, n,
+                                                    _capi.dp(got)), "eig_batch")
+            assert np.array_equal(got, want, equal_nan=True), (which, np.flatnonzero(
+                ~((got == want) | (np.isnan(got) & np.isnan(want))))[:10])
+        # and against LAPACK on the finite cases, as a sanity bound
+        fin = np.isfinite(A).all(axis=(1, 2)) & (np.abs(A).max(axis=(1, 2)) < 1e200)
+        ref = np.array([np.max(np.abs(np.linalg.eigvals(a))) for a in A[fin]])
+        assert np.all(np.abs(want[fin] - ref) <= 1e-7 * ref + 1e-300)
+    finally:
+        sol.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["euler1d", "euler2d"])
+def test_abs_apply_batch_bitwise_equal_to_oracle(name):
+    """|A| x of the Roe/Osher fluxes on the GPU equals the oracle's bit for bit
+    (finite cases of _eig_cases with real-diagonalisable, complex and
+    defective spectra)."""
+    V = systems.SYSTEMS[name]["V"]
+    ndim = 1 if name == "euler1d" else 2
+    u0 = np.ones((8,) * ndim + (V,))
+    sol = Solver(u0, [1.0] * ndim, systems.source(name), order=3, stiff=False, flux="roe")
+    try:
+        A = _eig_cases(V, 21 + V)
+        A = np.ascontiguousarray(A[np.isfinite(A).all(axis=(1, 2))
+                                   & (np.abs(A).max(axis=(1, 2)) < 1e200)])
+        n = A.shape[0]
+        x = np.ascontiguousarray(np.random.default_rng(3).standard_normal((n, V)))
+        want = np.zeros((n, V))
+        for i in range(n):
+            cc.oracle().oracle_abs_apply(cc.dp(np.ascontiguousarray(A[i])),
+                                         cc.dp(np.ascontiguousarray(x[i])), V,
+                                         cc.dp(want[i]))
+        got = np.zeros((n, V))
+        _capi.check(_capi.lib().pypde_eig_batch(sol.h, 2, _capi.dp(A), _capi.dp(x), n,
+                                                _capi.dp(got)), "eig_batch")
+        assert np.array_equal(got, want, equal_nan=True)
+    finally:
+        sol.close()
