@@ -229,7 +229,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--train-steps", type=int, default=12)
     ap.add_argument("--train-pairs", type=int, default=2_000_000)
     ap.add_argument("--epochs", type=int, default=6)

@@ -15,7 +15,7 @@
 #include <string.h>
 
 #define MAXN 6
-#define MAXV 8
+#define MAXV 17
 #define MAXD 3
 #define MAXND 216 /* MAXN^MAXD */
 #define HOUSE_TINY 0x1p-900

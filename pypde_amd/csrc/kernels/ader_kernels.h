@@ -20,7 +20,7 @@ struct Geom {
   long nwc;      // prod nw
   long nf[2];    // faces per direction: (nx_d + 1) * prod_{e != d} nx_e
   double dX[2];
-  double refl[2][8];  // reflective-wall multipliers per axis (V <= 8)
+  double refl[2][17];  // reflective-wall multipliers per axis (V <= 17)
   int ndt;
 };
 

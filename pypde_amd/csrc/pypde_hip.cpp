@@ -31,7 +31,7 @@ namespace {
 
 #include "kernels_embed.inc"  // PRELUDE_SRC, COMMON_SRC, KERNELS_SRC, STEP_SRC
 
-constexpr int MAXV = 8;
+constexpr int MAXV = 17;
 thread_local std::string g_err;
 
 int fail(int code, const char* fmt, ...) {
@@ -234,6 +234,9 @@ int check_spec(const Spec& s, bool secondOrder, int FLUX) {
     return fail(PYPDE_ERR_ARG, "secondOrder needs F (the viscous flux depends on the gradient)");
   if (FLUX < 0 || FLUX > 2)
     return fail(PYPDE_ERR_ARG, "FLUX=%d: 0 Rusanov, 1 Roe or 2 Osher (fluxes.h:6-8)", FLUX);
+  if (s.V > 8 && (FLUX != 0 || secondOrder))
+    return fail(PYPDE_ERR_UNSUPPORTED,
+                "V=%d: above 8 variables only the Rusanov flux and first-order F are built", s.V);
   if (!s.src.size() && (s.useF || s.useB || s.useS))
     return fail(PYPDE_ERR_ARG, "empty system source");
   return 0;

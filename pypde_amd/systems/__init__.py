@@ -19,6 +19,7 @@ SYSTEMS = {
     "ncadvect": dict(V=2, F=False, B=True, S=True, M=False),
     "advdiff": dict(V=1, F=True, B=False, S=False, M=False, second=True),
     "ns2d": dict(V=4, F=True, B=False, S=False, M=False, second=True),
+    "gpr": dict(V=17, F=True, B=True, S=True, M=False),
 }
 
 
@@ -55,6 +56,12 @@ def standard_configs(full=False):
         s = SYSTEMS[name]
         out.append(dict(src=source(name), useF=s["F"], useB=s["B"], useS=s["S"], ndim=ndim, N=N,
                         V=s["V"], stiff=False, use_M=s["M"], flux=0, second=True))
+    # 17-variable GPR (1D, order 2, the reference's B.dQ evaluation); each takes
+    # a few minutes to compile
+    for stiff in ((False, True) if full else ()):
+        s = SYSTEMS["gpr"]
+        out.append(dict(src=source("gpr"), useF=True, useB=True, useS=True, ndim=1, N=2, V=17,
+                        stiff=stiff, use_M=False, flux=0, b_quirk=1))
     return out
 
 
@@ -139,6 +146,25 @@ def ensemble_sizes(n=48, sizes=(64, 128, 256, 512), seed=7):
     r = splitmix64(seed, 2 * n).reshape(n, 2)
     k = np.minimum((r * len(sizes)).astype(int), len(sizes) - 1)
     return [(sizes[a], sizes[b]) for a, b in k]
+
+
+def gpr_state(rho, p, v, gamma=1.4):
+    """Conserved GPR state for density rho, pressure p, velocity v (3), with
+    A = rho^(1/3) I and J = 0 (pypde/tests/gpr/misc/utils.py:26-35): then
+    dev(A^T A) = 0 and E = p / ((gamma - 1) rho) + |v|^2 / 2."""
+    Q = np.zeros(17)
+    Q[0] = rho
+    Q[1] = rho * (p / (rho * (gamma - 1)) + np.dot(v, v) / 2)
+    Q[2:5] = rho * np.asarray(v, dtype=np.float64)
+    Q[5:14] = (rho ** (1 / 3) * np.eye(3)).ravel()
+    return Q
+
+
+def gpr_riemann(n=200, left=(1.0, 1.0, (0.0, 0.0, 0.0)), right=(0.125, 0.1, (0.0, 0.0, 0.0))):
+    """1D GPR Riemann problem (pypde/tests/gpr/misc/utils.py:38-50): cells with
+    i / n < 1/2 take the left state, the rest the right one."""
+    QL, QR = gpr_state(*left), gpr_state(*right)
+    return np.ascontiguousarray(np.stack([QL if i / n < 0.5 else QR for i in range(n)]))
 
 
 REACTIVE_REFLECT = np.array([[1, -1, 1, 1, 1], [1, 1, -1, 1, 1]], dtype=np.float64)

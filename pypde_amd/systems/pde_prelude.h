@@ -21,12 +21,14 @@
 #define PDE_RINT(x) __builtin_rint(x)
 #define PDE_LDEXP(x, k) __builtin_ldexp(x, k)
 #define PDE_FMA(a, b, c) __builtin_fma(a, b, c)
+#define PDE_FREXP(x, e) __builtin_frexp(x, e)
 #else
 #include <math.h>
 #define PDE_FN static inline
 #define PDE_RINT(x) rint(x)
 #define PDE_LDEXP(x, k) ldexp(x, k)
 #define PDE_FMA(a, b, c) fma(a, b, c)
+#define PDE_FREXP(x, e) frexp(x, e)
 #endif
 
 /* exp(x) by Cody-Waite argument reduction (x = k ln2 + r, |r| <= ln2/2, ln2 split
@@ -59,6 +61,26 @@ PDE_FN double pde_exp(double x) {
   p = PDE_FMA(p, r, 1.0);
   p = PDE_FMA(p, r, 1.0);
   return PDE_LDEXP(p, (int)k);
+}
+
+/* Real cube root of x > 0: x = m 2^(3k) with m in [0.5, 4) (exact exponent
+ * split), Newton's iteration y <- y - (y^3 - m) / (3 y^2) from the linear
+ * guess 0.62 + 0.26 m (within 9 % of m^(1/3) on [0.5, 4)), six steps (the
+ * error squares each step), then 2^k.  Plain IEEE operations in a fixed order,
+ * so host and device agree bit for bit.  0 for x = 0, NaN for x < 0 or NaN. */
+PDE_FN double pde_cbrt(double x) {
+  if (x == 0.0) return 0.0;
+  if (!(x > 0.0)) return 0.0 / 0.0;
+  if (x > 1.7976931348623157e308) return x;
+  int e;
+  double m = PDE_FREXP(x, &e); /* x = m 2^e, m in [0.5, 1) */
+  int r = e % 3;
+  if (r < 0) r += 3;
+  const int k = (e - r) / 3;
+  m = PDE_LDEXP(m, r);
+  double y = 0.62 + 0.26 * m;
+  for (int i = 0; i < 6; i++) y = y - (y * y * y - m) / (3.0 * (y * y));
+  return PDE_LDEXP(y, k);
 }
 
 #endif

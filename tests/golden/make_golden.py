@@ -7,8 +7,10 @@ tables.npz  basis/WENO/DG tables for N = 1..5 (poly/basis.cpp, weno_matrices.cpp
 weno.npz    weno_solver (api.cpp:32-48) on seeded random data
 solves.npz  pde_solver end states: C0 Sod (order 3; Rusanov, Roe and Osher
             fluxes) plus the reference's own sensitivity on each to 1-ulp input
-            changes, 24x24 periodic 2D Euler, orders 2-4, and the viscous
-            path (advection-diffusion 1D, Navier-Stokes 2D)
+            changes, 24x24 periodic 2D Euler, orders 2-4, the viscous
+            path (advection-diffusion 1D, Navier-Stokes 2D) and the
+            17-variable GPR Riemann problem (Picard and stiff)
+            (python tests/golden/make_golden.py gpr refreshes only those)
 """
 import os
 import sys
@@ -80,9 +82,29 @@ def main():
     s["advdiff_final"] = to.ref_viscous("advdiff", to.advdiff_init(64), 0.2, [1.0], 1, 3)
     s["ns2d_final"] = to.ref_viscous("ns2d", to.systems.euler2d_modes(16, seed=2), 0.02,
                                      [1.0, 1.0], 1, 3)
+    s.update(gpr_vectors())
     np.savez_compressed(os.path.join(HERE, "solves.npz"), **s)
     print("golden vectors written:", sorted(os.listdir(HERE)))
 
 
+def gpr_vectors():
+    """17-variable GPR Riemann problem (1D, 200 cells, order 2, CFL 0.6) after
+    44 steps, Picard and stiff predictors (the stiff run takes about a minute)."""
+    u = to.systems.gpr_riemann(200)
+    out = {}
+    for stiff, name in ((False, "gpr_picard_final"), (True, "gpr_stiff_final")):
+        ur, _ = cc.ref_solve("gpr", u, to.GPR_TF, [1.0], 0, 2, stiff=stiff, cfl=0.6, nthreads=8)
+        out[name] = ur
+    return out
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["gpr"]:
+        # add or refresh only the GPR entries of solves.npz
+        path = os.path.join(HERE, "solves.npz")
+        s = dict(np.load(path, allow_pickle=False))
+        s.update(gpr_vectors())
+        np.savez_compressed(path, **s)
+        print("GPR golden vectors written")
+    else:
+        main()

@@ -409,16 +409,24 @@ def _eig_cases(V, seed):
     return np.ascontiguousarray(np.stack(out))
 
 
+def _eig_ctx(name):
+    """A small context whose kernel configuration is among the precompiled ones."""
+    V = systems.SYSTEMS[name]["V"]
+    if name == "gpr":
+        return Solver(systems.gpr_riemann(16), [1.0], systems.source(name), order=2, stiff=True,
+                      b_quirk=True)
+    ndim = 1 if name == "euler1d" else 2
+    return Solver(np.ones((8,) * ndim + (V,)), [1.0] * ndim, systems.source(name), order=3,
+                  stiff=name == "reactive2d")
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["euler1d", "euler2d", "reactive2d"])
+@pytest.mark.parametrize("name", ["euler1d", "euler2d", "reactive2d", "gpr"])
 def test_max_abs_eig_batch_bitwise_equal_to_oracle(name):
     """The GPU max |lambda| (single and two-at-a-time) equals the oracle's bit
-    for bit on every case of _eig_cases, V = nvar of the system."""
-    s = systems.SYSTEMS[name]
-    V = s["V"]
-    ndim = 1 if name == "euler1d" else 2
-    u0 = np.ones((8,) * ndim + (V,))
-    sol = Solver(u0, [1.0] * ndim, systems.source(name), order=3, stiff=name == "reactive2d")
+    for bit on every case of _eig_cases, V = nvar of the system (3, 4, 5, 17)."""
+    V = systems.SYSTEMS[name]["V"]
+    sol = _eig_ctx(name)
     try:
         A = _eig_cases(V, 11 + V)
         n = A.shape[0] - (A.shape[0] & 1)
@@ -466,3 +474,22 @@ def test_abs_apply_batch_bitwise_equal_to_oracle(name):
         assert np.array_equal(got, want, equal_nan=True)
     finally:
         sol.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("stiff", [False, True])
+def test_gpr_17_variables_bitwise_equal_to_oracle(stiff):
+    """V = 17 with F, B and S (the reference's GPR model, pypde/tests/gpr/
+    system.py:8-75): 1D Riemann problem, 200 cells, order 2, CFL 0.6, the
+    reference's B.dQ evaluation, 30 steps, Picard and stiff Newton predictors.
+    Tolerance: bitwise."""
+    u0 = systems.gpr_riemann(200)
+    uo, _, st, rc = cc.oracle_solve("gpr", u0, 1.0, [1.0], 0, 2, stiff=stiff, cfl=0.6, b_quirk=1,
+                                    max_steps=30)
+    assert rc == 0
+    ug = u0.copy()
+    stats = {}
+    pde_solver(ug, 1.0, [1.0], system=systems.source("gpr"), order=2, ndt=1, stiff=stiff,
+               cfl=0.6, b_quirk=True, max_steps=30, stats=stats)
+    assert stats["steps"] == st.steps == 30
+    assert np.array_equal(ug, uo)

@@ -741,3 +741,60 @@ def test_advection_diffusion_converges_to_exact():
     print("advection-diffusion L1 errors", errs, "orders", orders)
     assert np.allclose(errs, [4.039684724e-3, 8.632007961e-4, 1.949539833e-4], rtol=1e-6)
     assert min(orders) >= 2.0
+
+
+# ------------------------------------------------------------ 17-variable GPR
+
+GPR_TF = 0.005  # 44 steps of the 200-cell Riemann problem at CFL 0.6
+
+
+def gpr_relerr(a, b):
+    """relerr with each variable's scale floored at 1e-8 of the largest: in
+    this 1D problem the transverse momenta and most of A and J stay at zero
+    or at 1e-18 rounding residue."""
+    V = b.shape[-1]
+    sc = np.max(np.abs(b.reshape(-1, V)), axis=0)
+    sc = np.maximum(sc, 1e-8 * sc.max())
+    return float(np.max(np.abs((a - b).reshape(-1, V)) / sc))
+
+
+def oracle_gpr(stiff):
+    """GPR Riemann problem (1D, 200 cells, order 2, CFL 0.6, transmissive,
+    Rusanov) as the reference's GPR example runs it (pypde/tests/gpr/
+    __init__.py:32-36: F, B and S; order and flux at pde_solver's defaults),
+    with B.dQ evaluated the reference's way (dg.cpp:110, fv.cpp:234)."""
+    uo, _, st, rc = cc.oracle_solve("gpr", systems.gpr_riemann(200), GPR_TF, [1.0], 0, 2,
+                                    stiff=stiff, cfl=0.6, b_quirk=1)
+    assert rc == 0 and st.steps == 44
+    return uo
+
+
+@pytest.mark.ref
+def test_gpr_picard_reference():
+    """V = 17 with B != 0 and S (system.py:8-75): with the Picard predictor the
+    oracle and the reference run the same iteration, and agree to rounding
+    (3e-10 measured; the eigenvalue path differs: QR / polynomial here,
+    Spectra's Arnoldi there, system.cpp:37-42)."""
+    ur, _ = cc.ref_solve("gpr", systems.gpr_riemann(200), GPR_TF, [1.0], 0, 2, stiff=False,
+                         cfl=0.6, nthreads=8)
+    assert gpr_relerr(oracle_gpr(False), ur) <= 1e-8
+
+
+def test_gpr_golden():
+    """Against the reference's end states (tests/golden/make_golden.py).  The
+    stiff predictors differ by design: the reference's Newton-Krylov stops at
+    an absolute residual of 6e-6 (newton_krylov.cpp:54-76), the oracle's
+    per-node Newton at 1e-12, which over 44 steps moves the state by 2.5e-5
+    (relative L-inf); the bound is 1e-4."""
+    g = load_gold("solves")
+    assert gpr_relerr(oracle_gpr(False), g["gpr_picard_final"]) <= 1e-8
+    assert gpr_relerr(oracle_gpr(True), g["gpr_stiff_final"]) <= 1e-4
+
+
+def test_gpr_b_quirk_matters():
+    """The reference's B.dQ evaluation changes the GPR solution at the percent
+    level (5 % in rho u at 44 steps), so the comparison above depends on it."""
+    g = load_gold("solves")
+    uo, _, _, _ = cc.oracle_solve("gpr", systems.gpr_riemann(200), GPR_TF, [1.0], 0, 2,
+                                  stiff=False, cfl=0.6, b_quirk=0)
+    assert gpr_relerr(uo, g["gpr_picard_final"]) > 1e-3
