@@ -251,22 +251,18 @@ DEV void laguerre_multi(const double (&c)[K][V + 1], const double (&x0)[K], doub
       const bool conv = !(as > 4.0 * 2.220446049250313e-16 * ax) ||
                         as <= 2.2250738585072014e-308 ||
                         (as <= 1e-8 * ax && as * ax <= prev[k] * prev[k]);
-      if (state[k] == 0) {
-        dp[k] = d;
-        if (b == 0.0) {
-          root[k] = xk;
-          state[k] = 1;
-        } else if (!(den != 0.0)) {
-          state[k] = 2;
-        } else {
-          x[k] = xn;
-          prev[k] = as;
-          if (conv) {
-            root[k] = xn;
-            state[k] = ax <= 1e300 ? 1 : 2;
-          }
-        }
-      }
+      // state update as selects (no branches): a running root records p'; an
+      // exact zero of p stops at xk, a zero denominator fails, otherwise the
+      // step is taken and convergence (finite root) or failure decided
+      const bool run = state[k] == 0;
+      const bool exact = b == 0.0;
+      const bool step_ok = run && !exact && (den != 0.0);
+      const int ns = exact ? 1 : !(den != 0.0) ? 2 : conv ? (ax <= 1e300 ? 1 : 2) : 0;
+      dp[k] = run ? d : dp[k];
+      root[k] = run ? (exact ? xk : xn) : root[k];
+      x[k] = step_ok ? xn : xk;
+      prev[k] = step_ok ? as : prev[k];
+      state[k] = run ? ns : state[k];
     }
   }
 #pragma unroll
