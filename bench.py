@@ -214,11 +214,21 @@ def cpu_baseline(t_target, threads):
             os.close(saved)
         f.seek(0)
         steps = len(re.findall(r"^t = ", f.read(), re.M))
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as fc:
+            for line in fc:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return dict(value=NX * NY * steps / el, unit="cell-updates/s", cores=threads,
-                kind="reference",
+                kind="reference", host_cpus=os.cpu_count(), cpu_model=model,
                 sample=f"reference pde_solver (oracle/_ref, -O3), {NX}x{NY} reactive channel, "
                        f"order 3, stiff Newton-Krylov (its 6e-6 tolerance), {steps} timesteps "
-                       f"from t=0, {threads} threads, transmissive walls, {el:.1f} s")
+                       f"from t=0, {threads} threads (the host share of one GPU; "
+                       f"{os.cpu_count()} CPUs visible), transmissive walls, {el:.1f} s")
 
 
 def main():
