@@ -16,3 +16,4 @@ void sys_S(double *out, double *Q) { S(out, Q); }
 void sys_M(double *out, double *Q) { M(out, Q); }
 #endif
 double sys_exp(double x) { return pde_exp(x); }
+double sys_cbrt(double x) { return pde_cbrt(x); }

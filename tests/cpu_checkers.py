@@ -137,6 +137,8 @@ class System:
         self.F, self.B, self.S, self.M = sym("sys_F"), sym("sys_B"), sym("sys_S"), sym("sys_M")
         self.lib.sys_exp.restype = C.c_double
         self.lib.sys_exp.argtypes = [C.c_double]
+        self.lib.sys_cbrt.restype = C.c_double
+        self.lib.sys_cbrt.argtypes = [C.c_double]
 
 
 def system(name):

@@ -185,6 +185,26 @@ def test_exp_within_one_ulp():
         assert abs(got - want) <= np.spacing(want), x
 
 
+def test_cbrt_within_one_ulp():
+    """The system prelude's pde_cbrt (used by the GPR source term for
+    det(A)^(5/3)) against 50-digit decimal arithmetic over the double range,
+    plus its special values; it is the same text on host and device."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 50
+    rng = np.random.default_rng(4)
+    s = cc.system("gpr")
+    xs = np.concatenate([np.exp(rng.uniform(-700, 700, 3000)), rng.uniform(0.4, 4.1, 3000),
+                         [1.0, 8.0, 27.0, 0.125, 2.0 ** -1074, 1.7976931348623157e308]])
+    third = Decimal(1) / Decimal(3)
+    for x in xs:
+        got = s.lib.sys_cbrt(float(x))
+        want = Decimal(float(x)) ** third
+        assert abs(Decimal(got) - want) <= Decimal(np.spacing(got)), x
+    assert s.lib.sys_cbrt(0.0) == 0.0
+    assert np.isnan(s.lib.sys_cbrt(-1.0)) and np.isnan(s.lib.sys_cbrt(float("nan")))
+    assert s.lib.sys_cbrt(float("inf")) == float("inf")
+
+
 # ----------------------------------------------------------------- WENO
 
 def _weno(u, N):
