@@ -37,7 +37,8 @@ def standard_configs(full=False):
     combos = [("euler1d", 1, 3, False), ("euler2d", 2, 2, False), ("euler2d", 2, 3, False),
               ("euler2d", 2, 4, False), ("reactive2d", 2, 3, True)]
     if full:
-        combos += [("euler1d", 1, 2, False), ("advect", 1, 3, False), ("advect", 2, 3, False),
+        combos += [("euler1d", 1, 2, False), ("euler1d", 1, 1, False), ("euler1d", 1, 6, False),
+                   ("advect", 1, 3, False), ("advect", 2, 3, False),
                    ("ncadvect", 2, 3, False), ("reactive2d", 2, 3, False),
                    ("euler2d", 2, 3, True), ("ncadvect", 2, 3, True)]
     out = []

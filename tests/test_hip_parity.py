@@ -493,3 +493,31 @@ def test_gpr_17_variables_bitwise_equal_to_oracle(stiff):
                cfl=0.6, b_quirk=True, max_steps=30, stats=stats)
     assert stats["steps"] == st.steps == 30
     assert np.array_equal(ug, uo)
+
+
+@pytest.mark.gpu
+def test_edge_cases_bitwise_equal_to_oracle():
+    """Edge cases of the time loop (iterator.cpp:99-150), GPU against the
+    oracle, bitwise: tf = 0 (no step; only the last snapshot row is written),
+    ndt = 7 snapshot rows over 20 steps, one cell, orders 1 and 6, and a NaN
+    in the initial state (the step it is reported at, as a negative return).
+    Tolerance: bitwise, with equal step counts and return codes."""
+    u0 = systems.sod1d(64)
+    cases = [("tf0", u0, 0.0, 3, 3), ("ndt7", u0, 0.1, 3, 7), ("onecell", u0[:1], 0.05, 3, 1),
+             ("order1", u0, 0.05, 1, 1), ("order6", u0, 0.05, 6, 1)]
+    un = u0.copy()
+    un[10, 0] = np.nan
+    cases.append(("nan", un, 0.01, 3, 1))
+    for name, u, tf, N, ndt in cases:
+        uo, ro, st, rc = cc.oracle_solve("euler1d", u, tf, [1.0], 0, N, ndt=ndt)
+        ug, rg, sg = gpu_run("euler1d", u, tf, [1.0], "transitive", N, False, ndt=ndt)
+        assert sg["rc"] == rc and sg["steps"] == st.steps, name
+        assert np.array_equal(ug, uo, equal_nan=True), name
+        assert np.array_equal(rg, ro, equal_nan=True), name
+
+
+def test_zero_cells_rejected():
+    """A grid with no cells is an argument error, not a kernel launch."""
+    with pytest.raises(_capi.PypdeError, match="code 1"):
+        pde_solver(np.zeros((0, 3)), 0.1, [1.0], system=systems.source("euler1d"), order=3,
+                   ndt=1, stiff=False)
