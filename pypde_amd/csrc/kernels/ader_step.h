@@ -208,6 +208,66 @@ DEV void node_rhs(const NodeCtx& c, PredLDS& sm, const double (&q)[N][V], const 
   }
 }
 
+// node_rhs for an iterate that is constant in time (q(t) = w at every time
+// node: the first iteration from the plain guess).  F, B, S and the spatial
+// derivatives are then the same at every t, so they are evaluated once, at
+// t = 0, and reused; every value equals the one node_rhs computes.
+DEV void node_rhs_const(const NodeCtx& c, PredLDS& sm, const double (&q)[N][V], const double* w,
+                        double dt, const Geom& g, bool active, double (&rhs)[N][V],
+                        double (&dqs)[N][D][V]) {
+  if (active) {
+    static_for<0, D>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      lds_deriv<d>(c, sm.q, 0, 1.0 / g.dX[d], dqs[0][d]);
+    });
+#if PDE_USE_F
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      double f[V];
+      F(f, q[0], &dqs[0][0][0], d);
+#pragma unroll
+      for (int v = 0; v < V; v++) sm.f[d][c.slot][0][c.idx][v] = f[v];
+    }
+#endif
+  }
+#if PDE_USE_F
+  __syncthreads();
+#endif
+  if (active) {
+    double r[V];
+    callS(r, q[0]);
+    static_for<0, D>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+#if PDE_USE_B
+      double b[V * V], bd[V];
+      B(b, q[0], d);
+      b_times<PDE_B_QUIRK ? 1 : 0>(b, dqs[0][d], bd);
+#pragma unroll
+      for (int v = 0; v < V; v++) r[v] = r[v] - bd[v];
+#endif
+#if PDE_USE_F
+      double df[V];
+      lds_deriv<d>(c, sm.f[d], 0, 1.0 / g.dX[d], df);
+#pragma unroll
+      for (int v = 0; v < V; v++) r[v] = r[v] - df[v];
+#endif
+    });
+#pragma unroll
+    for (int t = 0; t < N; t++)
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        double rr = r[v] * c.ct[t];
+        rhs[t][v] = rr * dt + c.c0[t] * w[v];
+      }
+#pragma unroll
+    for (int t = 1; t < N; t++)
+#pragma unroll
+      for (int d = 0; d < D; d++)
+#pragma unroll
+        for (int v = 0; v < V; v++) dqs[t][d][v] = dqs[0][d][v];
+  }
+}
+
 #if PDE_STIFF
 constexpr double NEWTON_A_TINY = 1e-6;  // max |dt dS/dQ| below which the step drops A
 // Newton correction at one spatial node (see the CPU restatement for the
@@ -704,6 +764,7 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
   int learned = (PDE_GUESS && guess) ? 1 : 0;
   int fellback = 0;
   double r0 = 0.0;
+  int trip = 0;
   while (__ballot(active) != 0ull) {
     if (active) {
 #pragma unroll
@@ -712,7 +773,12 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
         for (int v = 0; v < V; v++) sm.q[c.slot][t][c.idx][v] = q[t][v];
     }
     __syncthreads();
-    node_rhs(c, sm, q, w, dt, g, active, rhs, dqs);
+    // first trip from the plain guess: q is constant in time (wave-uniform)
+    if (trip == 0 && !(PDE_GUESS && guess))
+      node_rhs_const(c, sm, q, w, dt, g, active, rhs, dqs);
+    else
+      node_rhs(c, sm, q, w, dt, g, active, rhs, dqs);
+    trip++;
     bool more = false;
     bool skip = false;
 #if PDE_STIFF
