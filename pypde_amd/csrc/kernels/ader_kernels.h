@@ -109,17 +109,20 @@ k_weno(const double* __restrict__ u, double* __restrict__ wh, const StepState* _
       }
     }
   } else {
+    // one variable per workgroup (blockIdx.z): 10.6 KB of LDS at N = 3, so
+    // many workgroups per CU hide the stencil's latency
     constexpr int TX = WENO_TX, TY = WENO_TY;
     constexpr int RX = TX + H, RY = TY + H, RYP = RY + 1;
-    __shared__ double s0[V][RX][RYP];
-    __shared__ double s1[V][N][TX][RYP];
+    __shared__ double s0[RX][RYP];
+    __shared__ double s1[N][TX][RYP];
+    const int v = blockIdx.z;
     const int i0 = blockIdx.y * TX, j0 = blockIdx.x * TY;
     const int nb0 = ghost ? g.nb[0] : nin0, nb1 = ghost ? g.nb[1] : nin1;
     const int no0 = nb0 - H, no1 = nb1 - H;
     const long in_stride = ghost ? g.ncell : (long)nin0 * nin1;
     const int in_row = ghost ? g.nx[1] : nin1;
-    for (int e = tid; e < V * RX * RY; e += 256) {
-      int v = e / (RX * RY), r = (e / RY) % RX, c = e % RY;
+    for (int e = tid; e < RX * RY; e += 256) {
+      int r = e / RY, c = e % RY;
       int I = i0 + r, J = j0 + c;
       double val = 0.0;
       if (I < nb0 && J < nb1) {
@@ -131,25 +134,25 @@ k_weno(const double* __restrict__ u, double* __restrict__ wh, const StepState* _
         if (f1) sg = sg * g.refl[1][v];
         val = u[(long)v * in_stride + (long)s0i * in_row + s1i] * sg;
       }
-      s0[v][r][c] = val;
+      s0[r][c] = val;
     }
     __syncthreads();
-    for (int e = tid; e < V * TX * RY; e += 256) {
-      int v = e / (TX * RY), r = (e / RY) % TX, c = e % RY;
+    for (int e = tid; e < TX * RY; e += 256) {
+      int r = e / RY, c = e % RY;
       if (i0 + r < no0) {
         double out[N];
-        weno_line([&](int m) { return s0[v][r + m][c]; }, out);
+        weno_line([&](int m) { return s0[r + m][c]; }, out);
 #pragma unroll
-        for (int k = 0; k < N; k++) s1[v][k][r][c] = out[k];
+        for (int k = 0; k < N; k++) s1[k][r][c] = out[k];
       }
     }
     __syncthreads();
-    for (int e = tid; e < V * N * TX * TY; e += 256) {
-      int c = e % TY, r = (e / TY) % TX, k0 = (e / (TY * TX)) % N, v = e / (TY * TX * N);
+    for (int e = tid; e < N * TX * TY; e += 256) {
+      int c = e % TY, r = (e / TY) % TX, k0 = e / (TY * TX);
       int i = i0 + r, j = j0 + c;
       if (i < no0 && j < no1) {
         double out[N];
-        weno_line([&](int m) { return s1[v][k0][r][c + m]; }, out);
+        weno_line([&](int m) { return s1[k0][r][c + m]; }, out);
         long cell = (long)i * no1 + j;
 #pragma unroll
         for (int k1 = 0; k1 < N; k1++) wh[(long)((k0 * N + k1) * V + v) * out_stride + cell] = out[k1];

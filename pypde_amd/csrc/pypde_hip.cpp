@@ -357,7 +357,8 @@ int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
     long os = g.nwc;
     std::vector<void*> a = {&uu, &w, &st, &g, &ghost, &z, &z, &os};
     dim3 grid = c->D == 1 ? dim3((unsigned)cdiv(g.nw[0], 256))
-                          : dim3((unsigned)cdiv(g.nw[1], 32), (unsigned)cdiv(g.nw[0], 8));
+                          : dim3((unsigned)cdiv(g.nw[1], 32), (unsigned)cdiv(g.nw[0], 8),
+                                 (unsigned)c->V);
     if ((rc = launch(c, K_WENO, grid, dim3(256), a))) return rc;
   }
   mark();
@@ -1259,7 +1260,8 @@ int weno_solver(double* ret, double* u, int* nX, int ndim, int N, int V) {
     void* a[] = {&dus, &dw, &st, &g, &ghost, &n0, &n1, &nout};
     unsigned gx = ndim == 1 ? (unsigned)cdiv(no[0], 256) : (unsigned)cdiv(no[1], 32);
     unsigned gy = ndim == 1 ? 1 : (unsigned)cdiv(no[0], 8);
-    HIPCHK(hipModuleLaunchKernel(fw, gx, gy, 1, 256, 1, 1, 0, nullptr, a, nullptr));
+    unsigned gz = ndim == 1 ? 1 : (unsigned)V;  // 2D: one variable per workgroup
+    HIPCHK(hipModuleLaunchKernel(fw, gx, gy, gz, 256, 1, 1, 0, nullptr, a, nullptr));
   }
   std::vector<double> h(nout * ND * V);
   HIPCHK(hipMemcpy(h.data(), dw, h.size() * sizeof(double), hipMemcpyDeviceToHost));
