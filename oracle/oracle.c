@@ -1443,9 +1443,11 @@ static int picard_cell(const Ctx *C, const double *w, double *q, double *rhs,
 }
 
 /* LU with partial pivoting of a V x V matrix, then solve for n right-hand sides
- * x_t = -P^-1 h_t. */
+ * x_t = -P^-1 h_t.  Each pivot is inverted once (rd[k] = 1 / u_kk) and the
+ * multipliers and the back substitution multiply by it. */
 static void lu_solve_neg(int V, double *P, int n, double h[MAXN][MAXV]) {
   int piv[MAXV];
+  double rd[MAXV];
   for (int k = 0; k < V; k++) {
     int p = k;
     for (int i = k + 1; i < V; i++)
@@ -1457,8 +1459,9 @@ static void lu_solve_neg(int V, double *P, int n, double h[MAXN][MAXV]) {
         P[k * V + j] = P[p * V + j];
         P[p * V + j] = t;
       }
+    rd[k] = 1.0 / P[k * V + k];
     for (int i = k + 1; i < V; i++) {
-      double l = P[i * V + k] / P[k * V + k];
+      double l = P[i * V + k] * rd[k];
       P[i * V + k] = l;
       for (int j = k + 1; j < V; j++) P[i * V + j] = fma(-l, P[k * V + j], P[i * V + j]);
     }
@@ -1481,7 +1484,7 @@ static void lu_solve_neg(int V, double *P, int n, double h[MAXN][MAXV]) {
     for (int i = V - 1; i >= 0; i--) {
       double acc = b[i];
       for (int j = i + 1; j < V; j++) acc = fma(-P[i * V + j], b[j], acc);
-      b[i] = -(acc / P[i * V + i]);
+      b[i] = -(acc * rd[i]);
     }
   }
 }

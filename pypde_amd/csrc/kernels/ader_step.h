@@ -387,8 +387,12 @@ DEV void newton_node(const NodeCtx& c, const double (&q)[N][V], double (&G)[N][V
       P[i * V + i] = P[i * V + i] + T_FLC[j];
     }
   }
-  // LU with partial pivoting (first largest pivot), static indices only
+  // LU with partial pivoting (first largest pivot), static indices only; each
+  // pivot inverted once (rd[k] = 1 / u_kk); the row swaps, done with selects,
+  // are skipped for the whole wavefront where no lane pivots
   int piv[V];
+  double rd[V];
+  bool swapped = false;
 #pragma unroll
   for (int k = 0; k < V; k++) {
     int p = k;
@@ -402,36 +406,43 @@ DEV void newton_node(const NodeCtx& c, const double (&q)[N][V], double (&G)[N][V
       }
     }
     piv[k] = p;
+    swapped = swapped || p != k;
+    if (__ballot(p != k) != 0ull) {
 #pragma unroll
-    for (int i = k + 1; i < V; i++) {
-      if (p == i) {
+      for (int i = k + 1; i < V; i++) {
+        if (p == i) {
 #pragma unroll
-        for (int j = 0; j < V; j++) {
-          double tmp = P[k * V + j];
-          P[k * V + j] = P[i * V + j];
-          P[i * V + j] = tmp;
+          for (int j = 0; j < V; j++) {
+            double tmp = P[k * V + j];
+            P[k * V + j] = P[i * V + j];
+            P[i * V + j] = tmp;
+          }
         }
       }
     }
+    rd[k] = 1.0 / P[k * V + k];
 #pragma unroll
     for (int i = k + 1; i < V; i++) {
-      double l = P[i * V + k] / P[k * V + k];
+      double l = P[i * V + k] * rd[k];
       P[i * V + k] = l;
 #pragma unroll
       for (int j = k + 1; j < V; j++) P[i * V + j] = dfma(-l, P[k * V + j], P[i * V + j]);
     }
   }
+  const bool any_swap = __ballot(swapped) != 0ull;
 #pragma unroll
   for (int t = 0; t < N; t++) {
     double* b = G[t];
+    if (any_swap) {
 #pragma unroll
-    for (int k = 0; k < V; k++) {
+      for (int k = 0; k < V; k++) {
 #pragma unroll
-      for (int i = k + 1; i < V; i++) {
-        if (piv[k] == i) {
-          double tmp = b[k];
-          b[k] = b[i];
-          b[i] = tmp;
+        for (int i = k + 1; i < V; i++) {
+          if (piv[k] == i) {
+            double tmp = b[k];
+            b[k] = b[i];
+            b[i] = tmp;
+          }
         }
       }
     }
@@ -447,7 +458,7 @@ DEV void newton_node(const NodeCtx& c, const double (&q)[N][V], double (&G)[N][V
       double acc = b[i];
 #pragma unroll
       for (int j = i + 1; j < V; j++) acc = dfma(-P[i * V + j], b[j], acc);
-      b[i] = -(acc / P[i * V + i]);
+      b[i] = -(acc * rd[i]);
     }
   }
 }
