@@ -16,8 +16,9 @@ channel at 256 x 64 cells and the same cell size).  `value` is the learned
 leg's rate where its final state stays within 1e-8 (relative L_inf) of the
 plain one and it is faster, else the plain leg's (`value_leg` says which and
 why); both legs' rates, Newton iteration counts and their difference in the
-final state are reported beside it, and an ensemble leg (48 small 2D Euler
-grids, one graph launch per timestep) after them.
+final state are reported beside it, then an ensemble leg (48 small 2D Euler
+grids, one graph launch per timestep) and a 512 x 512 periodic 2D Euler leg at
+order 4 (both skipped with --no-ensemble).
 
     python bench.py --calibrate-copy   # 3 x 256 MiB device copy, for HBM
                                        # counter calibration under rocprofv3
@@ -164,6 +165,22 @@ def ensemble_leg(steps):
                 sequential_value=done / seq, graph_speedup=seq / (t1 - t0))
 
 
+def euler_modes_leg(steps, warmup=3):
+    """2D periodic Euler, 512 x 512, order 4, plain FP64 (Picard predictor):
+    density, velocities from 8 seeded Fourier modes each (seed 0), p = 1."""
+    u0 = systems.euler2d_modes(512, seed=0)
+    src = systems.source("euler2d")
+    sol = Solver(u0, [1.0, 1.0], src, order=4, boundaryTypes="periodic")
+    sol.step(warmup)
+    t0 = time.perf_counter()
+    st = sol.step(steps)
+    t1 = time.perf_counter()
+    sol.close()
+    return dict(grid="512x512", order=4, steps=steps, value=512 * 512 * steps / (t1 - t0),
+                unit="cell-updates/s", ms_per_step=(t1 - t0) * 1e3 / steps,
+                iters_per_cell=st["pred_iters"] / max(1, st["pred_cells"]), finite=True)
+
+
 def rel_linf(a, b):
     """max over cells and variables of |a - b| / max(|b[var]|_inf, 1e-300)."""
     scale = np.maximum(np.max(np.abs(b.reshape(-1, V)), 0), 1e-300)
@@ -238,7 +255,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
-    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the ensemble and the 512 x 512 Euler legs")
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--train-steps", type=int, default=12)
     ap.add_argument("--train-pairs", type=int, default=2_000_000)
@@ -304,6 +322,10 @@ def main():
     if not args.no_ensemble:
         ens = ensemble_leg(20)
         log(f"ensemble: {ens}")
+    modes = None
+    if not args.no_ensemble:
+        modes = euler_modes_leg(20)
+        log(f"euler 512^2 order 4: {modes}")
     cpu = None
     if not args.no_cpu:
         # t after args.cpu_steps steps from t = 0, from a short GPU run
@@ -335,6 +357,7 @@ def main():
         "plain": plain,
         "learned": learned,
         "ensemble": ens,
+        "euler2d_512_order4": modes,
         "t_end": st_p["t"],
         "finite": bool(np.isfinite(u_p).all()),
     }
