@@ -306,12 +306,13 @@ def test_ensemble_members_equal_single_runs():
 
 @pytest.mark.gpu
 def test_predictor_schedules_agree():
-    """The static (one wavefront per cell group) and the persistent work-queued
-    predictor give the same bits and the same iteration statistics."""
+    """The static (one wavefront per cell group), its two-wavefront register
+    budget variant and the persistent work-queued predictor give the same bits
+    and the same iteration statistics."""
     u0 = reactive_small(64, 16, seed=2)
     L = [64 / 1024, 16 / 1024]
     outs = []
-    for sched in ("static", "queue"):
+    for sched in ("static", "static2", "queue"):
         os.environ["PYPDE_PRED_SCHEDULE"] = sched
         try:
             s = Solver(u0, L, systems.source("reactive2d"), order=3, boundaryTypes="reflective",
@@ -321,10 +322,11 @@ def test_predictor_schedules_agree():
         st = s.step(8)
         outs.append((s.state(), st))
         s.close()
-    (ua, sa), (ub, sb) = outs
-    assert np.array_equal(ua, ub)
-    for k in ("steps", "pred_iters", "pred_cells", "pred_maxit", "pred_capped"):
-        assert sa[k] == sb[k], k
+    ua, sa = outs[0]
+    for ub, sb in outs[1:]:
+        assert np.array_equal(ua, ub)
+        for k in ("steps", "pred_iters", "pred_cells", "pred_maxit", "pred_capped"):
+            assert sa[k] == sb[k], k
 
 
 @pytest.mark.gpu
