@@ -739,6 +739,11 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
   }
 }
 
+// The space-time DG predictor over all reconstructed cells (dg.cpp:204-224):
+// initial guess (dg.cpp:12-20, 156-171), Picard (dg.cpp:187-202) or the
+// per-node stiff Newton in place of the reference's Newton-Krylov
+// (dg.cpp:173-185), then the face traces (evaluations.cpp:49-70,
+// fv.cpp:102-122) and the cell-volume term (fv.cpp:33-85) for the corrector.
 // Static schedule: one wavefront per group of CPW consecutive cells, which
 // stays until its slowest cell has converged.
 DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
@@ -1022,6 +1027,10 @@ DEV void face_cells(const Geom& g, long f, long* wl, long* wr) {
   }
 }
 
+// Numerical flux at one face point along axis d (fv.cpp:124-198): Rusanov
+// (fluxes.cpp:72-101), Roe or Osher (fluxes.cpp:20-70), the viscous term
+// (fluxes.cpp:96-100) and the non-conservative path integral Bint
+// (fluxes.cpp:103-116), from the two cells' face traces.
 template <int d>
 DEV void flux_dir(const double* __restrict__ tr, double* __restrict__ ff, double* __restrict__ fb,
                   const StepState* st, const Geom& g, const double* __restrict__ trd) {
