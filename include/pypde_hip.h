@@ -137,11 +137,9 @@ int pypde_geometry(pypde_ctx *c, long *nwc, int *n_in, int *n_out);
 /* Diagnostics: the GPU wave-speed routines on n host matrices (V x V,
  * row-major, V of the context's system), for checking them against a CPU
  * restatement.  which 0: max |lambda| of each matrix (the eigenvalue step of
- * the reference's max_abs_eigs, system.cpp:28-43), out[n]; 1: the same by
- * the two-at-a-time routine of the wave-speed kernel on the pairs (A_2i,
- * A_2i+1), n even, out[n]; 2: |A_i| x_i (the |A| product of the Roe and
- * Osher fluxes, fluxes.cpp:20-70; a context with FLUX 1 or 2), x and out
- * n x V.  No counterpart in the reference's C API. */
+ * the reference's max_abs_eigs, system.cpp:28-43), out[n]; 2: |A_i| x_i (the
+ * |A| product of the Roe and Osher fluxes, fluxes.cpp:20-70; a context with
+ * FLUX 1 or 2), x and out n x V.  No counterpart in the reference's C API. */
 int pypde_eig_batch(pypde_ctx *c, int which, const double *A, const double *x, long n,
                     double *out);
 

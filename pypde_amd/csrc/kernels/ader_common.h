@@ -561,62 +561,6 @@ DEV double max_abs_eig(double* H) {
   return __builtin_ldexp(eig_qr(H), e2);
 }
 
-// max_abs_eig of two matrices at once (both destroyed): the same arithmetic
-// for each, with the four Laguerre iterations in one loop (used where both
-// matrices fit the register budget: the per-cell wave speeds of k_dt).
-DEV void max_abs_eig2(double* HA, double* HB, double* ra, double* rb) {
-  int ea = 0, eb = 0;
-  double xa = 0.0, xb = 0.0;
-  const bool pa = eig_prepare(HA, &ea, &xa);
-  const bool pb = eig_prepare(HB, &eb, &xb);
-  bool okA = false, okB = false;
-  double mA = 0.0, mB = 0.0;
-#if PDE_V >= 3
-  {
-    // a matrix answered by eig_prepare joins as the zero matrix, whose
-    // Laguerre iterations stop at once
-    if (!pa)
-#pragma unroll
-      for (int i = 0; i < V * V; i++) HA[i] = 0.0;
-    if (!pb)
-#pragma unroll
-      for (int i = 0; i < V * V; i++) HB[i] = 0.0;
-    double c[4][V + 1], x0[4], root[4], dp[4];
-    bool ok[4];
-    charpoly_setup(HA, c[0], &x0[0], &x0[1]);
-    charpoly_setup(HB, c[2], &x0[2], &x0[3]);
-#pragma unroll
-    for (int j = 0; j <= V; j++) {
-      c[1][j] = c[0][j];
-      c[3][j] = c[2][j];
-    }
-    laguerre_multi<4>(c, x0, root, dp, ok);
-    okA = pa && ok[0] && ok[1] && charpoly_certify(c[0], root[0], root[1], dp[0], dp[1], &mA);
-    okB = pb && ok[2] && ok[3] && charpoly_certify(c[2], root[2], root[3], dp[2], dp[3], &mB);
-  }
-#endif
-  const bool needA = pa && !okA, needB = pb && !okB;
-  double qa = 0.0, qb = 0.0;
-  if (needA || needB) {
-    double Hf[V * V];
-#pragma unroll 1
-    for (int pass = 0; pass < 2; pass++) {
-      const bool doit = pass == 0 ? needA : needB;
-      if (doit) {
-#pragma unroll
-        for (int i = 0; i < V * V; i++) Hf[i] = pass == 0 ? HA[i] : HB[i];
-        const double r = eig_qr(Hf);
-        if (pass == 0)
-          qa = r;
-        else
-          qb = r;
-      }
-    }
-  }
-  *ra = !pa ? xa : __builtin_ldexp(okA ? mA : qa, ea);
-  *rb = !pb ? xb : __builtin_ldexp(okB ? mB : qb, eb);
-}
-
 // ------------------------------------------------------ system functions
 DEV void callF(double* out, const double* q, const double* dq, int d) {
 #if PDE_USE_F
@@ -668,15 +612,6 @@ DEV double max_abs_eigs(const double* q, const double* dq, int d, double* f0) {
   double A[V * V];
   system_matrix(q, dq, d, A, f0);
   return max_abs_eig(A);
-}
-
-// max_abs_eigs for two states (or directions) at once.
-DEV void max_abs_eigs2(const double* qa, const double* dqa, int da, double* fa, const double* qb,
-                       const double* dqb, int db, double* fb, double* la, double* lb) {
-  double A[V * V], B[V * V];
-  system_matrix(qa, dqa, da, A, fa);
-  system_matrix(qb, dqb, db, B, fb);
-  max_abs_eig2(A, B, la, lb);
 }
 
 #ifndef PDE_SECOND

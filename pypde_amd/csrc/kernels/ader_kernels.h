@@ -210,17 +210,10 @@ k_dt(const double* __restrict__ wh, StepState* st, Geom g) {
 #pragma unroll
       for (int v = 0; v < V; v++) dq[d * V + v] = acc[v];
     });
-    double lams[D];
-    if constexpr (D == 2) {
-      double f0[V], f1[V];
-      max_abs_eigs2(q, dq, 0, f0, q, dq, 1, f1, &lams[0], &lams[1]);
-    } else {
-      double f0[V];
-      lams[0] = max_abs_eigs(q, dq, 0, f0);
-    }
     static_for<0, D>([&](auto dc) {
       constexpr int d = decltype(dc)::value;
-      double lam = lams[d];
+      double f0[V];
+      double lam = max_abs_eigs(q, dq, d, f0);
 #if PDE_SECOND
       lam = lam + max_abs_eigs_second_order<d>(q, dq, g.dX[d]);
 #endif
@@ -256,23 +249,12 @@ extern "C" __global__ void k_dt_finalize(StepState* st) {
 // ------------------------------------------------------- eigen diagnostics
 // The wave-speed and |A| routines on a batch of V x V matrices (row-major,
 // one per thread), for checking them against the CPU restatement:
-// which 0: max_abs_eig; 1: max_abs_eig2 on the pairs (A_2i, A_2i+1), out of
-// n entries; 2: abs_apply(A_i, x_i) (Roe and Osher modules), out of n V.
+// which 0: max_abs_eig, out of n entries; 2: abs_apply(A_i, x_i) (Roe and
+// Osher modules), out of n V.
 extern "C" __global__ void __launch_bounds__(256)
 k_eig_batch(const double* __restrict__ A, const double* __restrict__ x, double* __restrict__ out,
             long n, int which) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (which == 1) {
-    if (2 * i + 1 >= n) return;
-    double HA[V * V], HB[V * V];
-#pragma unroll
-    for (int k = 0; k < V * V; k++) {
-      HA[k] = A[(2 * i) * V * V + k];
-      HB[k] = A[(2 * i + 1) * V * V + k];
-    }
-    max_abs_eig2(HA, HB, &out[2 * i], &out[2 * i + 1]);
-    return;
-  }
   if (i >= n) return;
   double H[V * V];
 #pragma unroll

@@ -987,11 +987,10 @@ int pypde_set_guess_model(pypde_ctx* c, const pypde_mlp* m) {
 
 int pypde_eig_batch(pypde_ctx* c, int which, const double* A, const double* x, long n,
                     double* out) {
-  if (!c || !A || !out || n < 0 || which < 0 || which > 2)
+  if (!c || !A || !out || n < 0 || (which != 0 && which != 2))
     return fail(PYPDE_ERR_ARG, "pypde_eig_batch: bad arguments");
   if (which == 2 && (c->spec.flux == 0 || !x))
     return fail(PYPDE_ERR_ARG, "pypde_eig_batch: abs_apply needs a Roe or Osher context and x");
-  if (which == 1 && (n & 1)) return fail(PYPDE_ERR_ARG, "pypde_eig_batch: pairs need an even n");
   if (n == 0) return 0;
   HIPCHK(hipSetDevice(c->device));
   const long V = c->V, nout = which == 2 ? n * V : n;
@@ -1015,8 +1014,7 @@ int pypde_eig_batch(pypde_ctx* c, int which, const double* A, const double* x, l
   const double* a0 = dA;
   const double* x0 = dx;
   std::vector<void*> a = {&a0, &x0, &dout, &n, &which};
-  const long threads = which == 1 ? n / 2 : n;
-  if ((rc = launch(c, K_EIG, dim3((unsigned)cdiv(threads, 256)), dim3(256), a))) return done(rc);
+  if ((rc = launch(c, K_EIG, dim3((unsigned)cdiv(n, 256)), dim3(256), a))) return done(rc);
   if (hipMemcpyAsync(out, dout, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream) !=
           hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
