@@ -128,6 +128,29 @@ def test_euler2d_matches_oracle(N, bc):
     assert relerr(ug, uo) <= TOL
 
 
+EULER2D_REFLECT = np.array([[1, -1, 1, 1], [1, 1, -1, 1]], dtype=np.float64)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bcs,N", [((1, 2), 3), ((0, 1), 2), ((2, 0), 4)])
+def test_mixed_boundaries_bitwise_equal_to_oracle(bcs, N):
+    """One boundary type per axis (boundaries.cpp:7-53 takes them per axis;
+    reflective is this build's addition): 2D Euler on a 40 x 24 window,
+    reflective walls flip the wall-normal momentum.  Tolerance: bitwise, with
+    equal step and Picard iteration counts."""
+    u0 = systems.euler2d_modes(40, seed=5)[:, :24].copy()
+    L = [1.0, 24 / 40]
+    refl = EULER2D_REFLECT if 2 in bcs else None
+    uo, ro, st, rc = oracle_run("euler2d", u0, 0.04, L, list(bcs), N, False,
+                                reflect=None if refl is None else refl.ravel())
+    ug, rg, sg = gpu_run("euler2d", u0, 0.04, L, [BCNAME[b] for b in bcs], N, False,
+                         reflect=refl)
+    print(f"mixed bcs {bcs} N={N}: steps {sg['steps']} oracle {st.steps}")
+    assert rc == 0 and sg["rc"] == 0
+    assert sg["steps"] == st.steps and sg["pred_iters"] == st.pred_iters
+    assert np.array_equal(ug, uo)
+
+
 def reactive_small(nx=64, ny=16, seed=0):
     return systems.reactive_channel(nx, ny, seed=seed)
 
