@@ -340,12 +340,14 @@ long predictor_blocks(pypde_ctx* c) {
 int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
   Geom g = c->g;
   int rc;
+  hipError_t mark_err = hipSuccess;  // first failure of a profiling event
   auto mark = [&]() {
-    if (evs) {
+    if (evs && mark_err == hipSuccess) {
       hipEvent_t e;
-      hipEventCreate(&e);
-      hipEventRecord(e, c->stream);
+      mark_err = hipEventCreate(&e);
+      if (mark_err != hipSuccess) return;
       evs->push_back(e);
+      mark_err = hipEventRecord(e, c->stream);
     }
   };
   mark();
@@ -428,6 +430,8 @@ int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
     if ((rc = launch(c, K_ADVANCE, dim3(1), dim3(64), a))) return rc;
   }
   mark();
+  if (mark_err != hipSuccess)
+    return fail(PYPDE_ERR_HIP, "profiling event: %s", hipGetErrorString(mark_err));
   return 0;
 }
 
@@ -504,19 +508,20 @@ int pypde_hip_compile(const char* src, bool useF, bool useB, bool useS, int ndim
   return 0;
 }
 
+// Teardown has no status to return: failures of the releases are ignored.
 void pypde_destroy(pypde_ctx* c) {
   if (!c) return;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (double* p : {c->u, c->wh, c->tr, c->vol, c->ff[0], c->ff[1], c->fb[0], c->fb[1], c->ret,
                     c->guess, c->tmp, c->trd})
-    if (p) hipFree(p);
-  if (c->st) hipFree(c->st);
-  if (c->gbuf) hipFree(c->gbuf);
-  if (c->qh) hipFree(c->qh);
-  if (c->mod) hipModuleUnload(c->mod);
-  if (c->mod_guess) hipModuleUnload(c->mod_guess);
-  if (c->stream) hipStreamDestroy(c->stream);
+    if (p) (void)hipFree(p);
+  if (c->st) (void)hipFree(c->st);
+  if (c->gbuf) (void)hipFree(c->gbuf);
+  if (c->qh) (void)hipFree(c->qh);
+  if (c->mod) (void)hipModuleUnload(c->mod);
+  if (c->mod_guess) (void)hipModuleUnload(c->mod_guess);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
@@ -687,21 +692,25 @@ static int capture_steps(const std::vector<pypde_ctx*>& cs, hipGraphExec_t* exec
   int rc = 0;
   hipGraph_t graph = nullptr;
   HIPCHK(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
-  hipEventRecord(evs[0], s0);
+  // fork and join: an error here is reported, and the capture still ends
+  auto chk = [&](hipError_t e, const char* what) {
+    if (!rc && e != hipSuccess) rc = fail(PYPDE_ERR_HIP, "graph capture, %s: %s", what, hipGetErrorString(e));
+  };
+  chk(hipEventRecord(evs[0], s0), "fork event");
   for (size_t i = 0; i < cs.size() && !rc; i++) {
-    if (i > 0) hipStreamWaitEvent(cs[i]->stream, evs[0], 0);
-    rc = enqueue_step(cs[i], false, nullptr);
+    if (i > 0) chk(hipStreamWaitEvent(cs[i]->stream, evs[0], 0), "fork wait");
+    if (!rc) rc = enqueue_step(cs[i], false, nullptr);
     if (!rc && i > 0) {
-      hipEventRecord(evs[i + 1], cs[i]->stream);
-      hipStreamWaitEvent(s0, evs[i + 1], 0);
+      chk(hipEventRecord(evs[i + 1], cs[i]->stream), "join event");
+      chk(hipStreamWaitEvent(s0, evs[i + 1], 0), "join wait");
     }
   }
   hipError_t e = hipStreamEndCapture(s0, &graph);
   if (!rc && e != hipSuccess) rc = fail(PYPDE_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
   if (!rc && hipGraphInstantiate(exec, graph, nullptr, nullptr, 0) != hipSuccess)
     rc = fail(PYPDE_ERR_HIP, "graph instantiation failed");
-  if (graph) hipGraphDestroy(graph);
-  for (auto ev : evs) hipEventDestroy(ev);
+  if (graph) (void)hipGraphDestroy(graph);
+  for (auto ev : evs) (void)hipEventDestroy(ev);
   return rc;
 }
 
@@ -735,18 +744,21 @@ static int run_loop(pypde_ctx* c, long max_steps, pypde_stats* stats, bool first
     if (c->host_st.done) break;
     if (chunk < 64) chunk *= 2;
   }
-  if (exec) hipGraphExecDestroy(exec);
-  if (rc) return rc;
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (rc) {
+    for (auto e : evs) (void)hipEventDestroy(e);
+    return rc;
+  }
   auto t1 = std::chrono::steady_clock::now();
   if (c->opts.profile) {
     // 9 marks per step bracket: weno | dt | guess | predictor | flux0 | flux1 | update | advance
     for (size_t s0 = 0; s0 + 9 <= evs.size(); s0 += 9)
       for (int k = 0; k < 8; k++) {
         float ms = 0;
-        hipEventElapsedTime(&ms, evs[s0 + k], evs[s0 + k + 1]);
-        c->ms_kernel[k] += ms;
+        if (hipEventElapsedTime(&ms, evs[s0 + k], evs[s0 + k + 1]) == hipSuccess)
+          c->ms_kernel[k] += ms;
       }
-    for (auto e : evs) hipEventDestroy(e);
+    for (auto e : evs) (void)hipEventDestroy(e);
   }
   if (stats) {
     stats->steps = c->host_st.count;
@@ -769,7 +781,7 @@ int pypde_run(pypde_ctx* c, double tf, long max_steps, int ndt, double* ret, pyp
   size_t rows = (size_t)ndt;
   size_t row = c->g.ncell * c->V;
   if (rows > c->ret_rows) {
-    if (c->ret) hipFree(c->ret);
+    if (c->ret) (void)hipFree(c->ret);
     c->ret = nullptr;
     HIPCHK(hipMalloc((void**)&c->ret, rows * row * sizeof(double)));
     c->ret_rows = rows;
@@ -913,7 +925,7 @@ int pypde_ensemble_run(pypde_ensemble* e, double tf, long max_steps, pypde_stats
     if (all) break;
     if (chunk < 64) chunk *= 2;
   }
-  if (exec) hipGraphExecDestroy(exec);
+  if (exec) (void)hipGraphExecDestroy(exec);
   auto t1 = std::chrono::steady_clock::now();
   if (rc) return rc;
   if (ms_loop) *ms_loop = std::chrono::duration<double, std::milli>(t1 - t0).count();
@@ -997,9 +1009,9 @@ int pypde_eig_batch(pypde_ctx* c, int which, const double* A, const double* x, l
   double *dA = nullptr, *dx = nullptr, *dout = nullptr;
   int rc = 0;
   auto done = [&](int r) {
-    if (dA) hipFree(dA);
-    if (dx) hipFree(dx);
-    if (dout) hipFree(dout);
+    if (dA) (void)hipFree(dA);
+    if (dx) (void)hipFree(dx);
+    if (dout) (void)hipFree(dout);
     return r;
   };
   if (hipMalloc((void**)&dA, n * V * V * sizeof(double)) != hipSuccess ||
@@ -1035,7 +1047,7 @@ int pypde_harvest_step(pypde_ctx* c, float* X, float* Y, long ldx, long col0, py
   rc = launch(c, K_HARVEST, dim3((unsigned)cdiv(nwc, 256)), dim3(256), a);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
-  hipFree(c->qh);
+  (void)hipFree(c->qh);
   c->qh = nullptr;
   return 0;
 }
@@ -1234,14 +1246,24 @@ int weno_solver(double* ret, double* u, int* nX, int ndim, int N, int V) {
   std::vector<char> code;
   if ((rc = compile_code(s, &code))) return rc;
   HIPCHK(hipSetDevice(0));
-  hipModule_t mod;
-  HIPCHK(hipModuleLoadData(&mod, code.data()));
+  // the module and device buffers are released on every return path
+  struct Release {
+    hipModule_t mod = nullptr;
+    double *du = nullptr, *dus = nullptr, *dw = nullptr;
+    ~Release() {
+      for (double* p : {du, dus, dw})
+        if (p) (void)hipFree(p);
+      if (mod) (void)hipModuleUnload(mod);
+    }
+  } rel;
+  HIPCHK(hipModuleLoadData(&rel.mod, code.data()));
+  hipModule_t mod = rel.mod;
   hipFunction_t fw, fa;
   HIPCHK(hipModuleGetFunction(&fw, mod, "k_weno"));
   HIPCHK(hipModuleGetFunction(&fa, mod, "k_aos_to_soa"));
   int ND = 1;
   for (int d = 0; d < ndim; d++) ND *= N;
-  double *du = nullptr, *dus = nullptr, *dw = nullptr;
+  double *&du = rel.du, *&dus = rel.dus, *&dw = rel.dw;
   HIPCHK(hipMalloc((void**)&du, nin * V * sizeof(double)));
   HIPCHK(hipMalloc((void**)&dus, nin * V * sizeof(double)));
   HIPCHK(hipMalloc((void**)&dw, nout * ND * V * sizeof(double)));
@@ -1266,10 +1288,6 @@ int weno_solver(double* ret, double* u, int* nX, int ndim, int N, int V) {
   // planes [node*V+v][cell] -> reference layout [cell][node][v]
   for (long cc = 0; cc < nout; cc++)
     for (int p = 0; p < ND * V; p++) ret[cc * ND * V + p] = h[(size_t)p * nout + cc];
-  hipFree(du);
-  hipFree(dus);
-  hipFree(dw);
-  hipModuleUnload(mod);
   return 0;
 }
 
