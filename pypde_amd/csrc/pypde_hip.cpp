@@ -160,7 +160,11 @@ int compile_code(const Spec& spec, std::vector<char>* code) {
   hiprtcVersion(&ver, &ver);
   key += "\nhiprtc" + std::to_string(ver);
   uint64_t h = fnv1a(key);
-  {
+  // PYPDE_HIP_RECOMPILE=1: compile even when a cached code object exists (the
+  // build's compile check); the result still replaces the cache entry
+  const char* re = getenv("PYPDE_HIP_RECOMPILE");
+  const bool recompile = re && *re && *re != '0';
+  if (!recompile) {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_codes.find(h);
     if (it != g_codes.end()) {
@@ -172,7 +176,7 @@ int compile_code(const Spec& spec, std::vector<char>* code) {
   snprintf(name, sizeof name, "%016llx", (unsigned long long)h);
   std::string dir = cache_dir();
   std::string path = dir + "/" + name + ".hsaco";
-  {
+  if (!recompile) {
     std::ifstream f(path, std::ios::binary);
     if (f) {
       std::vector<char> c((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
