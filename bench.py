@@ -48,39 +48,77 @@ NX, NY, ORDER, V, D = 1024, 256, 3, 5, 2
 L = [1.0, NY / NX]
 PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
-# HBM bytes per predictor launch on this workload, from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE runs scaled by a 256 MiB copy kernel's readings
-# (profiles/reactive_channel_1024x256_kernels.md)
-PREDICTOR_HBM_BYTES = 863.3e6
+PROFILE = os.path.join(ROOT, "profiles", "reactive_channel_1024x256_kernels.md")
 
 # Flop counts of the reactive system functions (pypde_amd/systems/reactive2d.pde),
-# counted by hand: + - * / sqrt each 1, fma 2; pde_exp = 33.
-FLOPS_F, FLOPS_S, FLOPS_M = 17, 46, 78
+# counted by hand: + - * / each 1, fma 2; pde_exp = 32 (two fma reductions, a
+# 13-term fma Horner chain, one multiply).
+FLOPS_F, FLOPS_S, FLOPS_M = 18, 47, 79
 
 
-def predictor_flops_per_iteration(N=ORDER, D=D, V=V, cF=FLOPS_F, cS=FLOPS_S, cM=FLOPS_M):
-    """Algorithmic FP64 flops of one stiff-predictor iteration of one cell
-    (one space-time residual plus the per-node Newton correction)."""
+def predictor_flop_model(N=ORDER, D=D, V=V, cF=FLOPS_F, cS=FLOPS_S, cM=FLOPS_M):
+    """FP64 operations of the stiff predictor kernel (k_predictor_static) per
+    unit of work, counted from its source (+ - * / 1, fma 2; comparisons and
+    data movement 0).  Units: one lane (one spatial node of one cell) for one
+    iteration, its Newton correction by path, and one cell's epilogue."""
     ND = N ** D
-    deriv = 2 * N                                    # N-term dot product + divide
-    f = N * ND * (D * V * deriv                      # grad q
-                  + D * cF + D * V * deriv           # F and div F
-                  + cS + V * (D + 4)                 # S, assembly, *W, *dt, +Ww
-                  + V * (2 * N + 1))                 # residual G = rhs - K q
-    node = (N * V + 2 * N * V + cM + V * V           # scaling, mean state, dS/dQ, A
-            + N * (N * V * 2 * N + (N - 1) * (V * 2 * V + V))  # h_t (Horner)
-            + (N - 1) * (V * V * 2 * V + V)          # P = p(A)
-            + sum((V - k - 1) * (1 + 2 * (V - k - 1)) for k in range(V))  # LU
-            + N * (2 * V * V)                        # triangular solves
-            + 2 * N * V)                             # update + test
-    return f + ND * node
+    deriv = 2 * N                                          # N-term dot product, times 1/dX
+    rhs = N * (2 * D * V * deriv + D * cF + cS + D * V + 4 * V)   # node_rhs, all N time nodes
+    rhs_const = 2 * D * V * deriv + D * cF + cS + D * V + 4 * N * V  # first trip, q const in t
+    resid = N * V * (2 + 3 * (N - 1) + 1)                  # G = rhs - (DG_MAT x W) q
+    upd = 3 * N * V                                        # q += dq; |dq| <= tol (1 + |q|)
+    common = N * V + V * (1 + 2 * (N - 1)) + cM + V * V    # g, qbar, M(qbar), dt A
+    short = common + N * V * (2 * N)                       # A = 0: dq = -(M_N g) / c_0
+    horner = N * (N * V * (1 + 2 * (N - 1)) + (N - 1) * (V * (1 + 2 * (V - 1)) + V))
+    poly = V + (N - 1) * V * (V * (1 + 2 * (V - 1)) + 1)
+    lu = sum(1 + (V - k - 1) * (1 + 2 * (V - k - 1)) for k in range(V))
+    tri = N * (sum(2 * i for i in range(1, V)) + sum(2 * (V - 1 - i) + 1 for i in range(V)))
+    full = common + horner + poly + lu + tri
+    traces = D * 2 * V * (1 + 2 * (N - 1))                 # per lane
+    volume = N * (cS + (D + 1) + 2 * V) - V                # per lane
+    epilogue = ND * (traces + volume) + V * (ND - 1)       # per cell
+    return dict(rhs=rhs, rhs_const=rhs_const, resid=resid, upd=upd, short=short, full=full,
+                epilogue=epilogue, ND=ND)
 
 
-def predictor_flops_epilogue(N=ORDER, D=D, V=V, cS=FLOPS_S):
-    ND = N ** D
-    traces = 2 * D * N * N ** (D - 1) * V * (2 * N - 1)
-    volume = N * ND * (cS + (D + 1) + 2 * V) + ND * V
-    return traces + volume
+def predictor_flops(win, guess=False):
+    """Algorithmic FP64 flops the predictor performed over a window of steps,
+    from the device counters of that window: cell iterations, cells, and node
+    corrections by path (full source-Jacobian solve or the A = 0 shortcut).
+    Every cell's first iteration from the plain guess is the time-constant one."""
+    m = predictor_flop_model()
+    lane_its = m["ND"] * win["pred_iters"]
+    const_lanes = 0 if guess else m["ND"] * win["pred_cells"]
+    return ((lane_its - const_lanes) * m["rhs"] + const_lanes * m["rhs_const"]
+            + lane_its * (m["resid"] + m["upd"])
+            + win["pred_node_full"] * m["full"] + win["pred_node_short"] * m["short"]
+            + win["pred_cells"] * m["epilogue"])
+
+
+def survey_predictor_flops(win, N=ORDER, D=D, V=V, cF=FLOPS_F, cS=FLOPS_S):
+    """The cost model stated for this workload before the kernels existed:
+    2 n^(D+2) V (D (1 + [F]) + 1) + (2D + 3) n^(D+1) V structured flops per cell
+    iteration, plus user functions and a dense n^D (2/3)(nV)^3 block LU that the
+    per-node polynomial solve here never performs.  Reported beside the
+    kernel's own count, never used for frac."""
+    n = N
+    per_it = (2 * n ** (D + 2) * V * (D * 2 + 1) + (2 * D + 3) * n ** (D + 1) * V
+              + n ** (D + 1) * (D * cF + cS) + n ** D * (2.0 / 3.0) * (n * V) ** 3)
+    return per_it * win["pred_iters"]
+
+
+def profile_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel's readings), or None."""
+    try:
+        with open(PROFILE) as f:
+            for line in f:
+                cells = [x.strip() for x in line.strip().strip("|").split("|")]
+                if len(cells) == 6 and cells[0] == kernel:
+                    return float(cells[5]) * 1e6
+    except (OSError, ValueError):
+        return None
+    return None
 
 
 def make_solver(u0, L_, profile=False):
@@ -187,17 +225,58 @@ def rel_linf(a, b):
     return float(np.max(np.abs((a - b).reshape(-1, V)) / scale))
 
 
-def predictor_roofline(st, K):
-    ms = st["ms_kernel"]
-    pred_ms = ms["predictor"]
-    flops = (predictor_flops_per_iteration() * st["pred_iters"]
-             + predictor_flops_epilogue() * st["pred_cells"])
+def window(st, w):
+    """Counters of the timed steps only: st after them minus w before them."""
+    keys = ("steps", "pred_iters", "pred_cells", "pred_capped", "guess_fallbacks",
+            "pred_node_full", "pred_node_short")
+    return {k: st[k] - (w[k] if w else 0) for k in keys}
+
+
+def predictor_roofline(st_k, win, K, guess=False):
+    """Roofline of the dominant kernel, the predictor: its algorithmic FP64
+    flops over the timed steps (predictor_flops) / its HIP-event time over the
+    same steps, against MI355X's FP64 peak.  It runs on the vector ALUs, whose
+    FP64 peak equals the FP64 matrix-core peak (78.6 TFLOP/s); 'mfma' is the
+    contract's word for the compute bound."""
+    pred_ms = st_k["ms_kernel"]["predictor"]
+    flops = predictor_flops(win, guess)
     achieved = flops / (pred_ms * 1e-3) if pred_ms > 0 else 0.0
-    # compute-bound ("mfma" in the roofline vocabulary): FP64 on the vector ALUs,
-    # whose MI355X peak equals the FP64 matrix-core peak, 78.6 TFLOP/s
+    traffic = profile_traffic("k_predictor_static")
     return dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK_FP64 / 1e12, unit="TFLOP/s",
-                frac=achieved / PEAK_FP64, traffic=PREDICTOR_HBM_BYTES,
-                kernel="k_predictor_static", pipe="fp64 valu", avg_launch_ms=pred_ms / K)
+                frac=achieved / PEAK_FP64, traffic=traffic,
+                traffic_source=("profiles/reactive_channel_1024x256_kernels.md (rocprofv3 "
+                                "FETCH_SIZE + WRITE_SIZE per launch, copy-calibrated)")
+                if traffic else None,
+                kernel="k_predictor_static", pipe="fp64 vector ALU (no MFMA instructions)",
+                avg_launch_ms=pred_ms / K, flops_per_launch=flops / K,
+                survey_model_flops_per_launch=survey_predictor_flops(win) / K,
+                node_corrections_full=win["pred_node_full"],
+                node_corrections_shortcut=win["pred_node_short"])
+
+
+def stencil_rooflines(st_k, K, N=ORDER, D=D, V=V, nx=NX, ny=NY):
+    """HBM rooflines of the memory-facing kernels over the timed steps, each
+    with its algorithmic bytes per launch: every input read once, every
+    output written once (w-cells include the ghost layer)."""
+    ms = {k: v / K for k, v in st_k["ms_kernel"].items()}
+    ncell, nwc = nx * ny, (nx + 2) * (ny + 2)
+    ND = N ** D
+    nf = [(nx + 1) * ny, nx * (ny + 1)]
+    faces = sum(nf) * N * N ** (D - 1)             # face points per step
+    b_weno = 8 * (V * ncell + ND * V * nwc)           # u in, wh out
+    b_update = 8 * (2 * V * ncell + V * nwc + faces * V)  # u in/out, vol, face fluxes
+    b_flux = 8 * faces * 3 * V                        # two traces in, one flux out
+    b_fv = 8 * (N ** (D + 1) * V * nwc + 2 * V * ncell)   # the FV stage's floor
+    out = {}
+    for name, b, t in (("k_weno", b_weno, ms["weno"]), ("k_update", b_update, ms["update"]),
+                       ("k_flux0+k_flux1", b_flux, ms["flux0"] + ms["flux1"]),
+                       ("fv_stage", b_fv, ms["flux0"] + ms["flux1"] + ms["update"])):
+        bw = b / (t * 1e-3) if t > 0 else 0.0
+        out[name] = dict(bound="hbm", achieved=bw / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s",
+                         frac=bw / PEAK_HBM, bytes_per_launch=b, avg_launch_ms=t,
+                         traffic=profile_traffic(name) if name.startswith("k_") and "+" not in name
+                         else None)
+    return out
 
 
 def cpu_baseline(t_target, threads):
@@ -284,28 +363,33 @@ def main():
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     # timed run without per-kernel events; a second, profiled run of the same
     # steps gives the kernel breakdown and the roofline
-    st_p, el_p, _, u_p = gpu_run(K, W, profile=False)
-    st_k, el_k, _, _ = gpu_run(K, W, profile=True)
+    st_p, el_p, w_p, u_p = gpu_run(K, W, profile=False)
+    st_k, el_k, w_k, _ = gpu_run(K, W, profile=True)
+    win_p, win_k = window(st_p, w_p), window(st_k, w_k)
     log(f"plain: {el_p * 1e3 / K:.3f} ms/step ({el_k * 1e3 / K:.3f} with kernel events)")
     plain = dict(value=cells * K / el_p, ms_per_step=el_p * 1e3 / K,
-                 iters_per_cell=st_p["pred_iters"] / max(1, st_p["pred_cells"]),
-                 max_iters=st_p["pred_maxit"], capped=st_p["pred_capped"],
+                 iters_per_cell=win_p["pred_iters"] / max(1, win_p["pred_cells"]),
+                 max_iters_run=st_p["pred_maxit"], capped=win_p["pred_capped"],
+                 node_shortcut_frac=win_p["pred_node_short"]
+                 / max(1, win_p["pred_node_short"] + win_p["pred_node_full"]),
                  kernel_ms_per_step={k: v / K for k, v in st_k["ms_kernel"].items()},
-                 roofline=predictor_roofline(st_k, K))
+                 roofline=predictor_roofline(st_k, win_k, K),
+                 stencil_rooflines=stencil_rooflines(st_k, K))
     learned = None
     if not args.no_learned:
         net, info = train_guess_model(args.train_steps, args.train_pairs, args.epochs, log)
         log(f"trained: {info}")
-        st_l, el_l, _, u_l = gpu_run(K, W, profile=False, net=net)
-        st_lk, _, _, _ = gpu_run(K, W, profile=True, net=net)
+        st_l, el_l, w_l, u_l = gpu_run(K, W, profile=False, net=net)
+        st_lk, _, w_lk, _ = gpu_run(K, W, profile=True, net=net)
+        win_l, win_lk = window(st_l, w_l), window(st_lk, w_lk)
         log(f"learned: {el_l * 1e3 / K:.3f} ms/step")
         learned = dict(value=cells * K / el_l, ms_per_step=el_l * 1e3 / K,
-                       iters_per_cell=st_l["pred_iters"] / max(1, st_l["pred_cells"]),
-                       max_iters=st_l["pred_maxit"], capped=st_l["pred_capped"],
-                       guess_fallbacks=st_l["guess_fallbacks"],
+                       iters_per_cell=win_l["pred_iters"] / max(1, win_l["pred_cells"]),
+                       max_iters_run=st_l["pred_maxit"], capped=win_l["pred_capped"],
+                       guess_fallbacks=win_l["guess_fallbacks"],
                        kernel_ms_per_step={k: v / K for k, v in st_lk["ms_kernel"].items()},
                        linf_vs_plain=rel_linf(u_l, u_p), training=info,
-                       roofline=predictor_roofline(st_lk, K))
+                       roofline=predictor_roofline(st_lk, win_lk, K, guess=True))
     # the headline is the learned leg only where it keeps the metric's accuracy
     # bound (final state within 1e-8 of the plain FP64 one) and is faster
     use_learned = (learned is not None and learned["linf_vs_plain"] <= 1e-8

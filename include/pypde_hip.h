@@ -7,8 +7,11 @@
  * specialised for (ndim, N, V).
  *
  * Return codes: 0 success, >0 argument / compile / runtime errors (text via
- * pypde_hip_last_error), <0 NaN detected at step -rc (the reference prints
- * "NaNs found" and continues, iterator.cpp:141-145).
+ * pypde_hip_last_error), <0 NaN detected at step -rc.  A run stops at its
+ * first NaN step; the reference instead prints "NaNs found", writes the
+ * previous and the NaN state into rows pushCount and pushCount+1 of ret, and
+ * keeps stepping (iterator.cpp:141-147).  Rows pushed before the NaN step
+ * and the final row (the NaN state) agree with the reference.
  */
 #ifndef PYPDE_HIP_H
 #define PYPDE_HIP_H
@@ -57,6 +60,10 @@ typedef struct {
   double ms_loop;           /* wall time of the step loop */
   double ms_kernel[8];      /* profile: summed time per kernel (weno, dt, learned guess,
                                predictor, flux0, flux1, update, bookkeeping) */
+  long pred_node_full;      /* stiff predictor: per-node Newton corrections that solved
+                               with the source Jacobian, summed over nodes and steps */
+  long pred_node_short;     /* ... and those that took the A = 0 shortcut (max |dt dS/dQ|
+                               < 1e-6); both counts run from t = 0 like pred_iters */
 } pypde_stats;
 
 void pypde_default_options(pypde_options *opts);
@@ -74,8 +81,9 @@ int pde_solver_hip(const char *src, bool useF, bool useB, bool useS, double *u, 
                    const pypde_options *opts, pypde_stats *stats);
 
 /* GPU replacement of the reference's weno_solver (api.h:12-13, api.cpp:32-48):
- * same signature and semantics (no ghost cells; ret has shape
- * (nX - 2(N-1)) x N^ndim x V).  Returns 0 on success. */
+ * same arguments and semantics (no ghost cells; ret has shape
+ * (nX - 2(N-1)) x N^ndim x V).  The reference returns void; this returns an
+ * int status, 0 on success. */
 int weno_solver(double *ret, double *u, int *nX, int ndim, int N, int V);
 
 /* Persistent solver context: compile once, keep the state resident in HBM. */

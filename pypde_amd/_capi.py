@@ -25,7 +25,8 @@ class Stats(C.Structure):
     _fields_ = [("struct_size", C.c_int), ("steps", C.c_long), ("t", C.c_double),
                 ("pred_iters", C.c_long), ("pred_cells", C.c_long), ("pred_maxit", C.c_long),
                 ("pred_capped", C.c_long), ("guess_fallbacks", C.c_long), ("nan_step", C.c_int),
-                ("ms_loop", C.c_double), ("ms_kernel", C.c_double * 8)]
+                ("ms_loop", C.c_double), ("ms_kernel", C.c_double * 8),
+                ("pred_node_full", C.c_long), ("pred_node_short", C.c_long)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("struct_size", "ms_kernel")}

@@ -33,11 +33,14 @@ struct StepState {
   unsigned long long pred_iters, pred_cells, pred_capped, pred_fallbacks;
   unsigned int pred_maxit, pad3;
   unsigned long long qhead;     // predictor work queue: next unclaimed w-cell
+  // stiff predictor: Newton corrections per spatial node with the full source
+  // Jacobian solve and with the A = 0 shortcut, summed over nodes and steps
+  unsigned long long pred_nfull, pred_nshort;
   // predictor statistics spread over 64 slots, one 128-byte line each, so the
   // wavefronts' atomics do not queue on one address; k_advance folds them into
   // the totals above and clears them ([0] iterations, [1] cells, [2] capped,
-  // [3] fallbacks, [4] max iterations; [8] k_dt's partial maximum, folded by
-  // k_dt_finalize)
+  // [3] fallbacks, [4] max iterations, [5] full and [6] shortcut node
+  // corrections; [8] k_dt's partial maximum, folded by k_dt_finalize)
   unsigned long long pstat[64][16];
 };
 

@@ -273,7 +273,8 @@ constexpr double NEWTON_A_TINY = 1e-6;  // max |dt dS/dQ| below which the step d
 // Newton correction at one spatial node (see the CPU restatement for the
 // derivation): g_t = G_t / (W_t WP), A = dt dS/dQ at the time-weighted mean,
 // h_t = sum_j A^j sum_s M_{N-j}[t][s] g_s, P = p(A), dq = -P^-1 h.
-DEV void newton_node(const NodeCtx& c, const double (&q)[N][V], double (&G)[N][V], double dt) {
+// Returns true where the source Jacobian was solved, false for the shortcut.
+DEV bool newton_node(const NodeCtx& c, const double (&q)[N][V], double (&G)[N][V], double dt) {
   double g[N][V];
 #pragma unroll
   for (int t = 0; t < N; t++)
@@ -328,7 +329,7 @@ DEV void newton_node(const NodeCtx& c, const double (&q)[N][V], double (&G)[N][V
         for (int s2 = 1; s2 < N; s2++) acc = dfma(T_FLM[N - 1][t][s2], g[s2][v], acc);
         G[t][v] = -(acc * (1.0 / T_FLC[0]));
       }
-    return;
+    return false;
   }
   // h_t (Horner in A); result left in G
 #pragma unroll
@@ -461,6 +462,7 @@ DEV void newton_node(const NodeCtx& c, const double (&q)[N][V], double (&G)[N][V
       b[i] = -(acc * rd[i]);
     }
   }
+  return true;
 }
 #endif
 
@@ -499,6 +501,7 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
   // per-lane statistics over the cells this lane's slot has run
   unsigned long long s_iters = 0ull, s_cells = 0ull, s_cap = 0ull, s_fb = 0ull;
   unsigned int s_max = 0u;
+  unsigned int n_full = 0u, n_short = 0u;  // node corrections of this lane
 
   auto start_cell = [&](long cl) {
     cell = cl;
@@ -582,7 +585,8 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
     }
 #endif
     if (active && !skip) {
-      newton_node(c, q, rhs, dt);
+      if (newton_node(c, q, rhs, dt)) n_full++;
+      else n_short++;
 #pragma unroll
       for (int t = 0; t < N; t++)
 #pragma unroll
@@ -729,6 +733,7 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
   s_cap = wave_sum_u64(s_cap);
   s_fb = wave_sum_u64(s_fb);
   s_max = wave_max_u32(s_max);
+  const unsigned long long w_full = wave_sum_u64(n_full), w_short = wave_sum_u64(n_short);
   if (lane == 0 && s_cells) {
     unsigned long long* ps = st->pstat[blockIdx.x & 63];
     atomicAdd(&ps[0], s_iters);
@@ -736,6 +741,8 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
     if (s_cap) atomicAdd(&ps[2], s_cap);
     if (s_fb) atomicAdd(&ps[3], s_fb);
     atomicMax(&ps[4], (unsigned long long)s_max);
+    if (w_full) atomicAdd(&ps[5], w_full);
+    if (w_short) atomicAdd(&ps[6], w_short);
   }
 }
 
@@ -781,6 +788,7 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
   int fellback = 0;
   double r0 = 0.0;
   int trip = 0;
+  unsigned int n_full = 0u, n_short = 0u;  // node corrections of this lane
   while (__ballot(active) != 0ull) {
     if (active) {
 #pragma unroll
@@ -842,7 +850,8 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
     }
 #endif
     if (active && !skip) {
-      newton_node(c, q, rhs, dt);
+      if (newton_node(c, q, rhs, dt)) n_full++;
+      else n_short++;
 #pragma unroll
       for (int t = 0; t < N; t++)
 #pragma unroll
@@ -978,6 +987,7 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
   cap = wave_sum_u64(cap);
   fb = wave_sum_u64(fb);
   mx = wave_max_u32(mx);
+  const unsigned long long w_full = wave_sum_u64(n_full), w_short = wave_sum_u64(n_short);
   if (lane == 0) {
     unsigned long long* ps = st->pstat[blockIdx.x & 63];
     atomicAdd(&ps[0], mine);
@@ -985,6 +995,8 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
     if (cap) atomicAdd(&ps[2], cap);
     if (fb) atomicAdd(&ps[3], fb);
     atomicMax(&ps[4], (unsigned long long)mx);
+    if (w_full) atomicAdd(&ps[5], w_full);
+    if (w_short) atomicAdd(&ps[6], w_short);
   }
 }
 
@@ -1258,6 +1270,7 @@ extern "C" __global__ void k_advance(StepState* st, int ndt) {
     unsigned long long* ps = st->pstat[threadIdx.x & 63];
     unsigned long long a0 = wave_sum_u64(ps[0]), a1 = wave_sum_u64(ps[1]);
     unsigned long long a2 = wave_sum_u64(ps[2]), a3 = wave_sum_u64(ps[3]);
+    unsigned long long a5 = wave_sum_u64(ps[5]), a6 = wave_sum_u64(ps[6]);
     unsigned long long m4 = ps[4];
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
@@ -1265,12 +1278,14 @@ extern "C" __global__ void k_advance(StepState* st, int ndt) {
       m4 = m4 > y ? m4 : y;
     }
 #pragma unroll
-    for (int k = 0; k < 5; k++) ps[k] = 0ull;
+    for (int k = 0; k < 7; k++) ps[k] = 0ull;
     if (threadIdx.x == 0) {
       st->pred_iters += a0;
       st->pred_cells += a1;
       st->pred_capped += a2;
       st->pred_fallbacks += a3;
+      st->pred_nfull += a5;
+      st->pred_nshort += a6;
       if ((unsigned int)m4 > st->pred_maxit) st->pred_maxit = (unsigned int)m4;
     }
   }

@@ -76,11 +76,12 @@ struct StepState {
   unsigned long long pred_iters, pred_cells, pred_capped, pred_fallbacks;
   unsigned int pred_maxit, pad3;
   unsigned long long qhead;
+  unsigned long long pred_nfull, pred_nshort;
   // predictor statistics spread over 64 slots, one 128-byte line each, so the
   // wavefronts' atomics do not queue on one address; k_advance folds them into
   // the totals above and clears them ([0] iterations, [1] cells, [2] capped,
-  // [3] fallbacks, [4] max iterations; [8] k_dt's partial maximum, folded by
-  // k_dt_finalize)
+  // [3] fallbacks, [4] max iterations, [5] full and [6] shortcut node
+  // corrections; [8] k_dt's partial maximum, folded by k_dt_finalize)
   unsigned long long pstat[64][16];
 };
 
@@ -146,22 +147,18 @@ const std::vector<std::string>& compile_options() {
 std::mutex g_mu;
 std::map<uint64_t, std::vector<char>> g_codes;
 
-int compile_code(const Spec& spec, std::vector<char>* code) {
-  pypde::Tables T;
-  if (pypde::make_tables(spec.N, &T)) return fail(PYPDE_ERR_ARG, "bad order N=%d", spec.N);
-  std::string src = full_source(spec, T);
-  if (const char* dump = getenv("PYPDE_HIP_DUMP_SRC")) {
-    std::ofstream f(dump);
-    f << src;
-  }
+// Compiles `src` for gfx950 with hipRTC, through a code-object cache keyed by
+// the source text, the compile options and the hipRTC version (in memory, then
+// <cache_dir>/<key>.hsaco, written to a temporary name and renamed into place).
+// PYPDE_HIP_RECOMPILE=1 compiles even when a cached code object exists (the
+// build's compile check); the result still replaces the cache entry.
+int rtc_compile(const std::string& src, const char* what, std::vector<char>* code) {
   std::string key = src;
   for (auto& x : compile_options()) key += "\n" + x;
   int ver = 0;
   hiprtcVersion(&ver, &ver);
   key += "\nhiprtc" + std::to_string(ver);
   uint64_t h = fnv1a(key);
-  // PYPDE_HIP_RECOMPILE=1: compile even when a cached code object exists (the
-  // build's compile check); the result still replaces the cache entry
   const char* re = getenv("PYPDE_HIP_RECOMPILE");
   const bool recompile = re && *re && *re != '0';
   if (!recompile) {
@@ -189,9 +186,8 @@ int compile_code(const Spec& spec, std::vector<char>* code) {
     }
   }
   hiprtcProgram prog;
-  if (hiprtcCreateProgram(&prog, src.c_str(), "pypde_ader.hip", 0, nullptr, nullptr) !=
-      HIPRTC_SUCCESS)
-    return fail(PYPDE_ERR_COMPILE, "hiprtcCreateProgram failed");
+  if (hiprtcCreateProgram(&prog, src.c_str(), what, 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+    return fail(PYPDE_ERR_COMPILE, "hiprtcCreateProgram failed (%s)", what);
   std::vector<const char*> opts;
   for (auto& x : compile_options()) opts.push_back(x.c_str());
   hiprtcResult r = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
@@ -201,7 +197,7 @@ int compile_code(const Spec& spec, std::vector<char>* code) {
   if (ls) hiprtcGetProgramLog(prog, &log[0]);
   if (r != HIPRTC_SUCCESS) {
     hiprtcDestroyProgram(&prog);
-    return fail(PYPDE_ERR_COMPILE, "hipRTC compile failed:\n%s", log.c_str());
+    return fail(PYPDE_ERR_COMPILE, "hipRTC compile of %s failed:\n%s", what, log.c_str());
   }
   size_t cs = 0;
   hiprtcGetCodeSize(prog, &cs);
@@ -215,13 +211,25 @@ int compile_code(const Spec& spec, std::vector<char>* code) {
     if (f) {
       f.write(c.data(), (std::streamsize)c.size());
       f.close();
-      rename(tmp.c_str(), path.c_str());
+      if (f) rename(tmp.c_str(), path.c_str());
+      else unlink(tmp.c_str());
     }
   }
   std::lock_guard<std::mutex> lk(g_mu);
   g_codes[h] = c;
   *code = c;
   return 0;
+}
+
+int compile_code(const Spec& spec, std::vector<char>* code) {
+  pypde::Tables T;
+  if (pypde::make_tables(spec.N, &T)) return fail(PYPDE_ERR_ARG, "bad order N=%d", spec.N);
+  std::string src = full_source(spec, T);
+  if (const char* dump = getenv("PYPDE_HIP_DUMP_SRC")) {
+    std::ofstream f(dump);
+    f << src;
+  }
+  return rtc_compile(src, "pypde_ader.hip", code);
 }
 
 int check_spec(const Spec& s, bool secondOrder, int FLUX) {
@@ -316,6 +324,13 @@ int launch(pypde_ctx* c, int k, dim3 grid, dim3 block, std::vector<void*>& args)
 }
 
 long cdiv(long a, long b) { return (a + b - 1) / b; }
+
+// Callers built against the older, shorter pypde_stats leave out the node
+// counters; struct_size says which layout the caller has.
+bool stats_has_nodes(const pypde_stats* s) {
+  return s->struct_size <= 0 ||
+         (size_t)s->struct_size >= offsetof(pypde_stats, pred_node_short) + sizeof(long);
+}
 
 // Workgroups of the persistent predictor: as many as can be resident at once
 // (occupancy x compute units; PYPDE_PRED_BLOCKS overrides), never more than
@@ -775,6 +790,10 @@ static int run_loop(pypde_ctx* c, long max_steps, pypde_stats* stats, bool first
     stats->nan_step = c->host_st.nan_step;
     stats->ms_loop = std::chrono::duration<double, std::milli>(t1 - t0).count();
     for (int k = 0; k < 8; k++) stats->ms_kernel[k] = c->ms_kernel[k];
+    if (stats_has_nodes(stats)) {
+      stats->pred_node_full = (long)c->host_st.pred_nfull;
+      stats->pred_node_short = (long)c->host_st.pred_nshort;
+    }
   }
   return c->host_st.nan_step ? -c->host_st.nan_step : 0;
 }
@@ -949,6 +968,10 @@ int pypde_ensemble_run(pypde_ensemble* e, double tf, long max_steps, pypde_stats
       o->nan_step = h.nan_step;
       o->ms_loop = std::chrono::duration<double, std::milli>(t1 - t0).count();
       for (int k = 0; k < 8; k++) o->ms_kernel[k] = 0.0;
+      if (stats_has_nodes(stats)) {
+        o->pred_node_full = (long)h.pred_nfull;
+        o->pred_node_short = (long)h.pred_nshort;
+      }
     }
   }
   return nan ? -nan : 0;
@@ -1084,38 +1107,12 @@ static int mlp_fn(int k, hipFunction_t* f) {
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
   if (dev < 0 || dev >= 16) return fail(PYPDE_ERR_ARG, "device %d", dev);
-  std::lock_guard<std::mutex> lk(g_mu);
+  static std::mutex mu;  // not g_mu: rtc_compile takes that one
+  std::lock_guard<std::mutex> lk(mu);
   if (!g_mlp_mod[dev]) {
-    std::string src = MLP_SRC;
-    uint64_t h = fnv1a(src + "mlp");
-    char name[64];
-    snprintf(name, sizeof name, "%016llx", (unsigned long long)h);
-    std::string path = cache_dir() + "/" + name + ".hsaco";
     std::vector<char> code;
-    std::ifstream fin(path, std::ios::binary);
-    if (fin) code.assign((std::istreambuf_iterator<char>(fin)), std::istreambuf_iterator<char>());
-    if (code.empty()) {
-      hiprtcProgram prog;
-      hiprtcCreateProgram(&prog, src.c_str(), "pypde_mlp.hip", 0, nullptr, nullptr);
-      std::vector<const char*> o;
-      for (auto& x : compile_options()) o.push_back(x.c_str());
-      if (hiprtcCompileProgram(prog, (int)o.size(), o.data()) != HIPRTC_SUCCESS) {
-        size_t ls = 0;
-        hiprtcGetProgramLogSize(prog, &ls);
-        std::string log(ls, '\0');
-        if (ls) hiprtcGetProgramLog(prog, &log[0]);
-        hiprtcDestroyProgram(&prog);
-        return fail(PYPDE_ERR_COMPILE, "MLP kernels: %s", log.c_str());
-      }
-      size_t cs = 0;
-      hiprtcGetCodeSize(prog, &cs);
-      code.resize(cs);
-      hiprtcGetCode(prog, code.data());
-      hiprtcDestroyProgram(&prog);
-      mkdir(cache_dir().c_str(), 0755);
-      std::ofstream fo(path, std::ios::binary);
-      fo.write(code.data(), (std::streamsize)code.size());
-    }
+    int rc = rtc_compile(MLP_SRC, "pypde_mlp.hip", &code);
+    if (rc) return rc;
     HIPCHK(hipModuleLoadData(&g_mlp_mod[dev], code.data()));
     for (int i = 0; i < 6; i++) HIPCHK(hipModuleGetFunction(&g_mlp_fn[dev][i], g_mlp_mod[dev], mlp_names[i]));
   }
@@ -1171,32 +1168,8 @@ int pypde_device_count(void) {
 
 int pypde_mlp_compile(void) {
   // compile (or fetch) the training kernels without a GPU
-  std::string src = MLP_SRC;
-  hiprtcProgram prog;
-  hiprtcCreateProgram(&prog, src.c_str(), "pypde_mlp.hip", 0, nullptr, nullptr);
-  std::vector<const char*> o;
-  for (auto& x : compile_options()) o.push_back(x.c_str());
-  hiprtcResult r = hiprtcCompileProgram(prog, (int)o.size(), o.data());
-  size_t ls = 0;
-  hiprtcGetProgramLogSize(prog, &ls);
-  std::string log(ls, '\0');
-  if (ls) hiprtcGetProgramLog(prog, &log[0]);
-  if (r != HIPRTC_SUCCESS) {
-    hiprtcDestroyProgram(&prog);
-    return fail(PYPDE_ERR_COMPILE, "MLP kernels: %s", log.c_str());
-  }
-  size_t cs = 0;
-  hiprtcGetCodeSize(prog, &cs);
-  std::vector<char> code(cs);
-  hiprtcGetCode(prog, code.data());
-  hiprtcDestroyProgram(&prog);
-  uint64_t h = fnv1a(src + "mlp");
-  char name[64];
-  snprintf(name, sizeof name, "%016llx", (unsigned long long)h);
-  mkdir(cache_dir().c_str(), 0755);
-  std::ofstream fo(cache_dir() + "/" + name + ".hsaco", std::ios::binary);
-  fo.write(code.data(), (std::streamsize)code.size());
-  return 0;
+  std::vector<char> code;
+  return rtc_compile(MLP_SRC, "pypde_mlp.hip", &code);
 }
 
 int pypde_mlp_gemm(const float* A, int lda, int transA, const float* B, int ldb, int transB,
