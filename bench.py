@@ -2,23 +2,24 @@
 channel, order 3, stiff per-cell Newton predictor, reflective walls, FP64.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-learned]
-                    [--no-ensemble]
+                    [--no-mixed] [--no-ensemble]
 
 Prints one JSON line.  A step is one full ADER-WENO timestep of the whole
 grid (boundaries + WENO, CFL reduction, learned guess, predictor, face fluxes,
 corrector); the state is resident in HBM before the timed region.
 cell-updates/s = cells x K / wall time of the K timed steps.
 
-Two legs on the same seeded initial state (seed 99): the plain FP64 method
-(constant-in-time Newton initial guess) and the learned one (an MLP trained
+Legs on the same seeded initial state (seed 99): the plain FP64 method
+(constant-in-time Newton initial guess); the learned one (an MLP trained
 here, on the GPU, on predictor solutions harvested from seeds 0..15 of the same
-channel at 256 x 64 cells and the same cell size).  `value` is the learned
-leg's rate where its final state stays within 1e-8 (relative L_inf) of the
-plain one and it is faster, else the plain leg's (`value_leg` says which and
-why); both legs' rates, Newton iteration counts and their difference in the
-final state are reported beside it, then an ensemble leg (48 small 2D Euler
-grids, one graph launch per timestep) and a 512 x 512 periodic 2D Euler leg at
-order 4 (both skipped with --no-ensemble).
+channel at 256 x 64 cells and the same cell size); the mixed-precision one (an
+FP32 predictor phase, then the FP64 predictor from its iterate); and learned +
+mixed.  `value` is the fastest other leg's rate where its final state stays
+within 1e-8 (relative L_inf) of the plain one and it is faster, else the plain
+leg's (`value_leg` says which and why); every leg's rate, Newton iteration
+counts and difference in the final state are reported beside it, then an
+ensemble leg (48 small 2D Euler grids, one graph launch per timestep) and a
+512 x 512 periodic 2D Euler leg at order 4 (both skipped with --no-ensemble).
 
     python bench.py --calibrate-copy   # 3 x 256 MiB device copy, for HBM
                                        # counter calibration under rocprofv3
@@ -121,10 +122,10 @@ def profile_traffic(kernel):
     return None
 
 
-def make_solver(u0, L_, profile=False):
+def make_solver(u0, L_, profile=False, mixed=False):
     return Solver(u0, L_, systems.source("reactive2d"), order=ORDER, boundaryTypes="reflective",
                   stiff=True, reflect=systems.REACTIVE_REFLECT, newton_tol=1e-12,
-                  newton_maxit=100, profile=profile)
+                  newton_maxit=100, profile=profile, mixed=mixed)
 
 
 def train_guess_model(steps, pairs, epochs, log):
@@ -157,9 +158,9 @@ def train_guess_model(steps, pairs, epochs, log):
     return net, info
 
 
-def gpu_run(steps, warmup, profile, net=None):
+def gpu_run(steps, warmup, profile, net=None, mixed=False):
     u0 = systems.reactive_channel(NX, NY, seed=EVAL_SEED)
-    sol = make_solver(u0, L, profile=profile)
+    sol = make_solver(u0, L, profile=profile, mixed=mixed)
     if net is not None:
         net.attach(sol)
     w = sol.step(warmup) if warmup > 0 else None
@@ -228,7 +229,7 @@ def rel_linf(a, b):
 def window(st, w):
     """Counters of the timed steps only: st after them minus w before them."""
     keys = ("steps", "pred_iters", "pred_cells", "pred_capped", "guess_fallbacks",
-            "pred_node_full", "pred_node_short")
+            "pred_node_full", "pred_node_short", "pred_iters_f32")
     return {k: st[k] - (w[k] if w else 0) for k in keys}
 
 
@@ -334,6 +335,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision legs (plain and learned legs only)")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the ensemble and the 512 x 512 Euler legs")
     ap.add_argument("--cpu-steps", type=int, default=6)
@@ -390,18 +393,39 @@ def main():
                        kernel_ms_per_step={k: v / K for k, v in st_lk["ms_kernel"].items()},
                        linf_vs_plain=rel_linf(u_l, u_p), training=info,
                        roofline=predictor_roofline(st_lk, win_lk, K, guess=True))
-    # the headline is the learned leg only where it keeps the metric's accuracy
+    def leg(name, net_, mixed):
+        st_a, el_a, w_a, u_a = gpu_run(K, W, profile=False, net=net_, mixed=mixed)
+        st_b, _, w_b, _ = gpu_run(K, W, profile=True, net=net_, mixed=mixed)
+        wa, wb = window(st_a, w_a), window(st_b, w_b)
+        log(f"{name}: {el_a * 1e3 / K:.3f} ms/step")
+        return dict(value=cells * K / el_a, ms_per_step=el_a * 1e3 / K,
+                    iters_per_cell=wa["pred_iters"] / max(1, wa["pred_cells"]),
+                    fp32_iters_per_cell=(wa["pred_iters_f32"] / max(1, wa["pred_cells"]))
+                    if mixed else None,
+                    capped=wa["pred_capped"], guess_fallbacks=wa["guess_fallbacks"],
+                    kernel_ms_per_step={k: v / K for k, v in st_b["ms_kernel"].items()},
+                    linf_vs_plain=rel_linf(u_a, u_p),
+                    roofline=predictor_roofline(st_b, wb, K, guess=True))
+    mixed = lmixed = None
+    if not args.no_mixed:
+        mixed = leg("mixed", None, True)
+        if learned is not None:
+            lmixed = leg("learned+mixed", net, True)
+            lmixed["training"] = "the learned leg's network"
+    # the headline is another leg only where it keeps the metric's accuracy
     # bound (final state within 1e-8 of the plain FP64 one) and is faster
-    use_learned = (learned is not None and learned["linf_vs_plain"] <= 1e-8
-                   and learned["value"] > plain["value"])
-    head = learned if use_learned else plain
-    if learned is None:
-        value_leg = "plain (learned leg not run)"
-    elif use_learned:
-        value_leg = "learned"
+    legs = {"learned": learned, "mixed": mixed, "learned+mixed": lmixed}
+    ok = {k: v for k, v in legs.items()
+          if v is not None and v["linf_vs_plain"] <= 1e-8 and v["value"] > plain["value"]}
+    if ok:
+        value_leg = max(ok, key=lambda k: ok[k]["value"])
+        head = ok[value_leg]
     else:
-        value_leg = ("plain (learned leg: L_inf vs plain %.2e, %.2fx the plain rate)"
-                     % (learned["linf_vs_plain"], learned["value"] / plain["value"]))
+        head = plain
+        value_leg = "plain (" + "; ".join(
+            "%s leg: L_inf vs plain %.2e, %.2fx the plain rate"
+            % (k, v["linf_vs_plain"], v["value"] / plain["value"])
+            for k, v in legs.items() if v is not None) + ")"
     ens = None
     if not args.no_ensemble:
         ens = ensemble_leg(20)
@@ -432,14 +456,16 @@ def main():
         "config": {"workload": "2D reactive Euler, Arrhenius (Q=25, Ea=20, K=1e4, gamma=1.2), "
                                "1024x256 channel, order 3, Rusanov, CFL 0.9, reflective walls, "
                                "stiff per-cell Newton predictor (tol 1e-12), "
-                               + ("learned Newton initial guess (BF16 MLP)"
-                                  if use_learned else "plain FP64"),
+                               + (value_leg if ok else "plain FP64"),
                    "cells": cells, "order": ORDER, "nvar": V},
         "roofline": head["roofline"],
         "cpu_baseline": cpu,
-        "speedup_vs_plain": (learned["value"] / plain["value"]) if learned else None,
+        "speedup_vs_plain": {k: v["value"] / plain["value"] for k, v in legs.items()
+                             if v is not None},
         "plain": plain,
         "learned": learned,
+        "mixed": mixed,
+        "learned_mixed": lmixed,
         "ensemble": ens,
         "euler2d_512_order4": modes,
         "t_end": st_p["t"],

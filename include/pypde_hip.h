@@ -45,6 +45,13 @@ typedef struct {
                                nwc x N^(ndim+1) x V (testing the learned-guess path) */
   int use_graph;            /* capture one timestep in a HIP graph and replay it [0]
                                (ignored while profiling) */
+  int mixed;                /* mixed precision: an FP32 phase of the predictor (contractions,
+                               F and S in FP32; stiff Newton solves with the FP64 source
+                               Jacobian) runs until its corrections fall below mixed_tol,
+                               then the FP64 predictor iterates on from that iterate to the
+                               unchanged tolerance [0].  Systems with F, S (and M) only */
+  double mixed_tol;         /* FP32 phase: |dq| <= mixed_tol (1 + |q|) [6e-6] */
+  int mixed_maxit;          /* FP32 phase iteration cap [20] */
 } pypde_options;
 
 typedef struct {
@@ -64,6 +71,7 @@ typedef struct {
                                with the source Jacobian, summed over nodes and steps */
   long pred_node_short;     /* ... and those that took the A = 0 shortcut (max |dt dS/dQ|
                                < 1e-6); both counts run from t = 0 like pred_iters */
+  long pred_iters_f32;      /* mixed precision: FP32-phase iterations summed over cells */
 } pypde_stats;
 
 void pypde_default_options(pypde_options *opts);
@@ -86,8 +94,9 @@ int pde_solver_hip(const char *src, bool useF, bool useB, bool useS, double *u, 
  * int status, 0 on success. */
 int weno_solver(double *ret, double *u, int *nX, int ndim, int N, int V);
 
-/* Persistent solver context: compile once, keep the state resident in HBM. */
 typedef struct pypde_ctx pypde_ctx;
+
+/* Persistent solver context: compile once, keep the state resident in HBM. */
 pypde_ctx *pypde_create(const char *src, bool useF, bool useB, bool useS, int *nX, int ndim,
                         double *dX, double CFL, int *boundaryTypes, bool STIFF, int FLUX, int N,
                         int V, bool secondOrder, const pypde_options *opts, int *rc);

@@ -18,7 +18,8 @@ class Options(C.Structure):
     _fields_ = [("struct_size", C.c_int), ("newton_tol", C.c_double),
                 ("newton_maxit", C.c_int), ("max_steps", C.c_long), ("reflect", PD),
                 ("b_quirk", C.c_int), ("use_M", C.c_int), ("device", C.c_int),
-                ("profile", C.c_int), ("guess", PD), ("use_graph", C.c_int)]
+                ("profile", C.c_int), ("guess", PD), ("use_graph", C.c_int),
+                ("mixed", C.c_int), ("mixed_tol", C.c_double), ("mixed_maxit", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -26,7 +27,8 @@ class Stats(C.Structure):
                 ("pred_iters", C.c_long), ("pred_cells", C.c_long), ("pred_maxit", C.c_long),
                 ("pred_capped", C.c_long), ("guess_fallbacks", C.c_long), ("nan_step", C.c_int),
                 ("ms_loop", C.c_double), ("ms_kernel", C.c_double * 8),
-                ("pred_node_full", C.c_long), ("pred_node_short", C.c_long)]
+                ("pred_node_full", C.c_long), ("pred_node_short", C.c_long),
+                ("pred_iters_f32", C.c_long)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("struct_size", "ms_kernel")}
@@ -113,9 +115,15 @@ def ip(a):
 
 
 def make_options(newton_tol=1e-12, newton_maxit=100, max_steps=0, reflect=None, b_quirk=0,
-                 use_M=False, device=0, profile=False, guess=None, use_graph=False):
+                 use_M=False, device=0, profile=False, guess=None, use_graph=False, mixed=False,
+                 mixed_tol=None, mixed_maxit=None):
     o = Options()
     lib().pypde_default_options(C.byref(o))
+    o.mixed = int(bool(mixed))
+    if mixed_tol is not None:
+        o.mixed_tol = mixed_tol
+    if mixed_maxit is not None:
+        o.mixed_maxit = mixed_maxit
     o.newton_tol = newton_tol
     o.newton_maxit = newton_maxit
     o.max_steps = max_steps
@@ -138,8 +146,8 @@ def make_options(newton_tol=1e-12, newton_maxit=100, max_steps=0, reflect=None, 
 
 
 def compile_only(src, useF, useB, useS, ndim, N, V, stiff=False, use_M=False, b_quirk=0, flux=0,
-                 second=False):
-    o = make_options(use_M=use_M, b_quirk=b_quirk)
+                 second=False, mixed=False):
+    o = make_options(use_M=use_M, b_quirk=b_quirk, mixed=mixed)
     rc = lib().pypde_hip_compile(src.encode(), useF, useB, useS, ndim, N, V, stiff, int(flux),
                                  bool(second), C.byref(o))
     check(rc, f"hipRTC compile (ndim={ndim}, N={N}, V={V})")

@@ -36,11 +36,13 @@ struct StepState {
   // stiff predictor: Newton corrections per spatial node with the full source
   // Jacobian solve and with the A = 0 shortcut, summed over nodes and steps
   unsigned long long pred_nfull, pred_nshort;
+  unsigned long long pred_iters32;  // mixed precision: FP32-phase iterations over cells
   // predictor statistics spread over 64 slots, one 128-byte line each, so the
   // wavefronts' atomics do not queue on one address; k_advance folds them into
   // the totals above and clears them ([0] iterations, [1] cells, [2] capped,
   // [3] fallbacks, [4] max iterations, [5] full and [6] shortcut node
-  // corrections; [8] k_dt's partial maximum, folded by k_dt_finalize)
+  // corrections, [7] FP32-phase iterations; [8] k_dt's partial maximum,
+  // folded by k_dt_finalize)
   unsigned long long pstat[64][16];
 };
 

@@ -1020,6 +1020,226 @@ k_predictor_static2(const double* __restrict__ wh, const double* __restrict__ gu
   predictor_static_body(sm, wh, guess, tr, vol, qh, st, g, newton_tol, newton_maxit, trd);
 }
 
+#if PDE_MIXED
+// ---------------------------------------------- mixed precision: FP32 phase
+// The predictor's space-time iteration with its contractions (spatial
+// derivatives of q and of F, the time operator of the residual), F and S in
+// FP32; the stiff Newton correction still solves with the FP64 source
+// Jacobian (newton_node).  It runs until the FP32 corrections fall below
+// tol32 (1 + |q|), at most maxit32 iterations, and leaves its iterate as the
+// initial guess of the FP64 predictor, which iterates on from it to the
+// unchanged tolerance (the FP64 residual correction).  A cell whose FP32
+// iterate is not finite hands over the plain guess instead.
+struct PredLDS32 {
+  float q[CPW][N][ND][V];
+#if PDE_USE_F
+  float f[D][CPW][N][ND][V];
+#endif
+};
+
+DEV void F32(float* out, const float* q, const float* dq, int d) {
+  f32::R o[V], qq[V], dd[D * V];
+#pragma unroll
+  for (int v = 0; v < V; v++) qq[v] = f32::R(q[v]);
+#pragma unroll
+  for (int i = 0; i < D * V; i++) dd[i] = f32::R(dq[i]);
+  f32::F(o, qq, dd, d);
+#pragma unroll
+  for (int v = 0; v < V; v++) out[v] = o[v].v;
+}
+
+DEV void S32(float* out, const float* q) {
+#if PDE_USE_S
+  f32::R o[V], qq[V];
+#pragma unroll
+  for (int v = 0; v < V; v++) qq[v] = f32::R(q[v]);
+  f32::S(o, qq);
+#pragma unroll
+  for (int v = 0; v < V; v++) out[v] = o[v].v;
+#else
+#pragma unroll
+  for (int v = 0; v < V; v++) out[v] = 0.0f;
+#endif
+}
+
+template <int d>
+DEV void lds_deriv32(const NodeCtx& c, const float (*buf)[N][ND][V], int t, float rdx, float* out) {
+  constexpr int stride = ipow_c(N, D - 1 - d);
+#pragma unroll
+  for (int v = 0; v < V; v++) {
+    float acc = (float)c.der[d][0] * buf[c.slot][t][c.base[d]][v];
+#pragma unroll
+    for (int j = 1; j < N; j++)
+      acc = __builtin_fmaf((float)c.der[d][j], buf[c.slot][t][c.base[d] + j * stride][v], acc);
+    out[v] = acc * rdx;
+  }
+}
+
+// rhs of the space-time system in FP32 (node_rhs); CONST: q is constant in
+// time, so F, S and derivatives are evaluated once and reused for every t.
+template <bool CONST>
+DEV void node_rhs32(const NodeCtx& c, PredLDS32& sm, const float (&q)[N][V], const float* w,
+                    float dt, const float* rdx, bool active, float (&rhs)[N][V]) {
+  constexpr int NTE = CONST ? 1 : N;  // time nodes evaluated
+  float dqs[NTE][D][V];
+  if (active) {
+#pragma unroll
+    for (int t = 0; t < NTE; t++) {
+      static_for<0, D>([&](auto dc) {
+        constexpr int d = decltype(dc)::value;
+        lds_deriv32<d>(c, sm.q, t, rdx[d], dqs[t][d]);
+      });
+#if PDE_USE_F
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        float f[V];
+        F32(f, q[t], &dqs[t][0][0], d);
+#pragma unroll
+        for (int v = 0; v < V; v++) sm.f[d][c.slot][t][c.idx][v] = f[v];
+      }
+#endif
+    }
+  }
+#if PDE_USE_F
+  __syncthreads();
+#endif
+  if (active) {
+    float r[NTE][V];
+#pragma unroll
+    for (int t = 0; t < NTE; t++) {
+      S32(r[t], q[t]);
+#if PDE_USE_F
+      static_for<0, D>([&](auto dc) {
+        constexpr int d = decltype(dc)::value;
+        float df[V];
+        lds_deriv32<d>(c, sm.f[d], t, rdx[d], df);
+#pragma unroll
+        for (int v = 0; v < V; v++) r[t][v] = r[t][v] - df[v];
+      });
+#endif
+    }
+#pragma unroll
+    for (int t = 0; t < N; t++)
+#pragma unroll
+      for (int v = 0; v < V; v++)
+        rhs[t][v] = (r[CONST ? 0 : t][v] * (float)c.ct[t]) * dt + (float)c.c0[t] * w[v];
+  }
+}
+
+extern "C" __global__ void __launch_bounds__(64)
+k_predictor_f32(const double* __restrict__ wh, const double* __restrict__ gin,
+                double* __restrict__ gout, StepState* st, Geom g, float tol32, int maxit32) {
+  if (st->done) return;
+  __shared__ PredLDS32 sm;
+  const float dt = (float)st->dt;
+  const double dtd = st->dt;
+  const int lane = threadIdx.x;
+  NodeCtx c;
+  node_ctx(c, lane);
+  const long cell = (long)blockIdx.x * CPW + c.slot;
+  const bool valid = (c.slot < CPW) && (cell < g.nwc);
+  float rdx[D];
+#pragma unroll
+  for (int d = 0; d < D; d++) rdx[d] = (float)(1.0 / g.dX[d]);
+  float w[V], q[N][V], rhs[N][V];
+  if (valid) {
+#pragma unroll
+    for (int v = 0; v < V; v++) w[v] = (float)wh[(long)(c.idx * V + v) * g.nwc + cell];
+#pragma unroll
+    for (int t = 0; t < N; t++)
+#pragma unroll
+      for (int v = 0; v < V; v++)
+        q[t][v] = gin ? (float)gin[(long)((t * ND + c.idx) * V + v) * g.nwc + cell] : w[v];
+  }
+  const unsigned long long cellmask =
+      (ND == 64 ? ~0ull : ((1ull << ND) - 1ull)) << (c.slot < CPW ? c.slot * ND : 0);
+  bool active = valid;
+  int it = 0;
+  while (__ballot(active) != 0ull) {
+    if (active) {
+#pragma unroll
+      for (int t = 0; t < N; t++)
+#pragma unroll
+        for (int v = 0; v < V; v++) sm.q[c.slot][t][c.idx][v] = q[t][v];
+    }
+    __syncthreads();
+    if (it == 0 && !gin)
+      node_rhs32<true>(c, sm, q, w, dt, rdx, active, rhs);
+    else
+      node_rhs32<false>(c, sm, q, w, dt, rdx, active, rhs);
+    bool more = false;
+    if (active) {
+#if PDE_STIFF
+      // residual G = rhs - (DG_MAT (x) W) q, then the FP64 Newton correction
+      double qd[N][V], G[N][V];
+      const float wp = (float)c.WP;
+#pragma unroll
+      for (int t = 0; t < N; t++)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          float acc = ((float)T_DGMAT[t][0] * wp) * q[0][v];
+#pragma unroll
+          for (int s2 = 1; s2 < N; s2++) acc = __builtin_fmaf((float)T_DGMAT[t][s2] * wp, q[s2][v], acc);
+          G[t][v] = (double)(rhs[t][v] - acc);
+          qd[t][v] = (double)q[t][v];
+        }
+      (void)newton_node(c, qd, G, dtd);
+#pragma unroll
+      for (int t = 0; t < N; t++)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          const float dq = (float)G[t][v];
+          if (!(__builtin_fabsf(dq) <= tol32 * (1.0f + __builtin_fabsf(q[t][v])))) more = true;
+          q[t][v] = q[t][v] + dq;
+        }
+#else
+      // Picard: q1 = DG_MAT^-1 rhs / prod W
+      const float iwp = (float)c.IWP;
+#pragma unroll
+      for (int t = 0; t < N; t++)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          float acc = (float)T_DGINV[t][0] * rhs[0][v];
+#pragma unroll
+          for (int s2 = 1; s2 < N; s2++) acc = __builtin_fmaf((float)T_DGINV[t][s2], rhs[s2][v], acc);
+          const float q1 = acc * iwp;
+          if (!(__builtin_fabsf(q1 - q[t][v]) <= tol32 * (1.0f + __builtin_fabsf(q[t][v]))))
+            more = true;
+          q[t][v] = q1;
+        }
+#endif
+    }
+    const unsigned long long mb = __ballot(more);
+    if (active) {
+      it++;
+      if ((mb & cellmask) == 0ull || it >= maxit32) active = false;
+    }
+    __syncthreads();
+  }
+  // hand over: the FP32 iterate, or the plain guess where it is not finite
+  bool bad = false;
+  if (valid) {
+#pragma unroll
+    for (int t = 0; t < N; t++)
+#pragma unroll
+      for (int v = 0; v < V; v++)
+        if (!(__builtin_fabsf(q[t][v]) <= 3.4e38f)) bad = true;
+  }
+  const bool cell_bad = (__ballot(bad) & cellmask) != 0ull;
+  if (valid) {
+#pragma unroll
+    for (int t = 0; t < N; t++)
+#pragma unroll
+      for (int v = 0; v < V; v++)
+        gout[(long)((t * ND + c.idx) * V + v) * g.nwc + cell] =
+            cell_bad ? wh[(long)(c.idx * V + v) * g.nwc + cell] : (double)q[t][v];
+  }
+  unsigned long long mine = (valid && c.idx == 0) ? (unsigned long long)it : 0ull;
+  mine = wave_sum_u64(mine);
+  if (lane == 0 && mine) atomicAdd(&st->pstat[blockIdx.x & 63][7], mine);
+}
+#endif
+
 // ------------------------------------------------------------ face flux
 // Rusanov flux and B path integral at every face point (fluxes.cpp:72-116).
 // Faces along d: (nx_d + 1) x prod_{e != d} nx_e, all touching an interior cell.
@@ -1271,6 +1491,7 @@ extern "C" __global__ void k_advance(StepState* st, int ndt) {
     unsigned long long a0 = wave_sum_u64(ps[0]), a1 = wave_sum_u64(ps[1]);
     unsigned long long a2 = wave_sum_u64(ps[2]), a3 = wave_sum_u64(ps[3]);
     unsigned long long a5 = wave_sum_u64(ps[5]), a6 = wave_sum_u64(ps[6]);
+    unsigned long long a7 = wave_sum_u64(ps[7]);
     unsigned long long m4 = ps[4];
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
@@ -1278,7 +1499,7 @@ extern "C" __global__ void k_advance(StepState* st, int ndt) {
       m4 = m4 > y ? m4 : y;
     }
 #pragma unroll
-    for (int k = 0; k < 7; k++) ps[k] = 0ull;
+    for (int k = 0; k < 8; k++) ps[k] = 0ull;
     if (threadIdx.x == 0) {
       st->pred_iters += a0;
       st->pred_cells += a1;
@@ -1286,6 +1507,7 @@ extern "C" __global__ void k_advance(StepState* st, int ndt) {
       st->pred_fallbacks += a3;
       st->pred_nfull += a5;
       st->pred_nshort += a6;
+      st->pred_iters32 += a7;
       if ((unsigned int)m4 > st->pred_maxit) st->pred_maxit = (unsigned int)m4;
     }
   }

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 #include <sys/stat.h>
+#include <utime.h>
 #include <unistd.h>
 
 #include <chrono>
@@ -77,6 +78,7 @@ struct StepState {
   unsigned int pred_maxit, pad3;
   unsigned long long qhead;
   unsigned long long pred_nfull, pred_nshort;
+  unsigned long long pred_iters32;
   // predictor statistics spread over 64 slots, one 128-byte line each, so the
   // wavefronts' atomics do not queue on one address; k_advance folds them into
   // the totals above and clears them ([0] iterations, [1] cells, [2] capped,
@@ -117,6 +119,7 @@ struct Spec {
   int D, N, V, b_quirk;
   int flux = 0;  // 0 Rusanov, 1 Roe, 2 Osher (fluxes.h:6-8)
   bool second = false;  // F depends on the gradient (viscous, secondOrder)
+  bool mixed = false;   // FP32 predictor phase before the FP64 one
 };
 
 std::string full_source(const Spec& s, const pypde::Tables& T) {
@@ -128,12 +131,16 @@ std::string full_source(const Spec& s, const pypde::Tables& T) {
     << "\n";
   o << "#define PDE_STIFF " << (int)s.stiff << "\n#define PDE_B_QUIRK " << s.b_quirk << "\n";
   o << "#define PDE_GUESS " << (int)s.guess << "\n#define PDE_FLUX " << s.flux << "\n";
-  o << "#define PDE_SECOND " << (int)s.second << "\n";
+  o << "#define PDE_SECOND " << (int)s.second << "\n#define PDE_MIXED " << (int)s.mixed << "\n";
   o << "constexpr int ipow_c(int b, int e) { return e <= 0 ? 1 : b * ipow_c(b, e - 1); }\n";
   o << pypde::tables_source(T, s.D);
   o << PRELUDE_SRC << "\n";
   o << "#line 1 \"system\"\n" << s.src << "\n";
-  o << COMMON_SRC << "\n" << KERNELS_SRC << "\n" << STEP_SRC << "\n";
+  o << COMMON_SRC << "\n";
+  if (s.mixed)  // the same physics once more, in FP32, for the mixed predictor's first phase
+    o << REAL32_SRC << "\nnamespace f32 {\n#define double R\n#line 1 \"system (FP32)\"\n" << s.src
+      << "\n#undef double\n}  // namespace f32\n";
+  o << KERNELS_SRC << "\n" << STEP_SRC << "\n";
   return o.str();
 }
 
@@ -178,6 +185,7 @@ int rtc_compile(const std::string& src, const char* what, std::vector<char>* cod
     if (f) {
       std::vector<char> c((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
       if (!c.empty()) {
+        (void)utime(path.c_str(), nullptr);  // marks the entry as in use (the build prunes the rest)
         std::lock_guard<std::mutex> lk(g_mu);
         g_codes[h] = c;
         *code = c;
@@ -246,6 +254,9 @@ int check_spec(const Spec& s, bool secondOrder, int FLUX) {
     return fail(PYPDE_ERR_ARG, "secondOrder needs F (the viscous flux depends on the gradient)");
   if (FLUX < 0 || FLUX > 2)
     return fail(PYPDE_ERR_ARG, "FLUX=%d: 0 Rusanov, 1 Roe or 2 Osher (fluxes.h:6-8)", FLUX);
+  if (s.mixed && (s.useB || secondOrder))
+    return fail(PYPDE_ERR_UNSUPPORTED,
+                "mixed precision: built for F, S (and M) systems without B or a viscous F");
   if (s.V > 8 && (FLUX != 0 || secondOrder))
     return fail(PYPDE_ERR_UNSUPPORTED,
                 "V=%d: above 8 variables only the Rusanov flux and first-order F are built", s.V);
@@ -255,11 +266,12 @@ int check_spec(const Spec& s, bool secondOrder, int FLUX) {
 }
 
 enum { K_WENO, K_DT, K_DTFIN, K_PRED, K_FLUX0, K_FLUX1, K_UPDATE, K_ADVANCE, K_A2S, K_S2A,
-       K_HARVEST, K_MLPGUESS, K_PRED_STATIC, K_PRED_STATIC2, K_EIG, K_COUNT };
+       K_HARVEST, K_MLPGUESS, K_PRED_STATIC, K_PRED_STATIC2, K_EIG, K_PRED32, K_COUNT };
 const char* kernel_names[K_COUNT] = {"k_weno",   "k_dt",    "k_dt_finalize", "k_predictor",
                                      "k_flux0",  "k_flux1", "k_update",      "k_advance",
                                      "k_aos_to_soa", "k_soa_to_aos", "k_harvest", "k_mlp_guess",
-                                     "k_predictor_static", "k_predictor_static2", "k_eig_batch"};
+                                     "k_predictor_static", "k_predictor_static2", "k_eig_batch",
+                                     "k_predictor_f32"};
 
 struct MLPW {
   const float *W1t, *b1, *W2t, *b2, *W3t, *b3;
@@ -305,7 +317,8 @@ int load_module(pypde_ctx* c, const std::vector<char>& code, hipModule_t* mod, h
   if (hipModuleLoadData(mod, code.data()) != hipSuccess) return fail(PYPDE_ERR_HIP, "hipModuleLoadData failed");
   for (int k = 0; k < K_COUNT; k++) {
     fn[k] = nullptr;
-    bool optional = (k == K_FLUX1 && c->D < 2) || k == K_MLPGUESS;
+    bool optional = (k == K_FLUX1 && c->D < 2) || k == K_MLPGUESS ||
+                    (k == K_PRED32 && !c->spec.mixed);
     if (hipModuleGetFunction(&fn[k], *mod, kernel_names[k]) != hipSuccess) {
       fn[k] = nullptr;
       if (!optional) return fail(PYPDE_ERR_HIP, "kernel %s missing", kernel_names[k]);
@@ -329,7 +342,7 @@ long cdiv(long a, long b) { return (a + b - 1) / b; }
 // counters; struct_size says which layout the caller has.
 bool stats_has_nodes(const pypde_stats* s) {
   return s->struct_size <= 0 ||
-         (size_t)s->struct_size >= offsetof(pypde_stats, pred_node_short) + sizeof(long);
+         (size_t)s->struct_size >= offsetof(pypde_stats, pred_iters_f32) + sizeof(long);
 }
 
 // Workgroups of the persistent predictor: as many as can be resident at once
@@ -401,10 +414,22 @@ int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
     std::vector<void*> a = {&w, &st, &m, &gb, &nwc};
     if ((rc = launch(c, K_MLPGUESS, dim3((unsigned)cdiv(g.nwc, 128)), dim3(256), a))) return rc;
   }
+  if (c->spec.mixed) {
+    // FP32 phase: from the learned guess, a caller's guess or the plain one;
+    // its iterate becomes the FP64 predictor's initial guess
+    const double* w = c->wh;
+    const double* gin = c->use_model ? c->gbuf : (guess_now ? c->guess : nullptr);
+    double* gout = c->gbuf;
+    StepState* st = c->st;
+    float tol32 = (float)c->opts.mixed_tol;
+    int maxit32 = c->opts.mixed_maxit;
+    std::vector<void*> a = {&w, &gin, &gout, &st, &g, &tol32, &maxit32};
+    if ((rc = launch(c, K_PRED32, dim3((unsigned)cdiv(g.nwc, c->CPW)), dim3(64), a))) return rc;
+  }
   mark();
   {
     const double* w = c->wh;
-    const double* gs = c->use_model ? c->gbuf : (guess_now ? c->guess : nullptr);
+    const double* gs = (c->use_model || c->spec.mixed) ? c->gbuf : (guess_now ? c->guess : nullptr);
     double *t = c->tr, *v = c->vol, *qh = c->qh;
     StepState* st = c->st;
     double tol = c->opts.newton_tol;
@@ -479,6 +504,8 @@ void pypde_default_options(pypde_options* o) {
   o->struct_size = sizeof *o;
   o->newton_tol = 1e-12;
   o->newton_maxit = 100;
+  o->mixed_tol = 6e-6;
+  o->mixed_maxit = 20;
 }
 
 const char* pypde_hip_last_error(void) { return g_err.c_str(); }
@@ -516,10 +543,14 @@ int pypde_hip_compile(const char* src, bool useF, bool useB, bool useS, int ndim
   make_spec(&s, src, useF, useB, useS, ndim, N, V, STIFF, opts);
   s.flux = FLUX;
   s.second = secondOrder;
+  s.mixed = opts && (size_t)opts->struct_size >= offsetof(pypde_options, mixed_maxit) + sizeof(int) &&
+            opts->mixed;
+  s.guess = s.mixed;
   int rc = check_spec(s, secondOrder, FLUX);
   if (rc) return rc;
   std::vector<char> code;
   if ((rc = compile_code(s, &code))) return rc;
+  if (s.mixed) return 0;
   if (STIFF) {  // the learned-guess variant of the stiff predictor
     s.guess = true;
     if ((rc = compile_code(s, &code))) return rc;
@@ -559,10 +590,13 @@ pypde_ctx* pypde_create(const char* src, bool useF, bool useB, bool useS, int* n
     opts.struct_size = sizeof opts;
     if (!(opts.newton_tol > 0)) opts.newton_tol = 1e-12;
     if (opts.newton_maxit <= 0) opts.newton_maxit = 100;
+    if (!(opts.mixed_tol > 0)) opts.mixed_tol = 6e-6;
+    if (opts.mixed_maxit <= 0) opts.mixed_maxit = 20;
   }
   Spec s;
   make_spec(&s, src, useF, useB, useS, ndim, N, V, STIFF, &opts);
-  s.guess = opts.guess != nullptr;
+  s.mixed = opts.mixed != 0;
+  s.guess = opts.guess != nullptr || s.mixed;
   s.flux = FLUX;
   s.second = secondOrder;
   if ((rc = check_spec(s, secondOrder, FLUX))) return nullptr;
@@ -666,6 +700,7 @@ pypde_ctx* pypde_create(const char* src, bool useF, bool useB, bool useS, int* n
       ok = hipMemcpy(c->guess, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
     }
   }
+  if (ok && s.mixed) ok = alloc(&c->gbuf, g.nwc * c->NT * V);
   ok = ok && hipMalloc((void**)&c->st, sizeof(StepState)) == hipSuccess;
   if (!ok) return bail(fail(PYPDE_ERR_HIP, "device allocation failed"));
   memset(&c->host_st, 0, sizeof c->host_st);
@@ -793,6 +828,7 @@ static int run_loop(pypde_ctx* c, long max_steps, pypde_stats* stats, bool first
     if (stats_has_nodes(stats)) {
       stats->pred_node_full = (long)c->host_st.pred_nfull;
       stats->pred_node_short = (long)c->host_st.pred_nshort;
+      stats->pred_iters_f32 = (long)c->host_st.pred_iters32;
     }
   }
   return c->host_st.nan_step ? -c->host_st.nan_step : 0;
@@ -971,6 +1007,7 @@ int pypde_ensemble_run(pypde_ensemble* e, double tf, long max_steps, pypde_stats
       if (stats_has_nodes(stats)) {
         o->pred_node_full = (long)h.pred_nfull;
         o->pred_node_short = (long)h.pred_nshort;
+        o->pred_iters_f32 = (long)h.pred_iters32;
       }
     }
   }
@@ -1004,7 +1041,7 @@ int pypde_set_guess_model(pypde_ctx* c, const pypde_mlp* m) {
   HIPCHK(hipSetDevice(c->device));
   if (!m) {
     c->use_model = false;
-    memcpy(c->fn, c->spec.guess ? c->fn_guess : c->fn_plain, sizeof c->fn);
+    memcpy(c->fn, c->fn_plain, sizeof c->fn);  // the module the context was created with
     return 0;
   }
   if (!c->spec.stiff) return fail(PYPDE_ERR_ARG, "a guess model needs the stiff (Newton) predictor");

@@ -58,7 +58,7 @@ def _second_order(src, useF):
 def pde_solver(Q0, tf, L, F=None, B=None, S=None, boundaryTypes="transitive", cfl=0.9,
                order=2, ndt=100, flux="rusanov", stiff=True, nThreads=-1, M=None,
                system=None, reflect=None, newton_tol=1e-12, newton_maxit=100, max_steps=0,
-               b_quirk=False, device=0, stats=None):
+               b_quirk=False, device=0, stats=None, mixed=False):
     """Solve dQ/dt + div F(Q) + B(Q).grad Q = S(Q) on [0, L_1] x ... x [0, L_n]
     with the ADER-WENO method on one MI355X.  Returns ``(ndt,) + Q0.shape``.
 
@@ -66,7 +66,8 @@ def pde_solver(Q0, tf, L, F=None, B=None, S=None, boundaryTypes="transitive", cf
     ``reflect`` (ndim x V multipliers for 'reflective' walls), ``newton_tol`` /
     ``newton_maxit`` (stiff predictor), ``max_steps`` (fixed step window),
     ``b_quirk`` (evaluate B.dQ as the reference's build does), ``stats`` (dict
-    filled with run statistics).  ``nThreads`` is accepted and ignored."""
+    filled with run statistics), ``mixed`` (an FP32 predictor phase before the
+    FP64 one).  ``nThreads`` is accepted and ignored."""
     Q0 = np.asarray(Q0)
     nX = np.array(Q0.shape[:-1], dtype=np.int32)
     ndim = len(nX)
@@ -84,7 +85,7 @@ def pde_solver(Q0, tf, L, F=None, B=None, S=None, boundaryTypes="transitive", cf
         reflect = np.ones((ndim, V))
     opts = _capi.make_options(newton_tol=newton_tol, newton_maxit=newton_maxit,
                               max_steps=max_steps, reflect=reflect, b_quirk=int(b_quirk),
-                              use_M=useM, device=device)
+                              use_M=useM, device=device, mixed=mixed)
     st = _capi.Stats()
     st.struct_size = C.sizeof(_capi.Stats)
     rc = _capi.lib().pde_solver_hip(src.encode(), useF, useB, useS, _capi.dp(u), float(tf),
@@ -118,7 +119,8 @@ class Solver:
 
     def __init__(self, u0, L, system, ndim=None, order=3, boundaryTypes="transitive", cfl=0.9,
                  stiff=False, reflect=None, newton_tol=1e-12, newton_maxit=100, b_quirk=False,
-                 profile=False, device=0, guess=None, use_graph=False, flux="rusanov"):
+                 profile=False, device=0, guess=None, use_graph=False, flux="rusanov",
+                 mixed=False, mixed_tol=None, mixed_maxit=None):
         u0 = np.ascontiguousarray(u0, dtype=np.float64)
         self.shape = u0.shape
         self.nX = np.array(u0.shape[:-1], dtype=np.int32)
@@ -132,7 +134,8 @@ class Solver:
         self.opts = _capi.make_options(newton_tol=newton_tol, newton_maxit=newton_maxit,
                                        reflect=reflect, b_quirk=int(b_quirk), use_M=useM,
                                        profile=profile, device=device, guess=guess,
-                                       use_graph=use_graph)
+                                       use_graph=use_graph, mixed=mixed, mixed_tol=mixed_tol,
+                                       mixed_maxit=mixed_maxit)
         rc = C.c_int(0)
         self.h = _capi.lib().pypde_create(src.encode(), useF, useB, useS, _capi.ip(self.nX), nd,
                                           _capi.dp(self.dX), float(cfl), _capi.ip(self.bcs),

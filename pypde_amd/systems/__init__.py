@@ -57,6 +57,14 @@ def standard_configs(full=False):
         s = SYSTEMS[name]
         out.append(dict(src=source(name), useF=s["F"], useB=s["B"], useS=s["S"], ndim=ndim, N=N,
                         V=s["V"], stiff=False, use_M=s["M"], flux=0, second=True))
+    # mixed precision (FP32 predictor phase): the reactive channel, the 2D Euler
+    # configurations and Sod
+    for name, ndim, N, stiff in ([("reactive2d", 2, 3, True), ("euler2d", 2, 3, False),
+                                  ("euler2d", 2, 4, False), ("euler1d", 1, 3, False)]
+                                 if full else []):
+        s = SYSTEMS[name]
+        out.append(dict(src=source(name), useF=s["F"], useB=s["B"], useS=s["S"], ndim=ndim, N=N,
+                        V=s["V"], stiff=stiff, use_M=s["M"], flux=0, mixed=True))
     # 17-variable GPR (1D, order 2, the reference's B.dQ evaluation); each takes
     # a few minutes to compile
     for stiff in ((False, True) if full else ()):

@@ -235,3 +235,17 @@ class quiet_stdout:
         os.dup2(self.saved, 1)
         os.close(self.null)
         os.close(self.saved)
+
+
+def density_wave_errors(u, n, t):
+    """Density errors of the advected wave (systems.density_wave) at time t:
+    L1 over the grid, L1 over the bands of phase within 0.05 of the zero
+    crossings (farthest from the extrema), and L_inf."""
+    from pypde_amd import systems
+    ex = systems.density_wave(n, 2, t=t)
+    e = np.abs(u[..., 0] - ex[..., 0])
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    ph = (X + Y - 2 * 0.5 * t) % 1.0        # rho - 1 = 0.2 sin(2 pi ph)
+    band = (np.abs(ph - 0.5) < 0.05) | (ph < 0.05) | (ph > 0.95)
+    return float(e.mean()), float(e[band].mean()), float(e.max())
