@@ -96,6 +96,21 @@ int weno_solver(double *ret, double *u, int *nX, int ndim, int N, int V);
 
 typedef struct pypde_ctx pypde_ctx;
 
+/* Allocation-free variant of weno_solver, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  u and ret are
+ * device pointers in weno_solver's layouts (u: prod(nX) x V, ret:
+ * prod(nX - 2(N-1)) x N^ndim x V); ndim, N and V are the context's, whose
+ * loaded kernels it launches.  Its staging (u as planes, the reconstruction
+ * planes) lives in one 64 MiB device array per GPU, held until the process
+ * ends: 8 (V prod(nX) + N^ndim V prod(nX - 2(N-1))) bytes (the first part
+ * rounded up to 256 bytes) must fit, else PYPDE_ERR_ARG.  It calls no
+ * hipMalloc or hipFree.  With stream NULL it runs on the context's stream and
+ * returns when done; otherwise it only enqueues on `stream`.  Every call on
+ * one GPU shares that array: calls that overlap on one GPU race. */
+int weno_solver_noalloc(pypde_ctx *c, double *ret, const double *u, const int *nX,
+                        void *stream);
+
 /* Persistent solver context: compile once, keep the state resident in HBM. */
 pypde_ctx *pypde_create(const char *src, bool useF, bool useB, bool useS, int *nX, int ndim,
                         double *dX, double CFL, int *boundaryTypes, bool STIFF, int FLUX, int N,

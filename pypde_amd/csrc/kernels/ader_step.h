@@ -1539,6 +1539,17 @@ extern "C" __global__ void k_soa_to_aos(const double* __restrict__ s, double* __
   a[i] = s[(long)v * ncell + c];
 }
 
+// Reconstruction planes [node * V + v][cell] -> the reference's weno_solver
+// layout [cell][node][v] (api.cpp:32-48).
+extern "C" __global__ void k_planes_to_rows(const double* __restrict__ s, double* __restrict__ a,
+                                            long ncell) {
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= ncell * ND * V) return;
+  long c = i / (ND * V);
+  int p = (int)(i % (ND * V));
+  a[i] = s[(long)p * ncell + c];
+}
+
 // ------------------------------------------------ learned Newton guess (MLP)
 // Training pairs: features x = (wh of the cell (N^D * V values), dt), targets
 // y = (qh - w broadcast in time) / dt (N^(D+1) * V values), written as fp32

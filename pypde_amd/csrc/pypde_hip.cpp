@@ -266,12 +266,12 @@ int check_spec(const Spec& s, bool secondOrder, int FLUX) {
 }
 
 enum { K_WENO, K_DT, K_DTFIN, K_PRED, K_FLUX0, K_FLUX1, K_UPDATE, K_ADVANCE, K_A2S, K_S2A,
-       K_HARVEST, K_MLPGUESS, K_PRED_STATIC, K_PRED_STATIC2, K_EIG, K_PRED32, K_COUNT };
+       K_HARVEST, K_MLPGUESS, K_PRED_STATIC, K_PRED_STATIC2, K_EIG, K_P2R, K_PRED32, K_COUNT };
 const char* kernel_names[K_COUNT] = {"k_weno",   "k_dt",    "k_dt_finalize", "k_predictor",
                                      "k_flux0",  "k_flux1", "k_update",      "k_advance",
                                      "k_aos_to_soa", "k_soa_to_aos", "k_harvest", "k_mlp_guess",
                                      "k_predictor_static", "k_predictor_static2", "k_eig_batch",
-                                     "k_predictor_f32"};
+                                     "k_planes_to_rows", "k_predictor_f32"};
 
 struct MLPW {
   const float *W1t, *b1, *W2t, *b2, *W3t, *b3;
@@ -1302,6 +1302,112 @@ int weno_solver(double* ret, double* u, int* nX, int ndim, int N, int V) {
   // planes [node*V+v][cell] -> reference layout [cell][node][v]
   for (long cc = 0; cc < nout; cc++)
     for (int p = 0; p < ND * V; p++) ret[cc * ND * V + p] = h[(size_t)p * nout + cc];
+  return 0;
+}
+
+// ------------------------------------------------ allocation-free WENO
+// Device scratch of weno_solver_noalloc: one array per GPU, held until the
+// process ends.  Host code never passes `scratch` itself to a kernel (in host
+// code the name stands for a host-side shadow); it uses the per-device
+// address that hipGetSymbolAddress returns.
+}  // extern "C"
+
+constexpr size_t NOALLOC_SCRATCH_BYTES = 64ull << 20;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The scratch array's device address on `dev`, looked up once per device
+// with that device current.
+int scratch_address(int dev, double** out) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(PYPDE_ERR_ARG, "device %d", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = -1;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(hipSetDevice(dev));
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      return fail(PYPDE_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                  hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+bool is_device_pointer(const void* p) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice;
+}
+}  // namespace
+
+extern "C" {
+
+int weno_solver_noalloc(pypde_ctx* c, double* ret, const double* u, const int* nX, void* stream) {
+  if (!c || !ret || !u || !nX) return fail(PYPDE_ERR_ARG, "weno_solver_noalloc: null argument");
+  const int ndim = c->D, N = c->N, V = c->V;
+  long nin = 1, nout = 1;
+  int no[2] = {1, 1};
+  for (int d = 0; d < ndim; d++) {
+    no[d] = nX[d] - 2 * (N - 1);
+    if (nX[d] < 1 || no[d] < 1)
+      return fail(PYPDE_ERR_ARG, "weno_solver_noalloc: nX[%d]=%d too small for N=%d", d, nX[d], N);
+    nin *= nX[d];
+    nout *= no[d];
+  }
+  const size_t in_doubles = ((size_t)nin * V + 31) / 32 * 32;  // 256-byte aligned split
+  const size_t need = (in_doubles + (size_t)nout * c->ND * V) * sizeof(double);
+  if (need > NOALLOC_SCRATCH_BYTES)
+    return fail(PYPDE_ERR_ARG,
+                "weno_solver_noalloc: the grid needs %zu bytes of scratch (input %ld x %d "
+                "doubles, output %ld x %d doubles); the limit is %zu bytes",
+                need, nin, V, nout, c->ND * V, (size_t)NOALLOC_SCRATCH_BYTES);
+  HIPCHK(hipSetDevice(c->device));
+  if (!is_device_pointer(u) || !is_device_pointer(ret))
+    return fail(PYPDE_ERR_ARG, "weno_solver_noalloc: u and ret must be device memory");
+  double* sp = nullptr;
+  int rc = scratch_address(c->device, &sp);
+  if (rc) return rc;
+  double* s_in = sp;               // u as planes [v][cell]
+  double* s_out = sp + in_doubles;  // reconstruction planes [node * V + v][cell]
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  auto go = [&](int k, unsigned gx, unsigned gy, unsigned gz, void** a) -> int {
+    if (!c->fn[k]) return fail(PYPDE_ERR_HIP, "kernel %s not loaded", kernel_names[k]);
+    hipError_t e = hipModuleLaunchKernel(c->fn[k], gx, gy, gz, 256, 1, 1, 0, s, a, nullptr);
+    if (e != hipSuccess)
+      return fail(PYPDE_ERR_HIP, "launch %s: %s", kernel_names[k], hipGetErrorString(e));
+    return 0;
+  };
+  {
+    void* a[] = {&u, &s_in, &nin};
+    if ((rc = go(K_A2S, (unsigned)cdiv(nin * V, 256), 1, 1, a))) return rc;
+  }
+  {
+    Geom g;
+    memset(&g, 0, sizeof g);
+    const StepState* st = nullptr;
+    int ghost = 0, n0 = nX[0], n1 = ndim > 1 ? nX[1] : 1;
+    void* a[] = {&s_in, &s_out, &st, &g, &ghost, &n0, &n1, &nout};
+    unsigned gx = ndim == 1 ? (unsigned)cdiv(no[0], 256) : (unsigned)cdiv(no[1], 32);
+    unsigned gy = ndim == 1 ? 1 : (unsigned)cdiv(no[0], 8);
+    unsigned gz = ndim == 1 ? 1 : (unsigned)V;
+    if ((rc = go(K_WENO, gx, gy, gz, a))) return rc;
+  }
+  {
+    void* a[] = {&s_out, &ret, &nout};
+    if ((rc = go(K_P2R, (unsigned)cdiv(nout * c->ND * V, 256), 1, 1, a))) return rc;
+  }
+  if (!stream) HIPCHK(hipStreamSynchronize(s));
   return 0;
 }
 
