@@ -162,6 +162,18 @@ int pypde_set_guess_model(pypde_ctx *c, const pypde_mlp *m);
  * X (V N^ndim + 1 rows: wh, dt) and Y (V N^(ndim+1) rows: (qh - w) / dt), leading
  * dimension ldx. */
 int pypde_harvest_step(pypde_ctx *c, float *X, float *Y, long ldx, long col0, pypde_stats *stats);
+/* Autotuner (no counterpart in the reference, whose only partition knob is its
+ * thread-slab split, iterator.cpp:68-84): before its first step a context
+ * times every launch variant of its tunable kernels in real trial steps
+ * (state restored afterwards) and keeps the fastest, or reuses the choice
+ * recorded for the same problem in <code-object cache>/tuning.txt.
+ * choices[5]: WENO tile (0 8x32, 1 4x64, 2 16x16, 3 16x32), flux workgroup (0
+ * 256, 1 128, 2 64), corrector (0 one thread per cell, 1..4 one variable and
+ * 1, 2, 4, 8 x-rows per thread), predictor (0 static, 1 work queue), source (0
+ * defaults, 1 measured, 2 tuning table, 3 PYPDE_TUNE_FORCE); ms[4]: the chosen
+ * WENO, predictor, flux (both directions) and corrector trial times. */
+int pypde_get_tuning(pypde_ctx *c, int *choices, double *ms);
+
 /* Host copy of the last inferred guesses, nwc x N^(ndim+1) x V. */
 int pypde_last_guess(pypde_ctx *c, double *out);
 int pypde_geometry(pypde_ctx *c, long *nwc, int *n_in, int *n_out);

@@ -65,6 +65,7 @@ def lib():
                                      C.POINTER(Stats)]
         L.weno_solver.argtypes = [PD, PD, PI, C.c_int, C.c_int, C.c_int]
         L.weno_solver_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, PI, C.c_void_p]
+        L.pypde_get_tuning.argtypes = [C.c_void_p, PI, PD]
         L.pypde_create.restype = C.c_void_p
         L.pypde_create.argtypes = [C.c_char_p, C.c_bool, C.c_bool, C.c_bool, PI, C.c_int, PD,
                                    C.c_double, PI, C.c_bool, C.c_int, C.c_int, C.c_int, C.c_bool,

@@ -69,102 +69,114 @@ DEV unsigned int wave_max_u32(unsigned int x) {
 // Boundary ghosts fused into the load, both sweeps staged in LDS
 // (boundaries.cpp:25-53, weno.cpp:61-121).  ghost = false gives the
 // reference's weno_solver semantics (no ghost extension).
-#ifndef WENO_TX
-#define WENO_TX 8
-#endif
-#ifndef WENO_TY
-#define WENO_TY 32
-#endif
-
-extern "C" __global__ void __launch_bounds__(256)
-k_weno(const double* __restrict__ u, double* __restrict__ wh, const StepState* __restrict__ st,
-       Geom g, int ghost, int nin0, int nin1, long out_stride) {
-  if (st && st->done) return;
+// 2D: one variable per workgroup (blockIdx.z), a TX x TY tile of output cells
+// per workgroup (256 threads); sweep 1 along axis 0 over the TX + 2N - 2 rows
+// and TY + 2N - 2 columns the tile needs, then sweep 2 along axis 1 for every
+// coefficient index k0.  The autotuner picks the tile per problem.
+template <int TX, int TY>
+DEV void weno_2d(const double* __restrict__ u, double* __restrict__ wh, const Geom& g, int ghost,
+                 int nin0, int nin1, long out_stride) {
   constexpr int H = 2 * N - 2;
   const int tid = threadIdx.x;
-  if constexpr (D == 1) {
-    constexpr int TX = 256;
-    constexpr int RX = TX + H;
-    __shared__ double s0[V][RX];
-    const int i0 = blockIdx.x * TX;
-    const int nb0 = ghost ? g.nb[0] : nin0;
-    const int no0 = nb0 - H;
-    for (int e = tid; e < V * RX; e += 256) {
-      int v = e / RX, r = e % RX;
-      int I = i0 + r;
-      double val = 0.0;
-      if (I < nb0) {
-        bool f0 = false;
-        int s = ghost ? ghost_src(I, g.bc[0], g.nx[0], &f0) : I;
-        double sg = 1.0;
-        if (f0) sg = sg * g.refl[0][v];
-        val = u[(long)v * (ghost ? g.ncell : nin0) + s] * sg;
-      }
-      s0[v][r] = val;
+  constexpr int RX = TX + H, RY = TY + H, RYP = RY + 1;
+  __shared__ double s0[RX][RYP];
+  __shared__ double s1[N][TX][RYP];
+  const int v = blockIdx.z;
+  const int i0 = blockIdx.y * TX, j0 = blockIdx.x * TY;
+  const int nb0 = ghost ? g.nb[0] : nin0, nb1 = ghost ? g.nb[1] : nin1;
+  const int no0 = nb0 - H, no1 = nb1 - H;
+  const long in_stride = ghost ? g.ncell : (long)nin0 * nin1;
+  const int in_row = ghost ? g.nx[1] : nin1;
+  for (int e = tid; e < RX * RY; e += 256) {
+    int r = e / RY, c = e % RY;
+    int I = i0 + r, J = j0 + c;
+    double val = 0.0;
+    if (I < nb0 && J < nb1) {
+      bool f0 = false, f1 = false;
+      int s0i = ghost ? ghost_src(I, g.bc[0], g.nx[0], &f0) : I;
+      int s1i = ghost ? ghost_src(J, g.bc[1], g.nx[1], &f1) : J;
+      double sg = 1.0;
+      if (f0) sg = sg * g.refl[0][v];
+      if (f1) sg = sg * g.refl[1][v];
+      val = u[(long)v * in_stride + (long)s0i * in_row + s1i] * sg;
     }
-    __syncthreads();
-    for (int e = tid; e < V * TX; e += 256) {
-      int v = e / TX, r = e % TX;
-      int i = i0 + r;
-      if (i < no0) {
-        double out[N];
-        weno_line([&](int m) { return s0[v][r + m]; }, out);
+    s0[r][c] = val;
+  }
+  __syncthreads();
+  for (int e = tid; e < TX * RY; e += 256) {
+    int r = e / RY, c = e % RY;
+    if (i0 + r < no0) {
+      double out[N];
+      weno_line([&](int m) { return s0[r + m][c]; }, out);
 #pragma unroll
-        for (int k = 0; k < N; k++) wh[(long)(k * V + v) * out_stride + i] = out[k];
-      }
+      for (int k = 0; k < N; k++) s1[k][r][c] = out[k];
     }
-  } else {
-    // one variable per workgroup (blockIdx.z): 10.6 KB of LDS at N = 3, so
-    // many workgroups per CU hide the stencil's latency
-    constexpr int TX = WENO_TX, TY = WENO_TY;
-    constexpr int RX = TX + H, RY = TY + H, RYP = RY + 1;
-    __shared__ double s0[RX][RYP];
-    __shared__ double s1[N][TX][RYP];
-    const int v = blockIdx.z;
-    const int i0 = blockIdx.y * TX, j0 = blockIdx.x * TY;
-    const int nb0 = ghost ? g.nb[0] : nin0, nb1 = ghost ? g.nb[1] : nin1;
-    const int no0 = nb0 - H, no1 = nb1 - H;
-    const long in_stride = ghost ? g.ncell : (long)nin0 * nin1;
-    const int in_row = ghost ? g.nx[1] : nin1;
-    for (int e = tid; e < RX * RY; e += 256) {
-      int r = e / RY, c = e % RY;
-      int I = i0 + r, J = j0 + c;
-      double val = 0.0;
-      if (I < nb0 && J < nb1) {
-        bool f0 = false, f1 = false;
-        int s0i = ghost ? ghost_src(I, g.bc[0], g.nx[0], &f0) : I;
-        int s1i = ghost ? ghost_src(J, g.bc[1], g.nx[1], &f1) : J;
-        double sg = 1.0;
-        if (f0) sg = sg * g.refl[0][v];
-        if (f1) sg = sg * g.refl[1][v];
-        val = u[(long)v * in_stride + (long)s0i * in_row + s1i] * sg;
-      }
-      s0[r][c] = val;
-    }
-    __syncthreads();
-    for (int e = tid; e < TX * RY; e += 256) {
-      int r = e / RY, c = e % RY;
-      if (i0 + r < no0) {
-        double out[N];
-        weno_line([&](int m) { return s0[r + m][c]; }, out);
+  }
+  __syncthreads();
+  for (int e = tid; e < N * TX * TY; e += 256) {
+    int c = e % TY, r = (e / TY) % TX, k0 = e / (TY * TX);
+    int i = i0 + r, j = j0 + c;
+    if (i < no0 && j < no1) {
+      double out[N];
+      weno_line([&](int m) { return s1[k0][r][c + m]; }, out);
+      long cell = (long)i * no1 + j;
 #pragma unroll
-        for (int k = 0; k < N; k++) s1[k][r][c] = out[k];
-      }
-    }
-    __syncthreads();
-    for (int e = tid; e < N * TX * TY; e += 256) {
-      int c = e % TY, r = (e / TY) % TX, k0 = e / (TY * TX);
-      int i = i0 + r, j = j0 + c;
-      if (i < no0 && j < no1) {
-        double out[N];
-        weno_line([&](int m) { return s1[k0][r][c + m]; }, out);
-        long cell = (long)i * no1 + j;
-#pragma unroll
-        for (int k1 = 0; k1 < N; k1++) wh[(long)((k0 * N + k1) * V + v) * out_stride + cell] = out[k1];
-      }
+      for (int k1 = 0; k1 < N; k1++) wh[(long)((k0 * N + k1) * V + v) * out_stride + cell] = out[k1];
     }
   }
 }
+
+// 1D: 256 output cells per workgroup, all variables.
+DEV void weno_1d(const double* __restrict__ u, double* __restrict__ wh, const Geom& g, int ghost,
+                 int nin0, long out_stride) {
+  constexpr int H = 2 * N - 2;
+  const int tid = threadIdx.x;
+  constexpr int TX = 256;
+  constexpr int RX = TX + H;
+  __shared__ double s0[V][RX];
+  const int i0 = blockIdx.x * TX;
+  const int nb0 = ghost ? g.nb[0] : nin0;
+  const int no0 = nb0 - H;
+  for (int e = tid; e < V * RX; e += 256) {
+    int v = e / RX, r = e % RX;
+    int I = i0 + r;
+    double val = 0.0;
+    if (I < nb0) {
+      bool f0 = false;
+      int s = ghost ? ghost_src(I, g.bc[0], g.nx[0], &f0) : I;
+      double sg = 1.0;
+      if (f0) sg = sg * g.refl[0][v];
+      val = u[(long)v * (ghost ? g.ncell : nin0) + s] * sg;
+    }
+    s0[v][r] = val;
+  }
+  __syncthreads();
+  for (int e = tid; e < V * TX; e += 256) {
+    int v = e / TX, r = e % TX;
+    int i = i0 + r;
+    if (i < no0) {
+      double out[N];
+      weno_line([&](int m) { return s0[v][r + m]; }, out);
+#pragma unroll
+      for (int k = 0; k < N; k++) wh[(long)(k * V + v) * out_stride + i] = out[k];
+    }
+  }
+}
+
+#define PYPDE_WENO_KERNEL(NAME, TX, TY)                                                        \
+  extern "C" __global__ void __launch_bounds__(256)                                            \
+  NAME(const double* __restrict__ u, double* __restrict__ wh, const StepState* __restrict__ st, \
+       Geom g, int ghost, int nin0, int nin1, long out_stride) {                               \
+    if (st && st->done) return;                                                                \
+    if constexpr (D == 1)                                                                      \
+      weno_1d(u, wh, g, ghost, nin0, out_stride);                                              \
+    else                                                                                       \
+      weno_2d<TX, TY>(u, wh, g, ghost, nin0, nin1, out_stride);                                \
+  }
+PYPDE_WENO_KERNEL(k_weno, 8, 32)
+PYPDE_WENO_KERNEL(k_weno_4x64, 4, 64)
+PYPDE_WENO_KERNEL(k_weno_16x16, 16, 16)
+PYPDE_WENO_KERNEL(k_weno_16x32, 16, 32)
 
 // ------------------------------------------------------------ timestep
 // Largest sum_d lambda_d / dX_d over every reconstructed cell

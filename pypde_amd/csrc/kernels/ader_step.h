@@ -1263,11 +1263,11 @@ DEV void face_cells(const Geom& g, long f, long* wl, long* wr) {
 // (fluxes.cpp:72-101), Roe or Osher (fluxes.cpp:20-70), the viscous term
 // (fluxes.cpp:96-100) and the non-conservative path integral Bint
 // (fluxes.cpp:103-116), from the two cells' face traces.
-template <int d>
+template <int d, int BS = 256>
 DEV void flux_dir(const double* __restrict__ tr, double* __restrict__ ff, double* __restrict__ fb,
                   const StepState* st, const Geom& g, const double* __restrict__ trd) {
   const long nf = g.nf[d];
-  const long item = (long)blockIdx.x * 256 + threadIdx.x;
+  const long item = (long)blockIdx.x * BS + threadIdx.x;
   if (item >= nf * N * ND1) return;
   const long f = item % nf;
   const int j = (int)((item / nf) % ND1);
@@ -1374,19 +1374,22 @@ DEV void flux_dir(const double* __restrict__ tr, double* __restrict__ ff, double
 #endif
 }
 
-extern "C" __global__ void __launch_bounds__(256)
-k_flux0(const double* __restrict__ tr, double* __restrict__ ff, double* __restrict__ fb,
-        const StepState* st, Geom g, const double* __restrict__ trd) {
-  if (st->done) return;
-  flux_dir<0>(tr, ff, fb, st, g, trd);
-}
+// Workgroup sizes 256 (k_flux0, k_flux1), 128 and 64 (..._128, ..._64): the
+// autotuner picks one per problem; every face point's arithmetic is the same.
+#define PYPDE_FLUX_KERNEL(NAME, DIR, BS)                                                       \
+  extern "C" __global__ void __launch_bounds__(BS)                                             \
+  NAME(const double* __restrict__ tr, double* __restrict__ ff, double* __restrict__ fb,        \
+       const StepState* st, Geom g, const double* __restrict__ trd) {                          \
+    if (st->done) return;                                                                      \
+    flux_dir<DIR, BS>(tr, ff, fb, st, g, trd);                                                 \
+  }
+PYPDE_FLUX_KERNEL(k_flux0, 0, 256)
+PYPDE_FLUX_KERNEL(k_flux0_128, 0, 128)
+PYPDE_FLUX_KERNEL(k_flux0_64, 0, 64)
 #if PDE_D >= 2
-extern "C" __global__ void __launch_bounds__(256)
-k_flux1(const double* __restrict__ tr, double* __restrict__ ff, double* __restrict__ fb,
-        const StepState* st, Geom g, const double* __restrict__ trd) {
-  if (st->done) return;
-  flux_dir<1>(tr, ff, fb, st, g, trd);
-}
+PYPDE_FLUX_KERNEL(k_flux1, 1, 256)
+PYPDE_FLUX_KERNEL(k_flux1_128, 1, 128)
+PYPDE_FLUX_KERNEL(k_flux1_64, 1, 64)
 #endif
 
 // ------------------------------------------------------------- corrector
@@ -1481,6 +1484,134 @@ k_update(double* __restrict__ u, const double* __restrict__ vol, FaceBufs fbufs,
     for (int v = 0; v < V; v++) ret[((long)push * g.ncell + c) * V + v] = uc[v];
   }
 }
+
+// The corrector with one variable per thread and R consecutive x-rows per
+// thread (thread = (v, row group, y)): each x-face flux row between two of
+// its cells is read once and kept in registers for the next row, so the face
+// fluxes are read close to once (x faces (R+1)/R times).  Every cell and
+// variable accumulates the same terms in the same order as k_update, so the
+// results are identical; the autotuner picks R per problem.
+template <int R>
+DEV void update_rows(double* __restrict__ u, const double* __restrict__ vol, const FaceBufs& fbufs,
+                     double* __restrict__ ret, StepState* st, const Geom& g) {
+  if (st->done) return;
+  const int nx = g.nx[0], ny = g.nx[1];
+  const int nxg = (nx + R - 1) / R;
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)V * nxg * ny) return;
+  const int cy = (int)(idx % ny);
+  const long rest = idx / ny;
+  const int xg = (int)(rest % nxg);
+  const int v = (int)(rest / nxg);
+  const double dt = st->dt;
+  const double tn = st->t + dt;
+  const int push = st->push;
+  const bool snap = ret && push < g.ndt && tn >= (double)(push + 1) / (double)g.ndt * st->tf;
+  double c0[D][N];  // (dt W_t) / (2 dX_d)
+#pragma unroll
+  for (int d = 0; d < D; d++)
+#pragma unroll
+    for (int t = 0; t < N; t++) c0[d][t] = (dt * T_W[t]) / (2.0 * g.dX[d]);
+  auto coef = [&](int d, int t, int j) {
+    double tmp = c0[d][t];
+    if (D == 2) tmp = tmp * T_W[j];
+    return tmp;
+  };
+  auto plane = [&](int t, int j) { return (long)((t * ND1 + j) * V + v); };
+  const long nf0 = g.nf[0];
+  const int cx0 = xg * R;
+  // left x face of the first row: f = cx0 * ny + cy
+  double xlf[N][ND1], xlb[N][ND1];
+#pragma unroll
+  for (int t = 0; t < N; t++)
+#pragma unroll
+    for (int j = 0; j < ND1; j++) {
+      const long f = (long)cx0 * ny + cy;
+      xlf[t][j] = PDE_USE_F ? fbufs.ff[0][plane(t, j) * nf0 + f] : 0.0;
+      xlb[t][j] = PDE_USE_B ? fbufs.fb[0][plane(t, j) * nf0 + f] : 0.0;
+    }
+  bool bad = false;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int cx = cx0 + r;
+    if (cx >= nx) break;
+    const long c = (long)cx * ny + cy;
+    double uc = u[(long)v * g.ncell + c];
+#if PDE_USE_S || PDE_USE_B
+    {
+      const long wc = (D == 1) ? (long)(cx + 1) : (long)(cx + 1) * g.nw[1] + (cy + 1);
+      uc = uc + vol[(long)v * g.nwc + wc];
+    }
+#endif
+    double xrf[N][ND1], xrb[N][ND1];
+#pragma unroll
+    for (int t = 0; t < N; t++)
+#pragma unroll
+      for (int j = 0; j < ND1; j++) {
+        const long f = (long)(cx + 1) * ny + cy;
+        xrf[t][j] = PDE_USE_F ? fbufs.ff[0][plane(t, j) * nf0 + f] : 0.0;
+        xrb[t][j] = PDE_USE_B ? fbufs.fb[0][plane(t, j) * nf0 + f] : 0.0;
+      }
+#if PDE_USE_F || PDE_USE_B
+    // faces c + 1 - e_d (c on the right), d ascending, over t then j
+#pragma unroll
+    for (int t = 0; t < N; t++)
+#pragma unroll
+      for (int j = 0; j < ND1; j++) uc = uc - coef(0, t, j) * (xlb[t][j] - xlf[t][j]);
+    if constexpr (D == 2) {
+      const long nf1 = g.nf[1];
+      const long fb0 = (long)cx * (ny + 1) + cy;
+#pragma unroll
+      for (int t = 0; t < N; t++)
+#pragma unroll
+        for (int j = 0; j < ND1; j++) {
+          const double f = PDE_USE_F ? fbufs.ff[1][plane(t, j) * nf1 + fb0] : 0.0;
+          const double b = PDE_USE_B ? fbufs.fb[1][plane(t, j) * nf1 + fb0] : 0.0;
+          uc = uc - coef(1, t, j) * (b - f);
+        }
+      // face-cell c + 1 (c on the left): over t, then d, then j
+#pragma unroll
+      for (int t = 0; t < N; t++) {
+#pragma unroll
+        for (int j = 0; j < ND1; j++) uc = uc - coef(0, t, j) * (xrb[t][j] + xrf[t][j]);
+#pragma unroll
+        for (int j = 0; j < ND1; j++) {
+          const double f = PDE_USE_F ? fbufs.ff[1][plane(t, j) * nf1 + fb0 + 1] : 0.0;
+          const double b = PDE_USE_B ? fbufs.fb[1][plane(t, j) * nf1 + fb0 + 1] : 0.0;
+          uc = uc - coef(1, t, j) * (b + f);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < N; t++)
+#pragma unroll
+        for (int j = 0; j < ND1; j++) uc = uc - coef(0, t, j) * (xrb[t][j] + xrf[t][j]);
+    }
+#endif
+    u[(long)v * g.ncell + c] = uc;
+    if (uc != uc) bad = true;
+    if (snap) ret[((long)push * g.ncell + c) * V + v] = uc;
+#pragma unroll
+    for (int t = 0; t < N; t++)
+#pragma unroll
+      for (int j = 0; j < ND1; j++) {
+        xlf[t][j] = xrf[t][j];
+        xlb[t][j] = xrb[t][j];
+      }
+  }
+  if (bad) atomicOr(&st->nan_flag, 1u);
+}
+
+#define PYPDE_UPDATE_ROWS(R_)                                                                  \
+  extern "C" __global__ void __launch_bounds__(256)                                           \
+  k_update_r##R_(double* __restrict__ u, const double* __restrict__ vol, FaceBufs fbufs,      \
+                 double* __restrict__ ret, StepState* st, Geom g) {                            \
+    update_rows<R_>(u, vol, fbufs, ret, st, g);                                                \
+  }
+PYPDE_UPDATE_ROWS(1)
+PYPDE_UPDATE_ROWS(2)
+PYPDE_UPDATE_ROWS(4)
+PYPDE_UPDATE_ROWS(8)
 
 // t += dt; count++; snapshot row; NaN and end-of-run checks (iterator.cpp:129-147).
 extern "C" __global__ void k_advance(StepState* st, int ndt) {

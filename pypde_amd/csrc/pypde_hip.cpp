@@ -13,6 +13,7 @@
 #include <utime.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -266,12 +267,30 @@ int check_spec(const Spec& s, bool secondOrder, int FLUX) {
 }
 
 enum { K_WENO, K_DT, K_DTFIN, K_PRED, K_FLUX0, K_FLUX1, K_UPDATE, K_ADVANCE, K_A2S, K_S2A,
-       K_HARVEST, K_MLPGUESS, K_PRED_STATIC, K_PRED_STATIC2, K_EIG, K_P2R, K_PRED32, K_COUNT };
+       K_HARVEST, K_MLPGUESS, K_PRED_STATIC, K_PRED_STATIC2, K_EIG, K_P2R, K_PRED32,
+       K_WENO_4x64, K_WENO_16x16, K_WENO_16x32, K_FLUX0_128, K_FLUX0_64, K_FLUX1_128, K_FLUX1_64,
+       K_UPD1, K_UPD2, K_UPD4, K_UPD8, K_COUNT };
 const char* kernel_names[K_COUNT] = {"k_weno",   "k_dt",    "k_dt_finalize", "k_predictor",
                                      "k_flux0",  "k_flux1", "k_update",      "k_advance",
                                      "k_aos_to_soa", "k_soa_to_aos", "k_harvest", "k_mlp_guess",
                                      "k_predictor_static", "k_predictor_static2", "k_eig_batch",
-                                     "k_planes_to_rows", "k_predictor_f32"};
+                                     "k_planes_to_rows", "k_predictor_f32",
+                                     "k_weno_4x64", "k_weno_16x16", "k_weno_16x32",
+                                     "k_flux0_128", "k_flux0_64", "k_flux1_128", "k_flux1_64",
+                                     "k_update_r1", "k_update_r2", "k_update_r4", "k_update_r8"};
+
+// Launch variants the autotuner chooses between (every variant of a kernel
+// computes identical results): WENO tiles, flux workgroup sizes, corrector
+// mappings (one thread per cell and all variables, or one variable and R
+// x-rows per thread), predictor schedules (static, work queue).
+constexpr int N_WENO_V = 4, N_FLUX_V = 3, N_UPD_V = 5, N_PRED_V = 2;
+const int WENO_TILE[N_WENO_V][2] = {{8, 32}, {4, 64}, {16, 16}, {16, 32}};
+const int WENO_KID[N_WENO_V] = {K_WENO, K_WENO_4x64, K_WENO_16x16, K_WENO_16x32};
+const int FLUX_BS[N_FLUX_V] = {256, 128, 64};
+const int FLUX_KID[2][N_FLUX_V] = {{K_FLUX0, K_FLUX0_128, K_FLUX0_64},
+                                   {K_FLUX1, K_FLUX1_128, K_FLUX1_64}};
+const int UPD_KID[N_UPD_V] = {K_UPDATE, K_UPD1, K_UPD2, K_UPD4, K_UPD8};
+const int UPD_ROWS[N_UPD_V] = {0, 1, 2, 4, 8};
 
 struct MLPW {
   const float *W1t, *b1, *W2t, *b2, *W3t, *b3;
@@ -309,6 +328,13 @@ struct pypde_ctx {
   long pred_blocks = 0;     // persistent predictor workgroups for pred_fn
   bool pred_static = true;  // one wavefront per cell group instead of the work queue
   bool pred_w2 = false;     // the two-wavefronts-per-SIMD register budget
+  bool pred_fixed = false;  // schedule fixed by PYPDE_PRED_SCHEDULE (not tuned)
+  struct {
+    int weno = 0, flux = 0, update = 0;  // variant indices (WENO_KID, FLUX_KID, UPD_KID)
+    bool done = false;                   // tuned, loaded or fixed
+    int source = 0;                      // 0 defaults, 1 measured here, 2 tuning table, 3 forced
+    double ms[4] = {0, 0, 0, 0};         // chosen weno, predictor, flux (both), corrector
+  } tune;
 };
 
 namespace {
@@ -317,8 +343,8 @@ int load_module(pypde_ctx* c, const std::vector<char>& code, hipModule_t* mod, h
   if (hipModuleLoadData(mod, code.data()) != hipSuccess) return fail(PYPDE_ERR_HIP, "hipModuleLoadData failed");
   for (int k = 0; k < K_COUNT; k++) {
     fn[k] = nullptr;
-    bool optional = (k == K_FLUX1 && c->D < 2) || k == K_MLPGUESS ||
-                    (k == K_PRED32 && !c->spec.mixed);
+    bool optional = ((k == K_FLUX1 || k == K_FLUX1_128 || k == K_FLUX1_64) && c->D < 2) ||
+                    k == K_MLPGUESS || (k == K_PRED32 && !c->spec.mixed);
     if (hipModuleGetFunction(&fn[k], *mod, kernel_names[k]) != hipSuccess) {
       fn[k] = nullptr;
       if (!optional) return fail(PYPDE_ERR_HIP, "kernel %s missing", kernel_names[k]);
@@ -390,10 +416,12 @@ int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
     int ghost = 1, z = 0;
     long os = g.nwc;
     std::vector<void*> a = {&uu, &w, &st, &g, &ghost, &z, &z, &os};
+    const int wv = c->D == 1 ? 0 : c->tune.weno;
+    const int TX = WENO_TILE[wv][0], TY = WENO_TILE[wv][1];
     dim3 grid = c->D == 1 ? dim3((unsigned)cdiv(g.nw[0], 256))
-                          : dim3((unsigned)cdiv(g.nw[1], 32), (unsigned)cdiv(g.nw[0], 8),
+                          : dim3((unsigned)cdiv(g.nw[1], TY), (unsigned)cdiv(g.nw[0], TX),
                                  (unsigned)c->V);
-    if ((rc = launch(c, K_WENO, grid, dim3(256), a))) return rc;
+    if ((rc = launch(c, WENO_KID[wv], grid, dim3(256), a))) return rc;
   }
   mark();
   {
@@ -452,7 +480,8 @@ int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
     const double* td = c->trd;
     std::vector<void*> a = {&t, &f, &b, &st, &g, &td};
     long items = g.nf[d] * c->N * (c->ND / c->N);
-    if ((rc = launch(c, d == 0 ? K_FLUX0 : K_FLUX1, dim3((unsigned)cdiv(items, 256)), dim3(256), a)))
+    const int bs = FLUX_BS[c->tune.flux];
+    if ((rc = launch(c, FLUX_KID[d][c->tune.flux], dim3((unsigned)cdiv(items, bs)), dim3(bs), a)))
       return rc;
     mark();
   }
@@ -464,7 +493,10 @@ int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
     double* r = c->ret;
     StepState* st = c->st;
     std::vector<void*> a = {&uu, &v, &fbufs, &r, &st, &g};
-    if ((rc = launch(c, K_UPDATE, dim3((unsigned)cdiv(g.ncell, 256)), dim3(256), a))) return rc;
+    const int R = UPD_ROWS[c->tune.update];
+    const long threads = R == 0 ? g.ncell : (long)c->V * cdiv(g.nx[0], R) * g.nx[1];
+    if ((rc = launch(c, UPD_KID[c->tune.update], dim3((unsigned)cdiv(threads, 256)), dim3(256), a)))
+      return rc;
   }
   mark();
   {
@@ -644,6 +676,7 @@ pypde_ctx* pypde_create(const char* src, bool useF, bool useB, bool useS, int* n
   if (const char* e = getenv("PYPDE_PRED_SCHEDULE")) {
     c->pred_static = strcmp(e, "queue") != 0;
     c->pred_w2 = strcmp(e, "static2") == 0;
+    c->pred_fixed = true;
   }
   Geom& g = c->g;
   memset(&g, 0, sizeof g);
@@ -737,6 +770,155 @@ int pypde_get_state(pypde_ctx* c, double* u) {
 void* pypde_device_state(pypde_ctx* c) { return c->u; }
 void* pypde_stream(pypde_ctx* c) { return c->stream; }
 
+// ------------------------------------------------------------- autotuner
+// Picks, per problem (system source, ndim, order, nvar, flags, grid size,
+// GPU architecture), the WENO tile, the flux workgroup size, the corrector
+// mapping and the predictor schedule.  Trials are real timesteps from the
+// current state, each kernel bracketed by events; u and the step state are
+// restored after every trial, so tuning changes nothing but the launch
+// choices.  Choices persist in <code-object cache>/tuning.txt (one line per
+// problem; the last line for a key wins) and in memory for the process.
+// PYPDE_TUNE=0 keeps the defaults; PYPDE_TUNE_FORCE="w,f,u,p" fixes them.
+namespace {
+std::map<uint64_t, std::vector<int>> g_tuned;
+
+uint64_t tune_key(pypde_ctx* c) {
+  std::ostringstream k;
+  k << c->spec.src << "|" << c->D << c->N << c->V << c->spec.useF << c->spec.useB << c->spec.useS
+    << c->spec.useM << c->spec.stiff << c->spec.guess << c->spec.mixed << c->spec.flux
+    << c->spec.second << c->spec.b_quirk << "|" << c->g.nx[0] << "x" << c->g.nx[1] << "|";
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) k << prop.gcnArchName;
+  return fnv1a(k.str());
+}
+
+void apply_tuning(pypde_ctx* c, const std::vector<int>& v) {
+  c->tune.weno = (c->D == 2 && v[0] >= 0 && v[0] < N_WENO_V) ? v[0] : 0;
+  c->tune.flux = (v[1] >= 0 && v[1] < N_FLUX_V) ? v[1] : 0;
+  c->tune.update = (v[2] >= 0 && v[2] < N_UPD_V) ? v[2] : 0;
+  if (!c->pred_fixed) c->pred_static = v[3] != 1;
+}
+
+int autotune(pypde_ctx* c) {
+  if (c->tune.done) return 0;
+  c->tune.done = true;
+  const char* off = getenv("PYPDE_TUNE");
+  if (off && strcmp(off, "0") == 0) return 0;
+  if (const char* f = getenv("PYPDE_TUNE_FORCE")) {
+    std::vector<int> v(4, 0);
+    if (sscanf(f, "%d,%d,%d,%d", &v[0], &v[1], &v[2], &v[3]) != 4)
+      return fail(PYPDE_ERR_ARG, "PYPDE_TUNE_FORCE=%s: expected w,f,u,p", f);
+    apply_tuning(c, v);
+    c->tune.source = 3;
+    return 0;
+  }
+  const uint64_t key = tune_key(c);
+  const std::string path = cache_dir() + "/tuning.txt";
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_tuned.find(key);
+    if (it == g_tuned.end()) {
+      std::ifstream f(path);
+      std::string line;
+      while (std::getline(f, line)) {
+        unsigned long long k = 0;
+        std::vector<int> v(4, 0);
+        if (sscanf(line.c_str(), "%llx %d %d %d %d", &k, &v[0], &v[1], &v[2], &v[3]) == 5 &&
+            k == key)
+          g_tuned[key] = v;
+      }
+      it = g_tuned.find(key);
+    }
+    if (it != g_tuned.end()) {
+      apply_tuning(c, it->second);
+      c->tune.source = 2;
+      return 0;
+    }
+  }
+  if (c->host_st.done) return 0;  // nothing to time: keep the defaults
+  // back up the state, run the trials, restore after each
+  const size_t ub = c->g.ncell * c->V * sizeof(double);
+  StepState* st_bak = nullptr;
+  HIPCHK(hipMalloc((void**)&st_bak, sizeof(StepState)));
+  HIPCHK(hipMemcpyAsync(c->tmp, c->u, ub, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(st_bak, c->st, sizeof(StepState), hipMemcpyDeviceToDevice, c->stream));
+  double* keep_ret = c->ret;
+  const bool keep_static = c->pred_static;
+  c->ret = nullptr;
+  const int nw = c->D == 2 ? N_WENO_V : 1, np = c->pred_fixed ? 1 : N_PRED_V;
+  const int ntrial = std::max(std::max(nw, N_FLUX_V), std::max(N_UPD_V, np));
+  const int reps = 2;
+  std::vector<std::vector<hipEvent_t>> evs(1 + reps * ntrial);
+  int rc = 0;
+  for (size_t k = 0; k < evs.size() && !rc; k++) {
+    const int tk = k == 0 ? 0 : (int)((k - 1) % ntrial);
+    c->tune.weno = tk % nw;
+    c->tune.flux = tk % N_FLUX_V;
+    c->tune.update = tk % N_UPD_V;
+    if (!c->pred_fixed) c->pred_static = (tk % np) == 0;
+    rc = enqueue_step(c, false, &evs[k]);
+    if (!rc && hipMemcpyAsync(c->u, c->tmp, ub, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+      rc = fail(PYPDE_ERR_HIP, "autotune: state restore failed");
+    if (!rc && hipMemcpyAsync(c->st, st_bak, sizeof(StepState), hipMemcpyDeviceToDevice,
+                              c->stream) != hipSuccess)
+      rc = fail(PYPDE_ERR_HIP, "autotune: state restore failed");
+  }
+  if (!rc && hipStreamSynchronize(c->stream) != hipSuccess)
+    rc = fail(PYPDE_ERR_HIP, "autotune: trial steps failed");
+  // per variant, the fastest of its trials; event marks per step:
+  // weno | dt | guess | predictor | flux0 | flux1 | update | advance
+  double tw[N_WENO_V], tf[N_FLUX_V], tu[N_UPD_V], tp[N_PRED_V];
+  std::fill(tw, tw + N_WENO_V, 1e30);
+  std::fill(tf, tf + N_FLUX_V, 1e30);
+  std::fill(tu, tu + N_UPD_V, 1e30);
+  std::fill(tp, tp + N_PRED_V, 1e30);
+  auto span = [&](std::vector<hipEvent_t>& e, int a, int b) {
+    float ms = 0;
+    if ((int)e.size() > b && hipEventElapsedTime(&ms, e[a], e[b]) == hipSuccess) return (double)ms;
+    return 1e30;
+  };
+  for (size_t k = 1; k < evs.size() && !rc; k++) {
+    const int tk = (int)((k - 1) % ntrial);
+    tw[tk % nw] = std::min(tw[tk % nw], span(evs[k], 0, 1));
+    tp[tk % np] = std::min(tp[tk % np], span(evs[k], 3, 4));
+    tf[tk % N_FLUX_V] = std::min(tf[tk % N_FLUX_V], span(evs[k], 4, 6));
+    tu[tk % N_UPD_V] = std::min(tu[tk % N_UPD_V], span(evs[k], 6, 7));
+  }
+  for (auto& e : evs)
+    for (auto x : e) (void)hipEventDestroy(x);
+  (void)hipFree(st_bak);
+  c->ret = keep_ret;
+  c->pred_static = keep_static;
+  if (rc) return rc;
+  auto best = [](const double* t, int n) {
+    int b = 0;
+    for (int i = 1; i < n; i++)
+      if (t[i] < t[b]) b = i;
+    return b;
+  };
+  std::vector<int> v = {best(tw, nw), best(tf, N_FLUX_V), best(tu, N_UPD_V),
+                        c->pred_fixed ? (c->pred_static ? 0 : 1) : best(tp, np)};
+  apply_tuning(c, v);
+  c->tune.source = 1;
+  c->tune.ms[0] = tw[v[0]];
+  c->tune.ms[1] = tp[c->pred_fixed ? 0 : v[3]];
+  c->tune.ms[2] = tf[v[1]];
+  c->tune.ms[3] = tu[v[2]];
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_tuned[key] = v;
+  mkdir(cache_dir().c_str(), 0755);
+  std::ofstream f(path, std::ios::app);
+  if (f) {
+    char line[160];
+    snprintf(line, sizeof line, "%016llx %d %d %d %d   # weno %.4f pred %.4f flux %.4f update %.4f ms\n",
+             (unsigned long long)key, v[0], v[1], v[2], v[3], c->tune.ms[0], c->tune.ms[1],
+             c->tune.ms[2], c->tune.ms[3]);
+    f << line;
+  }
+  return 0;
+}
+}  // namespace
+
 // One timestep of each context captured into a graph: a fork event on the
 // first context's stream, every context's step on its own stream, joined back.
 static int capture_steps(const std::vector<pypde_ctx*>& cs, hipGraphExec_t* exec) {
@@ -769,6 +951,7 @@ static int capture_steps(const std::vector<pypde_ctx*>& cs, hipGraphExec_t* exec
 }
 
 static int run_loop(pypde_ctx* c, long max_steps, pypde_stats* stats, bool first_guess) {
+  if (int r = autotune(c)) return r;
   auto t0 = std::chrono::steady_clock::now();
   int chunk = 4;
   bool guess_now = first_guess;
@@ -963,6 +1146,8 @@ int pypde_ensemble_run(pypde_ensemble* e, double tf, long max_steps, pypde_stats
     HIPCHK(hipMemcpyAsync(c->st, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   }
+  for (auto c : e->m)
+    if (int r = autotune(c)) return r;
   auto t0 = std::chrono::steady_clock::now();
   hipGraphExec_t exec = nullptr;
   int rc = capture_steps(e->m, &exec);
@@ -1125,6 +1310,20 @@ int pypde_last_guess(pypde_ctx* c, double* out) {
   size_t per = (size_t)c->NT * c->V;
   for (long cc = 0; cc < c->g.nwc; cc++)
     for (size_t p = 0; p < per; p++) out[cc * per + p] = h[p * c->g.nwc + cc];
+  return 0;
+}
+
+int pypde_get_tuning(pypde_ctx* c, int* choices, double* ms) {
+  if (!c) return fail(PYPDE_ERR_ARG, "null context");
+  if (choices) {
+    choices[0] = c->tune.weno;
+    choices[1] = c->tune.flux;
+    choices[2] = c->tune.update;
+    choices[3] = c->pred_static ? 0 : 1;
+    choices[4] = c->tune.source;
+  }
+  if (ms)
+    for (int i = 0; i < 4; i++) ms[i] = c->tune.ms[i];
   return 0;
 }
 

@@ -175,6 +175,19 @@ class Solver:
         d["rc"] = rc
         return d
 
+    def tuning(self):
+        """The autotuner's launch choices for this problem (see pypde_get_tuning)."""
+        ch = np.zeros(5, dtype=np.int32)
+        ms = np.zeros(4)
+        _capi.check(_capi.lib().pypde_get_tuning(self.h, _capi.ip(ch), _capi.dp(ms)), "tuning")
+        weno = ["8x32", "4x64", "16x16", "16x32"][ch[0]]
+        flux = [256, 128, 64][ch[1]]
+        upd = ["cell", "rows1", "rows2", "rows4", "rows8"][ch[2]]
+        pred = ["static", "queue"][ch[3]]
+        src = ["defaults", "measured", "table", "forced"][ch[4]]
+        return dict(weno_tile=weno, flux_workgroup=flux, update=upd, predictor=pred, source=src,
+                    ms=dict(zip(["weno", "predictor", "flux", "update"], ms.tolist())))
+
     def close(self):
         if getattr(self, "h", None):
             _capi.lib().pypde_destroy(self.h)
