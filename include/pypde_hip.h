@@ -169,7 +169,8 @@ int pypde_harvest_step(pypde_ctx *c, float *X, float *Y, long ldx, long col0, py
  * recorded for the same problem in <code-object cache>/tuning.txt.
  * choices[5]: WENO tile (0 8x32, 1 4x64, 2 16x16, 3 16x32), flux workgroup (0
  * 256, 1 128, 2 64), corrector (0 one thread per cell, 1..4 one variable and
- * 1, 2, 4, 8 x-rows per thread), predictor (0 static, 1 work queue), source (0
+ * 1, 2, 4, 8 x-rows per thread), predictor (0 static, 1 work queue, 2 cold/hot split of the stiff
+ * predictor), source (0
  * defaults, 1 measured, 2 tuning table, 3 PYPDE_TUNE_FORCE); ms[4]: the chosen
  * WENO, predictor, flux (both directions) and corrector trial times. */
 int pypde_get_tuning(pypde_ctx *c, int *choices, double *ms);

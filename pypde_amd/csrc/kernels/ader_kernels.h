@@ -37,6 +37,7 @@ struct StepState {
   // Jacobian solve and with the A = 0 shortcut, summed over nodes and steps
   unsigned long long pred_nfull, pred_nshort;
   unsigned long long pred_iters32;  // mixed precision: FP32-phase iterations over cells
+  unsigned long long hot_count;     // split stiff predictor: cells handed to the hot pass
   // predictor statistics spread over 64 slots, one 128-byte line each, so the
   // wavefronts' atomics do not queue on one address; k_advance folds them into
   // the totals above and clears them ([0] iterations, [1] cells, [2] capped,
@@ -261,6 +262,7 @@ extern "C" __global__ void k_dt_finalize(StepState* st) {
   if (st->t + dt > st->tf) dt = st->tf - st->t;
   st->dt = dt;
   st->qhead = 0ull;
+  st->hot_count = 0ull;
 }
 
 // ------------------------------------------------------- eigen diagnostics

@@ -72,8 +72,8 @@ DEV void node_ctx(NodeCtx& c, int lane) {
 }
 
 // derivative along d at this lane's node of buffer buf[slot][t][node][v]
-template <int d>
-DEV void lds_deriv(const NodeCtx& c, const double (*buf)[N][ND][V], int t, double rdx, double* out) {
+template <int d, int NTB>
+DEV void lds_deriv(const NodeCtx& c, const double (*buf)[NTB][ND][V], int t, double rdx, double* out) {
   constexpr int stride = ipow_c(N, D - 1 - d);
 #pragma unroll
   for (int v = 0; v < V; v++) {
@@ -93,10 +93,13 @@ DEV void callS(double* out, const double* q) {
 #endif
 }
 
+// F is staged in LDS one time node at a time (NTF = 1), except for a viscous
+// F, whose gradient traces need every time node's derivatives (NTF = N).
+constexpr int NTF = PDE_SECOND ? N : 1;
 struct PredLDS {
   double q[CPW][N][ND][V];
 #if PDE_USE_F
-  double f[D][CPW][N][ND][V];
+  double f[D][CPW][NTF][ND][V];
 #endif
   double red[64];
 };
@@ -153,57 +156,79 @@ DEV void grad_traces(const NodeCtx& c, PredLDS& sm, const Geom& g, long cell, bo
 
 // rhs of the space-time system at this lane's node for all t (dg.cpp:50-126):
 // r = W_t prod W (S(q) - sum_d (B_d dq_d + d_d F_d)),  rhs = r dt + Ww.
-// Assumes sm.q holds the current iterate; leaves the lane's dq in dqs.
+// Assumes sm.q holds the current iterate; leaves the lane's dq in dqs.  With
+// NTF = 1, F of one time node at a time passes through LDS (two barriers per
+// time node); every (t, v) result is the same either way.
 DEV void node_rhs(const NodeCtx& c, PredLDS& sm, const double (&q)[N][V], const double* w,
                   double dt, const Geom& g, bool active, double (&rhs)[N][V],
                   double (&dqs)[N][D][V]) {
-  if (active) {
-#pragma unroll
-    for (int t = 0; t < N; t++) {
-      static_for<0, D>([&](auto dc) {
-        constexpr int d = decltype(dc)::value;
-        lds_deriv<d>(c, sm.q, t, 1.0 / g.dX[d], dqs[t][d]);
-      });
+  auto eval_f = [&](int t, int slot_t) {
+    static_for<0, D>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      lds_deriv<d>(c, sm.q, t, 1.0 / g.dX[d], dqs[t][d]);
+    });
 #if PDE_USE_F
 #pragma unroll
-      for (int d = 0; d < D; d++) {
-        double f[V];
-        F(f, q[t], &dqs[t][0][0], d);
+    for (int d = 0; d < D; d++) {
+      double f[V];
+      F(f, q[t], &dqs[t][0][0], d);
 #pragma unroll
-        for (int v = 0; v < V; v++) sm.f[d][c.slot][t][c.idx][v] = f[v];
-      }
-#endif
+      for (int v = 0; v < V; v++) sm.f[d][c.slot][slot_t][c.idx][v] = f[v];
     }
-  }
-#if PDE_USE_F
-  __syncthreads();
+#else
+    (void)slot_t;
 #endif
-  if (active) {
+  };
+  auto finish = [&](int t, int slot_t) {
+    double r[V];
+    callS(r, q[t]);
+    static_for<0, D>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+#if PDE_USE_B
+      double b[V * V], bd[V];
+      B(b, q[t], d);
+      b_times<PDE_B_QUIRK ? 1 : 0>(b, dqs[t][d], bd);
+#pragma unroll
+      for (int v = 0; v < V; v++) r[v] = r[v] - bd[v];
+#endif
+#if PDE_USE_F
+      double df[V];
+      lds_deriv<d>(c, sm.f[d], slot_t, 1.0 / g.dX[d], df);
+#pragma unroll
+      for (int v = 0; v < V; v++) r[v] = r[v] - df[v];
+#else
+      (void)slot_t;
+#endif
+    });
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      double rr = r[v] * c.ct[t];
+      rhs[t][v] = rr * dt + c.c0[t] * w[v];
+    }
+  };
+  if constexpr (NTF == N) {
+    if (active) {
+#pragma unroll
+      for (int t = 0; t < N; t++) eval_f(t, t);
+    }
+#if PDE_USE_F
+    __syncthreads();
+#endif
+    if (active) {
+#pragma unroll
+      for (int t = 0; t < N; t++) finish(t, t);
+    }
+  } else {
 #pragma unroll
     for (int t = 0; t < N; t++) {
-      double r[V];
-      callS(r, q[t]);
-      static_for<0, D>([&](auto dc) {
-        constexpr int d = decltype(dc)::value;
-#if PDE_USE_B
-        double b[V * V], bd[V];
-        B(b, q[t], d);
-        b_times<PDE_B_QUIRK ? 1 : 0>(b, dqs[t][d], bd);
-#pragma unroll
-        for (int v = 0; v < V; v++) r[v] = r[v] - bd[v];
-#endif
+      if (active) eval_f(t, 0);
 #if PDE_USE_F
-        double df[V];
-        lds_deriv<d>(c, sm.f[d], t, 1.0 / g.dX[d], df);
-#pragma unroll
-        for (int v = 0; v < V; v++) r[v] = r[v] - df[v];
+      __syncthreads();
 #endif
-      });
-#pragma unroll
-      for (int v = 0; v < V; v++) {
-        double rr = r[v] * c.ct[t];
-        rhs[t][v] = rr * dt + c.c0[t] * w[v];
-      }
+      if (active) finish(t, 0);
+#if PDE_USE_F
+      if (t + 1 < N) __syncthreads();
+#endif
     }
   }
 }
@@ -274,6 +299,9 @@ constexpr double NEWTON_A_TINY = 1e-6;  // max |dt dS/dQ| below which the step d
 // derivation): g_t = G_t / (W_t WP), A = dt dS/dQ at the time-weighted mean,
 // h_t = sum_j A^j sum_s M_{N-j}[t][s] g_s, P = p(A), dq = -P^-1 h.
 // Returns true where the source Jacobian was solved, false for the shortcut.
+// FULL = false builds only the shortcut: it returns true, leaving G as it
+// was, where the full solve would be needed (the caller hands the cell on).
+template <bool FULL = true>
 DEV bool newton_node(const NodeCtx& c, const double (&q)[N][V], double (&G)[N][V], double dt) {
   double g[N][V];
 #pragma unroll
@@ -331,6 +359,7 @@ DEV bool newton_node(const NodeCtx& c, const double (&q)[N][V], double (&G)[N][V
       }
     return false;
   }
+  if constexpr (!FULL) return true;
   // h_t (Horner in A); result left in G
 #pragma unroll
   for (int t = 0; t < N; t++) {
@@ -753,18 +782,34 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
 // fv.cpp:102-122) and the cell-volume term (fv.cpp:33-85) for the corrector.
 // Static schedule: one wavefront per group of CPW consecutive cells, which
 // stays until its slowest cell has converged.
+// MODE 0: every cell, by block.  MODE 1 (stiff, plain guess): the same cells,
+// but built without the full source-Jacobian solve: a cell where a node would
+// need it is dropped (no outputs, no statistics) and appended to hot[], so
+// this kernel needs far fewer registers.  MODE 2: the cells listed in hot[]
+// by MODE 1, with the full solve.  Every cell gets the same arithmetic as in
+// MODE 0.
+template <int MODE = 0>
 DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
                                const double* __restrict__ guess, double* __restrict__ tr,
                                double* __restrict__ vol, double* __restrict__ qh, StepState* st,
                                const Geom& g, double newton_tol, int newton_maxit,
-                               double* __restrict__ trd) {
+                               double* __restrict__ trd, unsigned int* __restrict__ hot = nullptr) {
   if (st->done) return;
+  if (MODE == 2 && (unsigned long long)blockIdx.x * CPW >= st->hot_count) return;
   const double dt = st->dt;
   const int lane = threadIdx.x;
   NodeCtx c;
   node_ctx(c, lane);
-  const long cell = (long)blockIdx.x * CPW + c.slot;
-  const bool valid = (c.slot < CPW) && (cell < g.nwc);
+  long cell;
+  bool valid;
+  if constexpr (MODE == 2) {
+    const unsigned long long i = (unsigned long long)blockIdx.x * CPW + c.slot;
+    valid = (c.slot < CPW) && (i < st->hot_count);
+    cell = valid ? (long)hot[i] : 0;
+  } else {
+    cell = (long)blockIdx.x * CPW + c.slot;
+    valid = (c.slot < CPW) && (cell < g.nwc);
+  }
   double w[V], q[N][V], rhs[N][V], dqs[N][D][V];
   if (valid) {
 #pragma unroll
@@ -789,6 +834,7 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
   double r0 = 0.0;
   int trip = 0;
   unsigned int n_full = 0u, n_short = 0u;  // node corrections of this lane
+  bool cell_hot = false;                   // MODE 1: the cell needs the full solve
   while (__ballot(active) != 0ull) {
     if (active) {
 #pragma unroll
@@ -849,18 +895,32 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
       }
     }
 #endif
+    bool node_hot = false;
     if (active && !skip) {
-      if (newton_node(c, q, rhs, dt)) n_full++;
-      else n_short++;
+      if (newton_node<MODE != 1>(c, q, rhs, dt)) {
+        n_full++;
+        node_hot = MODE == 1;
+      } else {
+        n_short++;
+      }
+      if (!node_hot) {
 #pragma unroll
-      for (int t = 0; t < N; t++)
+        for (int t = 0; t < N; t++)
 #pragma unroll
-        for (int v = 0; v < V; v++) {
-          if (!(dabs(rhs[t][v]) <= newton_tol * (1.0 + dabs(q[t][v])))) more = true;
-          q[t][v] = q[t][v] + rhs[t][v];
-        }
+          for (int v = 0; v < V; v++) {
+            if (!(dabs(rhs[t][v]) <= newton_tol * (1.0 + dabs(q[t][v])))) more = true;
+            q[t][v] = q[t][v] + rhs[t][v];
+          }
+      }
     }
     if (skip) more = true;
+    if constexpr (MODE == 1) {
+      // a cell with a node that needs the full solve leaves for the hot pass
+      if ((__ballot(node_hot) & cellmask) != 0ull && active) {
+        cell_hot = true;
+        active = false;
+      }
+    }
 #else
     (void)learned;
     (void)r0;
@@ -888,14 +948,24 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
     __syncthreads();
   }
   // ---- epilogue: traces, volume term, optional full qh
-  if (valid) {
+  if constexpr (MODE == 1) {
+    // hand hot cells to the hot pass: one atomic per wavefront, slots in order
+    const unsigned long long lead = __ballot(cell_hot && c.idx == 0);
+    unsigned long long base = 0;
+    if (lane == 0 && lead) base = atomicAdd(&st->hot_count, (unsigned long long)__popcll(lead));
+    base = __shfl(base, 0, 64);
+    if (cell_hot && c.idx == 0)
+      hot[base + __popcll(lead & ((1ull << (c.slot * ND)) - 1ull))] = (unsigned int)cell;
+  }
+  const bool done_here = valid && !cell_hot;
+  if (done_here) {
 #pragma unroll
     for (int t = 0; t < N; t++)
 #pragma unroll
       for (int v = 0; v < V; v++) sm.q[c.slot][t][c.idx][v] = q[t][v];
   }
   __syncthreads();
-  if (valid) {
+  if (done_here) {
     // traces with every lane busy: the N ND1 = ND lanes of a cell take one
     // (time node t, transverse node j) each and write both ends e of every
     // direction d and variable v
@@ -928,12 +998,12 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
     }
   }
 #if PDE_SECOND
-  grad_traces(c, sm, g, cell, valid, trd);
+  grad_traces(c, sm, g, cell, done_here, trd);
 #endif
 #if PDE_USE_S || PDE_USE_B
   // volume term (fv.cpp:33-85): per node sum over t, then over nodes in order
   double part[V];
-  if (valid) {
+  if (done_here) {
 #pragma unroll
     for (int t = 0; t < N; t++) {
       double s[V];
@@ -957,13 +1027,13 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
     }
   }
   __syncthreads();
-  if (valid) {
+  if (done_here) {
 #pragma unroll
     for (int v = 0; v < V; v++) sm.q[c.slot][0][c.idx][v] = part[v];
   }
   __syncthreads();
   // the cell's lane idx sums variable v = idx (+ ND, ...) over the nodes in order
-  if (valid) {
+  if (done_here) {
 #pragma unroll
     for (int v0 = 0; v0 < V; v0 += ND) {
       const int v = v0 + c.idx;
@@ -977,11 +1047,12 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
   }
 #endif
   // ---- statistics (one contribution per cell)
-  unsigned long long mine = (valid && c.idx == 0) ? (unsigned long long)it : 0ull;
-  unsigned long long cnt = (valid && c.idx == 0) ? 1ull : 0ull;
-  unsigned long long cap = (valid && c.idx == 0 && it >= maxit) ? 1ull : 0ull;
-  unsigned long long fb = (valid && c.idx == 0) ? (unsigned long long)fellback : 0ull;
-  unsigned int mx = (valid && c.idx == 0) ? (unsigned int)it : 0u;
+  unsigned long long mine = (done_here && c.idx == 0) ? (unsigned long long)it : 0ull;
+  unsigned long long cnt = (done_here && c.idx == 0) ? 1ull : 0ull;
+  unsigned long long cap = (done_here && c.idx == 0 && it >= maxit) ? 1ull : 0ull;
+  unsigned long long fb = (done_here && c.idx == 0) ? (unsigned long long)fellback : 0ull;
+  unsigned int mx = (done_here && c.idx == 0) ? (unsigned int)it : 0u;
+  if (!done_here) n_full = n_short = 0u;
   mine = wave_sum_u64(mine);
   cnt = wave_sum_u64(cnt);
   cap = wave_sum_u64(cap);
@@ -1008,6 +1079,28 @@ k_predictor_static(const double* __restrict__ wh, const double* __restrict__ gue
   __shared__ PredLDS sm;
   predictor_static_body(sm, wh, guess, tr, vol, qh, st, g, newton_tol, newton_maxit, trd);
 }
+
+#if PDE_STIFF && !PDE_GUESS
+// The split stiff predictor: the cold pass (no full source-Jacobian solve in
+// the code, so fewer registers and more wavefronts per SIMD) over every
+// cell, then the hot pass over the cells it handed on.
+extern "C" __global__ void __launch_bounds__(64)
+k_predictor_cold(const double* __restrict__ wh, double* __restrict__ tr, double* __restrict__ vol,
+                 double* __restrict__ qh, StepState* st, Geom g, double newton_tol,
+                 int newton_maxit, unsigned int* __restrict__ hot) {
+  __shared__ PredLDS sm;
+  predictor_static_body<1>(sm, wh, nullptr, tr, vol, qh, st, g, newton_tol, newton_maxit,
+                           nullptr, hot);
+}
+extern "C" __global__ void __launch_bounds__(64)
+k_predictor_hot(const double* __restrict__ wh, double* __restrict__ tr, double* __restrict__ vol,
+                double* __restrict__ qh, StepState* st, Geom g, double newton_tol,
+                int newton_maxit, unsigned int* __restrict__ hot) {
+  __shared__ PredLDS sm;
+  predictor_static_body<2>(sm, wh, nullptr, tr, vol, qh, st, g, newton_tol, newton_maxit,
+                           nullptr, hot);
+}
+#endif
 
 // The same with the register budget of two wavefronts per SIMD (spilling the
 // rest), for latency hiding where the single-wavefront kernel stalls.

@@ -80,6 +80,7 @@ struct StepState {
   unsigned long long qhead;
   unsigned long long pred_nfull, pred_nshort;
   unsigned long long pred_iters32;
+  unsigned long long hot_count;
   // predictor statistics spread over 64 slots, one 128-byte line each, so the
   // wavefronts' atomics do not queue on one address; k_advance folds them into
   // the totals above and clears them ([0] iterations, [1] cells, [2] capped,
@@ -269,7 +270,7 @@ int check_spec(const Spec& s, bool secondOrder, int FLUX) {
 enum { K_WENO, K_DT, K_DTFIN, K_PRED, K_FLUX0, K_FLUX1, K_UPDATE, K_ADVANCE, K_A2S, K_S2A,
        K_HARVEST, K_MLPGUESS, K_PRED_STATIC, K_PRED_STATIC2, K_EIG, K_P2R, K_PRED32,
        K_WENO_4x64, K_WENO_16x16, K_WENO_16x32, K_FLUX0_128, K_FLUX0_64, K_FLUX1_128, K_FLUX1_64,
-       K_UPD1, K_UPD2, K_UPD4, K_UPD8, K_COUNT };
+       K_UPD1, K_UPD2, K_UPD4, K_UPD8, K_PRED_COLD, K_PRED_HOT, K_COUNT };
 const char* kernel_names[K_COUNT] = {"k_weno",   "k_dt",    "k_dt_finalize", "k_predictor",
                                      "k_flux0",  "k_flux1", "k_update",      "k_advance",
                                      "k_aos_to_soa", "k_soa_to_aos", "k_harvest", "k_mlp_guess",
@@ -277,13 +278,15 @@ const char* kernel_names[K_COUNT] = {"k_weno",   "k_dt",    "k_dt_finalize", "k_
                                      "k_planes_to_rows", "k_predictor_f32",
                                      "k_weno_4x64", "k_weno_16x16", "k_weno_16x32",
                                      "k_flux0_128", "k_flux0_64", "k_flux1_128", "k_flux1_64",
-                                     "k_update_r1", "k_update_r2", "k_update_r4", "k_update_r8"};
+                                     "k_update_r1", "k_update_r2", "k_update_r4", "k_update_r8",
+                                     "k_predictor_cold", "k_predictor_hot"};
 
 // Launch variants the autotuner chooses between (every variant of a kernel
 // computes identical results): WENO tiles, flux workgroup sizes, corrector
 // mappings (one thread per cell and all variables, or one variable and R
-// x-rows per thread), predictor schedules (static, work queue).
-constexpr int N_WENO_V = 4, N_FLUX_V = 3, N_UPD_V = 5, N_PRED_V = 2;
+// x-rows per thread), predictor schedules (static, work queue, and for the
+// stiff predictor with the plain guess the cold/hot split).
+constexpr int N_WENO_V = 4, N_FLUX_V = 3, N_UPD_V = 5, N_PRED_V = 3;
 const int WENO_TILE[N_WENO_V][2] = {{8, 32}, {4, 64}, {16, 16}, {16, 32}};
 const int WENO_KID[N_WENO_V] = {K_WENO, K_WENO_4x64, K_WENO_16x16, K_WENO_16x32};
 const int FLUX_BS[N_FLUX_V] = {256, 128, 64};
@@ -329,6 +332,8 @@ struct pypde_ctx {
   bool pred_static = true;  // one wavefront per cell group instead of the work queue
   bool pred_w2 = false;     // the two-wavefronts-per-SIMD register budget
   bool pred_fixed = false;  // schedule fixed by PYPDE_PRED_SCHEDULE (not tuned)
+  bool pred_split = false;  // cold pass over every cell, hot pass over the cells it hands on
+  unsigned int* hot = nullptr;  // the hot pass's cell list (nwc entries)
   struct {
     int weno = 0, flux = 0, update = 0;  // variant indices (WENO_KID, FLUX_KID, UPD_KID)
     bool done = false;                   // tuned, loaded or fixed
@@ -344,7 +349,8 @@ int load_module(pypde_ctx* c, const std::vector<char>& code, hipModule_t* mod, h
   for (int k = 0; k < K_COUNT; k++) {
     fn[k] = nullptr;
     bool optional = ((k == K_FLUX1 || k == K_FLUX1_128 || k == K_FLUX1_64) && c->D < 2) ||
-                    k == K_MLPGUESS || (k == K_PRED32 && !c->spec.mixed);
+                    k == K_MLPGUESS || (k == K_PRED32 && !c->spec.mixed) || k == K_PRED_COLD ||
+                    k == K_PRED_HOT;
     if (hipModuleGetFunction(&fn[k], *mod, kernel_names[k]) != hipSuccess) {
       fn[k] = nullptr;
       if (!optional) return fail(PYPDE_ERR_HIP, "kernel %s missing", kernel_names[k]);
@@ -464,7 +470,13 @@ int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
     int maxit = c->opts.newton_maxit;
     double* td = c->trd;
     std::vector<void*> a = {&w, &gs, &t, &v, &qh, &st, &g, &tol, &maxit, &td};
-    if (c->pred_static) {
+    if (c->pred_split && c->fn[K_PRED_COLD] && c->fn[K_PRED_HOT] && !gs && c->hot) {
+      unsigned int* hl = c->hot;
+      std::vector<void*> b = {&w, &t, &v, &qh, &st, &g, &tol, &maxit, &hl};
+      const unsigned blocks = (unsigned)cdiv(g.nwc, c->CPW);
+      if ((rc = launch(c, K_PRED_COLD, dim3(blocks), dim3(64), b))) return rc;
+      if ((rc = launch(c, K_PRED_HOT, dim3(blocks), dim3(64), b))) return rc;
+    } else if (c->pred_static) {
       if ((rc = launch(c, c->pred_w2 ? K_PRED_STATIC2 : K_PRED_STATIC,
                        dim3((unsigned)cdiv(g.nwc, c->CPW)), dim3(64), a)))
         return rc;
@@ -600,6 +612,7 @@ void pypde_destroy(pypde_ctx* c) {
     if (p) (void)hipFree(p);
   if (c->st) (void)hipFree(c->st);
   if (c->gbuf) (void)hipFree(c->gbuf);
+  if (c->hot) (void)hipFree(c->hot);
   if (c->qh) (void)hipFree(c->qh);
   if (c->mod) (void)hipModuleUnload(c->mod);
   if (c->mod_guess) (void)hipModuleUnload(c->mod_guess);
@@ -676,6 +689,7 @@ pypde_ctx* pypde_create(const char* src, bool useF, bool useB, bool useS, int* n
   if (const char* e = getenv("PYPDE_PRED_SCHEDULE")) {
     c->pred_static = strcmp(e, "queue") != 0;
     c->pred_w2 = strcmp(e, "static2") == 0;
+    c->pred_split = strcmp(e, "split") == 0;
     c->pred_fixed = true;
   }
   Geom& g = c->g;
@@ -734,6 +748,8 @@ pypde_ctx* pypde_create(const char* src, bool useF, bool useB, bool useS, int* n
     }
   }
   if (ok && s.mixed) ok = alloc(&c->gbuf, g.nwc * c->NT * V);
+  if (ok && c->fn[K_PRED_COLD])
+    ok = hipMalloc((void**)&c->hot, std::max<long>(g.nwc, 1) * sizeof(unsigned int)) == hipSuccess;
   ok = ok && hipMalloc((void**)&c->st, sizeof(StepState)) == hipSuccess;
   if (!ok) return bail(fail(PYPDE_ERR_HIP, "device allocation failed"));
   memset(&c->host_st, 0, sizeof c->host_st);
@@ -796,7 +812,10 @@ void apply_tuning(pypde_ctx* c, const std::vector<int>& v) {
   c->tune.weno = (c->D == 2 && v[0] >= 0 && v[0] < N_WENO_V) ? v[0] : 0;
   c->tune.flux = (v[1] >= 0 && v[1] < N_FLUX_V) ? v[1] : 0;
   c->tune.update = (v[2] >= 0 && v[2] < N_UPD_V) ? v[2] : 0;
-  if (!c->pred_fixed) c->pred_static = v[3] != 1;
+  if (!c->pred_fixed) {
+    c->pred_static = v[3] != 1;
+    c->pred_split = v[3] == 2 && c->fn[K_PRED_COLD] != nullptr;
+  }
 }
 
 int autotune(pypde_ctx* c) {
@@ -843,9 +862,10 @@ int autotune(pypde_ctx* c) {
   HIPCHK(hipMemcpyAsync(c->tmp, c->u, ub, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(st_bak, c->st, sizeof(StepState), hipMemcpyDeviceToDevice, c->stream));
   double* keep_ret = c->ret;
-  const bool keep_static = c->pred_static;
+  const bool keep_static = c->pred_static, keep_split = c->pred_split;
   c->ret = nullptr;
-  const int nw = c->D == 2 ? N_WENO_V : 1, np = c->pred_fixed ? 1 : N_PRED_V;
+  const int nw = c->D == 2 ? N_WENO_V : 1;
+  const int np = c->pred_fixed ? 1 : (c->fn[K_PRED_COLD] && c->hot ? N_PRED_V : 2);
   const int ntrial = std::max(std::max(nw, N_FLUX_V), std::max(N_UPD_V, np));
   const int reps = 2;
   std::vector<std::vector<hipEvent_t>> evs(1 + reps * ntrial);
@@ -855,7 +875,10 @@ int autotune(pypde_ctx* c) {
     c->tune.weno = tk % nw;
     c->tune.flux = tk % N_FLUX_V;
     c->tune.update = tk % N_UPD_V;
-    if (!c->pred_fixed) c->pred_static = (tk % np) == 0;
+    if (!c->pred_fixed) {
+      c->pred_static = (tk % np) != 1;
+      c->pred_split = (tk % np) == 2;
+    }
     rc = enqueue_step(c, false, &evs[k]);
     if (!rc && hipMemcpyAsync(c->u, c->tmp, ub, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
       rc = fail(PYPDE_ERR_HIP, "autotune: state restore failed");
@@ -889,6 +912,7 @@ int autotune(pypde_ctx* c) {
   (void)hipFree(st_bak);
   c->ret = keep_ret;
   c->pred_static = keep_static;
+  c->pred_split = keep_split;
   if (rc) return rc;
   auto best = [](const double* t, int n) {
     int b = 0;
@@ -1319,7 +1343,7 @@ int pypde_get_tuning(pypde_ctx* c, int* choices, double* ms) {
     choices[0] = c->tune.weno;
     choices[1] = c->tune.flux;
     choices[2] = c->tune.update;
-    choices[3] = c->pred_static ? 0 : 1;
+    choices[3] = c->pred_split ? 2 : (c->pred_static ? 0 : 1);
     choices[4] = c->tune.source;
   }
   if (ms)
