@@ -161,13 +161,15 @@ std::map<uint64_t, std::vector<char>> g_codes;
 // <cache_dir>/<key>.hsaco, written to a temporary name and renamed into place).
 // PYPDE_HIP_RECOMPILE=1 compiles even when a cached code object exists (the
 // build's compile check); the result still replaces the cache entry.
-int rtc_compile(const std::string& src, const char* what, std::vector<char>* code) {
+int rtc_compile(const std::string& src, const char* what, std::vector<char>* code,
+                uint64_t* key_out = nullptr) {
   std::string key = src;
   for (auto& x : compile_options()) key += "\n" + x;
   int ver = 0;
   hiprtcVersion(&ver, &ver);
   key += "\nhiprtc" + std::to_string(ver);
   uint64_t h = fnv1a(key);
+  if (key_out) *key_out = h;
   const char* re = getenv("PYPDE_HIP_RECOMPILE");
   const bool recompile = re && *re && *re != '0';
   if (!recompile) {
@@ -231,7 +233,7 @@ int rtc_compile(const std::string& src, const char* what, std::vector<char>* cod
   return 0;
 }
 
-int compile_code(const Spec& spec, std::vector<char>* code) {
+int compile_code(const Spec& spec, std::vector<char>* code, uint64_t* key = nullptr) {
   pypde::Tables T;
   if (pypde::make_tables(spec.N, &T)) return fail(PYPDE_ERR_ARG, "bad order N=%d", spec.N);
   std::string src = full_source(spec, T);
@@ -239,7 +241,7 @@ int compile_code(const Spec& spec, std::vector<char>* code) {
     std::ofstream f(dump);
     f << src;
   }
-  return rtc_compile(src, "pypde_ader.hip", code);
+  return rtc_compile(src, "pypde_ader.hip", code, key);
 }
 
 int check_spec(const Spec& s, bool secondOrder, int FLUX) {
@@ -344,8 +346,44 @@ struct pypde_ctx {
 
 namespace {
 
-int load_module(pypde_ctx* c, const std::vector<char>& code, hipModule_t* mod, hipFunction_t* fn) {
-  if (hipModuleLoadData(mod, code.data()) != hipSuccess) return fail(PYPDE_ERR_HIP, "hipModuleLoadData failed");
+// Loaded code objects, one per (device, code-object key), shared by every
+// context of the same specification (ensemble members, repeated solves) and
+// unloaded when the last one is destroyed.
+struct ModEntry {
+  hipModule_t mod;
+  int refs;
+};
+std::map<std::pair<int, uint64_t>, ModEntry> g_mods;
+std::mutex g_mod_mu;
+
+void release_module(int device, hipModule_t mod) {
+  if (!mod) return;
+  std::lock_guard<std::mutex> lk(g_mod_mu);
+  for (auto it = g_mods.begin(); it != g_mods.end(); ++it)
+    if (it->first.first == device && it->second.mod == mod) {
+      if (--it->second.refs == 0) {
+        (void)hipModuleUnload(mod);
+        g_mods.erase(it);
+      }
+      return;
+    }
+  (void)hipModuleUnload(mod);
+}
+
+int load_module(pypde_ctx* c, const std::vector<char>& code, uint64_t key, hipModule_t* mod,
+                hipFunction_t* fn) {
+  {
+    std::lock_guard<std::mutex> lk(g_mod_mu);
+    auto it = g_mods.find({c->device, key});
+    if (it != g_mods.end()) {
+      it->second.refs++;
+      *mod = it->second.mod;
+    } else {
+      if (hipModuleLoadData(mod, code.data()) != hipSuccess)
+        return fail(PYPDE_ERR_HIP, "hipModuleLoadData failed");
+      g_mods[{c->device, key}] = {*mod, 1};
+    }
+  }
   for (int k = 0; k < K_COUNT; k++) {
     fn[k] = nullptr;
     bool optional = ((k == K_FLUX1 || k == K_FLUX1_128 || k == K_FLUX1_64) && c->D < 2) ||
@@ -614,8 +652,8 @@ void pypde_destroy(pypde_ctx* c) {
   if (c->gbuf) (void)hipFree(c->gbuf);
   if (c->hot) (void)hipFree(c->hot);
   if (c->qh) (void)hipFree(c->qh);
-  if (c->mod) (void)hipModuleUnload(c->mod);
-  if (c->mod_guess) (void)hipModuleUnload(c->mod_guess);
+  release_module(c->device, c->mod);
+  release_module(c->device, c->mod_guess);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -669,7 +707,8 @@ pypde_ctx* pypde_create(const char* src, bool useF, bool useB, bool useS, int* n
     }
   }
   std::vector<char> code;
-  if ((rc = compile_code(s, &code))) return nullptr;
+  uint64_t code_key = 0;
+  if ((rc = compile_code(s, &code, &code_key))) return nullptr;
   pypde_ctx* c = new pypde_ctx();
   c->spec = s;
   c->opts = opts;
@@ -721,7 +760,7 @@ pypde_ctx* pypde_create(const char* src, bool useF, bool useB, bool useS, int* n
   if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, c->device) !=
       hipSuccess)
     c->n_cu = 0;
-  if (load_module(c, code, &c->mod, c->fn_plain)) return bail(PYPDE_ERR_HIP);
+  if (load_module(c, code, code_key, &c->mod, c->fn_plain)) return bail(PYPDE_ERR_HIP);
   memcpy(c->fn, c->fn_plain, sizeof c->fn);
   if (hipStreamCreate(&c->stream) != hipSuccess)
     return bail(fail(PYPDE_ERR_HIP, "hipStreamCreate failed"));
@@ -1258,9 +1297,10 @@ int pypde_set_guess_model(pypde_ctx* c, const pypde_mlp* m) {
     Spec s = c->spec;
     s.guess = true;
     std::vector<char> code;
-    int rc = compile_code(s, &code);
+    uint64_t key = 0;
+    int rc = compile_code(s, &code, &key);
     if (rc) return rc;
-    if ((rc = load_module(c, code, &c->mod_guess, c->fn_guess))) return rc;
+    if ((rc = load_module(c, code, key, &c->mod_guess, c->fn_guess))) return rc;
   }
   if (!c->gbuf) HIPCHK(hipMalloc((void**)&c->gbuf, c->g.nwc * c->NT * c->V * sizeof(double)));
   memcpy(c->fn, c->fn_guess, sizeof c->fn);
