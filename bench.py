@@ -328,6 +328,40 @@ def cpu_baseline(t_target, threads):
                        f"{os.cpu_count()} CPUs visible), transmissive walls, {el:.1f} s")
 
 
+def same_window(steps, bc):
+    """The GPU over exactly a CPU sample's window: the first `steps` timesteps
+    from t = 0 of the 1024 x 256 channel (inside the x0.2 dt ramp), walls
+    `bc`, the time of the whole call (setup included, as the CPU's)."""
+    u0 = systems.reactive_channel(NX, NY, seed=EVAL_SEED)
+    refl = systems.REACTIVE_REFLECT if bc == "reflective" else None
+    t0 = time.perf_counter()
+    sol = Solver(u0, L, systems.source("reactive2d"), order=ORDER, boundaryTypes=bc, stiff=True,
+                 reflect=refl, newton_tol=1e-12, newton_maxit=100)
+    st = sol.run(1.0, max_steps=steps)
+    sol.close()
+    el = time.perf_counter() - t0
+    return dict(value=NX * NY * st["steps"] / el, unit="cell-updates/s", steps=st["steps"],
+                walls=bc, seconds=el)
+
+
+def cpu_port_baseline(steps):
+    """The repo's C restatement (oracle/, the parity checker) on the GPU leg's
+    own algorithm: reflective walls, per-node Newton to 1e-12, the first
+    `steps` timesteps from t = 0, its OpenMP loops on the host's cores."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import cpu_checkers as cc
+    u0 = systems.reactive_channel(NX, NY, seed=EVAL_SEED)
+    t0 = time.perf_counter()
+    _, _, st, rc = cc.oracle_solve("reactive2d", u0, 1.0, L, 2, ORDER, stiff=True,
+                                   max_steps=steps, reflect=systems.REACTIVE_REFLECT)
+    el = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=NX * NY * st.steps / el, unit="cell-updates/s", cores=threads,
+                kind="port", sample=f"oracle/oracle.c (the C restatement), {NX}x{NY} channel, "
+                f"reflective walls, Newton 1e-12, {st.steps} timesteps from t=0, {el:.1f} s, "
+                f"OpenMP loops on {threads} threads")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -434,11 +468,16 @@ def main():
     if not args.no_ensemble:
         modes = euler_modes_leg(20)
         log(f"euler 512^2 order 4: {modes}")
-    cpu = None
+    cpu = port = None
     if not args.no_cpu:
         # t after args.cpu_steps steps from t = 0, from a short GPU run
         st2, _, _, _ = gpu_run(args.cpu_steps, 0, profile=False)
         cpu = cpu_baseline(st2["t"], threads=min(16, os.cpu_count() or 1))
+        if cpu is not None:
+            # the GPU over the CPU sample's own window and walls
+            cpu["gpu_same_window"] = same_window(args.cpu_steps, "transitive")
+        port = cpu_port_baseline(args.cpu_steps)
+        port["gpu_same_window"] = same_window(args.cpu_steps, "reflective")
     out = {
         "metric": "cell-updates/s, 2D reactive Euler O3: learned+mixed vs plain FP64 @Linf<=1e-8",
         "value": head["value"],
@@ -460,6 +499,7 @@ def main():
                    "cells": cells, "order": ORDER, "nvar": V},
         "roofline": head["roofline"],
         "cpu_baseline": cpu,
+        "cpu_baseline_port": port,
         "speedup_vs_plain": {k: v["value"] / plain["value"] for k, v in legs.items()
                              if v is not None},
         "plain": plain,

@@ -152,7 +152,9 @@ void pypde_ensemble_destroy(pypde_ensemble *e);
  * in_mean / in_rstd, outputs (qh - w) / dt de-normalised by out_std / out_mean.
  * Once attached, every step infers guesses on the matrix cores (BF16, FP32
  * accumulate) before the stiff predictor; Newton still runs to newton_tol and
- * abandons a guess whose residual grows.  NULL detaches. */
+ * abandons a guess whose residual grows.  The weights are read (packed into
+ * BF16 fragments) when the model is attached: attach again after changing
+ * them.  NULL detaches. */
 typedef struct {
   const float *W1t, *b1, *W2t, *b2, *W3t, *b3;
   const float *in_mean, *in_rstd, *out_mean, *out_std;

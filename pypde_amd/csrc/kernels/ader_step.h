@@ -1831,82 +1831,141 @@ DEV bf16x4 wfrag(const float* Wt, int ld, int rows, int cols, int rb, int kb, in
   return a;
 }
 
-// One wave infers 32 cells (lane % 32 = cell); the transposed form
-// H^T = W^T X^T makes each layer's accumulators the next layer's B operand.
+// Weight fragments of the three layers, packed once per attached model by
+// k_mlp_pack in the order k_mlp_guess reads them: W1 [RB_HID][KB_IN], W2
+// [RB_HID][KH], W3 [RB_OUT][KH], each one bf16x4 per lane; then the FP32
+// vectors in_mean, in_rstd, b1, b2, b3, out_mean, out_std.
+constexpr int KH = MLP_HID / 8;                                    // k-blocks of a hidden layer
+constexpr int NFRAG = RB_HID * KB_IN + RB_HID * KH + RB_OUT * KH;  // fragments per model
+constexpr int AUX_IN_MEAN = 0, AUX_IN_RSTD = MLP_IN, AUX_B1 = 2 * MLP_IN,
+              AUX_B2 = AUX_B1 + MLP_HID, AUX_B3 = AUX_B2 + MLP_HID, AUX_OUT_MEAN = AUX_B3 + MLP_OUT,
+              AUX_OUT_STD = AUX_OUT_MEAN + MLP_OUT, NAUX = AUX_OUT_STD + MLP_OUT;
+
+DEV const float* aux_src(const MLPW& m, int i, int* k) {
+  if (i < AUX_IN_RSTD) { *k = i; return m.in_mean; }
+  if (i < AUX_B1) { *k = i - AUX_IN_RSTD; return m.in_rstd; }
+  if (i < AUX_B2) { *k = i - AUX_B1; return m.b1; }
+  if (i < AUX_B3) { *k = i - AUX_B2; return m.b2; }
+  if (i < AUX_OUT_MEAN) { *k = i - AUX_B3; return m.b3; }
+  if (i < AUX_OUT_STD) { *k = i - AUX_OUT_MEAN; return m.out_mean; }
+  *k = i - AUX_OUT_STD;
+  return m.out_std;
+}
+
 extern "C" __global__ void __launch_bounds__(256)
-k_mlp_guess(const double* __restrict__ wh, const StepState* st, MLPW m,
-            double* __restrict__ guess, long nwc) {
-  if (st->done) return;
-  const int lane = threadIdx.x & 63;
-  const long cell = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 32 + (lane & 31);
-  const bool ok = cell < nwc;
-  const double dt = st->dt;
-  // input fragments
-  bf16x4 xb[KB_IN];
-#pragma unroll
-  for (int kb = 0; kb < KB_IN; kb++)
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int k = kb * 8 + 4 * (lane >> 5) + i;
-      float x = 0.0f;
-      if (ok && k < MLP_IN) {
-        const double raw = (k < ND * V) ? wh[(long)k * nwc + cell] : dt;
-        x = ((float)raw - m.in_mean[k]) * m.in_rstd[k];
-      }
-      xb[kb][i] = to_bf16(x);
+k_mlp_pack(MLPW m, bf16x4* __restrict__ pack, float* __restrict__ aux) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < (long)NFRAG * 64) {
+    const int lane = (int)(i % 64), f = (int)(i / 64);
+    if (f < RB_HID * KB_IN)
+      pack[i] = wfrag(m.W1t, MLP_IN, MLP_HID, MLP_IN, f / KB_IN, f % KB_IN, lane);
+    else if (f < RB_HID * KB_IN + RB_HID * KH) {
+      const int h = f - RB_HID * KB_IN;
+      pack[i] = wfrag(m.W2t, MLP_HID, MLP_HID, MLP_HID, h / KH, h % KH, lane);
+    } else {
+      const int h = f - RB_HID * KB_IN - RB_HID * KH;
+      pack[i] = wfrag(m.W3t, MLP_HID, MLP_OUT, MLP_HID, h / KH, h % KH, lane);
     }
-  // layer 1
-  f32x16 h[RB_HID];
-#pragma unroll
-  for (int rb = 0; rb < RB_HID; rb++) {
-#pragma unroll
-    for (int j = 0; j < 16; j++) h[rb][j] = 0.0f;
+  }
+  if (i < NAUX) {
+    int k;
+    const float* src = aux_src(m, (int)i, &k);
+    aux[i] = src[k];
+  }
+}
+
+// Fused inference: a workgroup of four wavefronts stages every weight
+// fragment (84 KB at 46-128-128-135) and the FP32 vectors in LDS once, then
+// each wavefront infers 32 cells at a time (lane % 32 = cell), looping over
+// the grid.  The transposed form H^T = W^T X^T makes each layer's
+// accumulators the next layer's B operands; each hidden activation (bias +
+// SiLU, rounded to BF16) is formed once.
+constexpr int MLP_WAVES = 4;
+extern "C" __global__ void __launch_bounds__(64 * MLP_WAVES)
+k_mlp_guess(const double* __restrict__ wh, const StepState* st, const bf16x4* __restrict__ pack,
+            const float* __restrict__ aux, double* __restrict__ guess, long nwc) {
+  if (st->done) return;
+  __shared__ bf16x4 sw[NFRAG * 64];
+  __shared__ float sa[NAUX];
+  for (int i = threadIdx.x; i < NFRAG * 64; i += 64 * MLP_WAVES) sw[i] = pack[i];
+  for (int i = threadIdx.x; i < NAUX; i += 64 * MLP_WAVES) sa[i] = aux[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const bf16x4* w1 = sw;
+  const bf16x4* w2 = sw + RB_HID * KB_IN * 64;
+  const bf16x4* w3 = w2 + RB_HID * KH * 64;
+  const double dt = st->dt;
+  const long ncb = (nwc + 31) / 32;
+  for (long cb = (long)blockIdx.x * MLP_WAVES + (threadIdx.x >> 6); cb < ncb;
+       cb += (long)gridDim.x * MLP_WAVES) {
+    const long cell = cb * 32 + (lane & 31);
+    const bool ok = cell < nwc;
+    bf16x4 xb[KB_IN];
 #pragma unroll
     for (int kb = 0; kb < KB_IN; kb++)
-      h[rb] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
-          wfrag(m.W1t, MLP_IN, MLP_HID, MLP_IN, rb, kb, lane), xb[kb], h[rb], 0, 0, 0);
-  }
-  // layers 2 and 3 consume SiLU(h + b) as B fragments: k-block kb = rows
-  // 8 kb .. 8 kb + 7 = accumulator block kb / 4, group kb % 4
-  auto hidden_frag = [&](const f32x16* hh, const float* b, int kb) {
-    bf16x4 f;
-    const int rb = kb / 4, g = kb % 4;
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int row = rb * 32 + 8 * g + 4 * (lane >> 5) + r;
-      f[r] = to_bf16(silu(hh[rb][4 * g + r] + b[row]));
+      for (int i = 0; i < 4; i++) {
+        const int k = kb * 8 + 4 * (lane >> 5) + i;
+        float x = 0.0f;
+        if (ok && k < MLP_IN) {
+          const double raw = (k < ND * V) ? wh[(long)k * nwc + cell] : dt;
+          x = ((float)raw - sa[AUX_IN_MEAN + k]) * sa[AUX_IN_RSTD + k];
+        }
+        xb[kb][i] = to_bf16(x);
+      }
+    f32x16 h[RB_HID];
+#pragma unroll
+    for (int rb = 0; rb < RB_HID; rb++) {
+#pragma unroll
+      for (int j = 0; j < 16; j++) h[rb][j] = 0.0f;
+#pragma unroll
+      for (int kb = 0; kb < KB_IN; kb++)
+        h[rb] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w1[(rb * KB_IN + kb) * 64 + lane],
+                                                         xb[kb], h[rb], 0, 0, 0);
     }
-    return f;
-  };
-  f32x16 h2[RB_HID];
+    // k-block kb of the next layer's B operand = rows 8 kb .. 8 kb + 7 =
+    // accumulator block kb / 4, group kb % 4
+    bf16x4 a[KH];
+    auto activate = [&](const f32x16* hh, int boff) {
 #pragma unroll
-  for (int rb = 0; rb < RB_HID; rb++) {
+      for (int kb = 0; kb < KH; kb++) {
+        const int rb = kb / 4, gi = kb % 4;
 #pragma unroll
-    for (int j = 0; j < 16; j++) h2[rb][j] = 0.0f;
+        for (int r = 0; r < 4; r++) {
+          const int row = rb * 32 + 8 * gi + 4 * (lane >> 5) + r;
+          a[kb][r] = to_bf16(silu(hh[rb][4 * gi + r] + sa[boff + row]));
+        }
+      }
+    };
+    activate(h, AUX_B1);
 #pragma unroll
-    for (int kb = 0; kb < MLP_HID / 8; kb++)
-      h2[rb] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
-          wfrag(m.W2t, MLP_HID, MLP_HID, MLP_HID, rb, kb, lane), hidden_frag(h, m.b1, kb), h2[rb],
-          0, 0, 0);
-  }
+    for (int rb = 0; rb < RB_HID; rb++) {
 #pragma unroll
-  for (int rb = 0; rb < RB_OUT; rb++) {
-    f32x16 y;
+      for (int j = 0; j < 16; j++) h[rb][j] = 0.0f;
 #pragma unroll
-    for (int j = 0; j < 16; j++) y[j] = 0.0f;
+      for (int kb = 0; kb < KH; kb++)
+        h[rb] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w2[(rb * KH + kb) * 64 + lane], a[kb],
+                                                         h[rb], 0, 0, 0);
+    }
+    activate(h, AUX_B2);
+#pragma unroll 1
+    for (int rb = 0; rb < RB_OUT; rb++) {
+      f32x16 y;
 #pragma unroll
-    for (int kb = 0; kb < MLP_HID / 8; kb++)
-      y = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
-          wfrag(m.W3t, MLP_HID, MLP_OUT, MLP_HID, rb, kb, lane), hidden_frag(h2, m.b2, kb), y, 0,
-          0, 0);
-    if (ok) {
+      for (int j = 0; j < 16; j++) y[j] = 0.0f;
 #pragma unroll
-      for (int j = 0; j < 16; j++) {
-        const int o = rb * 32 + 8 * (j / 4) + 4 * (lane >> 5) + (j % 4);
-        if (o < MLP_OUT) {
-          const float val = (y[j] + m.b3[o]) * m.out_std[o] + m.out_mean[o];
-          const int p = o % (ND * V);
-          guess[(long)o * nwc + cell] = wh[(long)p * nwc + cell] + dt * (double)val;
+      for (int kb = 0; kb < KH; kb++)
+        y = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w3[(rb * KH + kb) * 64 + lane], a[kb], y, 0,
+                                                     0, 0);
+      if (ok) {
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+          const int o = rb * 32 + 8 * (j / 4) + 4 * (lane >> 5) + (j % 4);
+          if (o < MLP_OUT) {
+            const float val = (y[j] + sa[AUX_B3 + o]) * sa[AUX_OUT_STD + o] + sa[AUX_OUT_MEAN + o];
+            const int p = o % (ND * V);
+            guess[(long)o * nwc + cell] = wh[(long)p * nwc + cell] + dt * (double)val;
+          }
         }
       }
     }

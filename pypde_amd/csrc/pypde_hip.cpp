@@ -272,7 +272,7 @@ int check_spec(const Spec& s, bool secondOrder, int FLUX) {
 enum { K_WENO, K_DT, K_DTFIN, K_PRED, K_FLUX0, K_FLUX1, K_UPDATE, K_ADVANCE, K_A2S, K_S2A,
        K_HARVEST, K_MLPGUESS, K_PRED_STATIC, K_PRED_STATIC2, K_EIG, K_P2R, K_PRED32,
        K_WENO_4x64, K_WENO_16x16, K_WENO_16x32, K_FLUX0_128, K_FLUX0_64, K_FLUX1_128, K_FLUX1_64,
-       K_UPD1, K_UPD2, K_UPD4, K_UPD8, K_PRED_COLD, K_PRED_HOT, K_COUNT };
+       K_UPD1, K_UPD2, K_UPD4, K_UPD8, K_PRED_COLD, K_PRED_HOT, K_MLPPACK, K_COUNT };
 const char* kernel_names[K_COUNT] = {"k_weno",   "k_dt",    "k_dt_finalize", "k_predictor",
                                      "k_flux0",  "k_flux1", "k_update",      "k_advance",
                                      "k_aos_to_soa", "k_soa_to_aos", "k_harvest", "k_mlp_guess",
@@ -281,7 +281,7 @@ const char* kernel_names[K_COUNT] = {"k_weno",   "k_dt",    "k_dt_finalize", "k_
                                      "k_weno_4x64", "k_weno_16x16", "k_weno_16x32",
                                      "k_flux0_128", "k_flux0_64", "k_flux1_128", "k_flux1_64",
                                      "k_update_r1", "k_update_r2", "k_update_r4", "k_update_r8",
-                                     "k_predictor_cold", "k_predictor_hot"};
+                                     "k_predictor_cold", "k_predictor_hot", "k_mlp_pack"};
 
 // Launch variants the autotuner chooses between (every variant of a kernel
 // computes identical results): WENO tiles, flux workgroup sizes, corrector
@@ -327,6 +327,8 @@ struct pypde_ctx {
   bool use_model = false;
   MLPW model;
   double* gbuf = nullptr;   // per-step MLP guesses
+  void* mlp_pack = nullptr;  // the attached model's BF16 weight fragments
+  float* mlp_aux = nullptr;  // ... and its FP32 normalisation and bias vectors
   double* qh = nullptr;     // full predictor output (harvest)
   int n_cu = 0;             // compute units of the device
   hipFunction_t pred_fn = nullptr;
@@ -387,8 +389,8 @@ int load_module(pypde_ctx* c, const std::vector<char>& code, uint64_t key, hipMo
   for (int k = 0; k < K_COUNT; k++) {
     fn[k] = nullptr;
     bool optional = ((k == K_FLUX1 || k == K_FLUX1_128 || k == K_FLUX1_64) && c->D < 2) ||
-                    k == K_MLPGUESS || (k == K_PRED32 && !c->spec.mixed) || k == K_PRED_COLD ||
-                    k == K_PRED_HOT;
+                    k == K_MLPGUESS || k == K_MLPPACK || (k == K_PRED32 && !c->spec.mixed) ||
+                    k == K_PRED_COLD || k == K_PRED_HOT;
     if (hipModuleGetFunction(&fn[k], *mod, kernel_names[k]) != hipSuccess) {
       fn[k] = nullptr;
       if (!optional) return fail(PYPDE_ERR_HIP, "kernel %s missing", kernel_names[k]);
@@ -480,11 +482,15 @@ int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
   if (c->use_model) {
     const double* w = c->wh;
     const StepState* st = c->st;
-    MLPW m = c->model;
+    const void* pk = c->mlp_pack;
+    const float* ax = c->mlp_aux;
     double* gb = c->gbuf;
     long nwc = g.nwc;
-    std::vector<void*> a = {&w, &st, &m, &gb, &nwc};
-    if ((rc = launch(c, K_MLPGUESS, dim3((unsigned)cdiv(g.nwc, 128)), dim3(256), a))) return rc;
+    std::vector<void*> a = {&w, &st, &pk, &ax, &gb, &nwc};
+    // four wavefronts of 32 cells per workgroup, at most one workgroup per CU
+    // (the weights take most of a CU's LDS), looping over the cells
+    const long blocks = std::min<long>(cdiv(cdiv(g.nwc, 32), 4), c->n_cu > 0 ? c->n_cu : 256);
+    if ((rc = launch(c, K_MLPGUESS, dim3((unsigned)blocks), dim3(256), a))) return rc;
   }
   if (c->spec.mixed) {
     // FP32 phase: from the learned guess, a caller's guess or the plain one;
@@ -650,6 +656,8 @@ void pypde_destroy(pypde_ctx* c) {
     if (p) (void)hipFree(p);
   if (c->st) (void)hipFree(c->st);
   if (c->gbuf) (void)hipFree(c->gbuf);
+  if (c->mlp_pack) (void)hipFree(c->mlp_pack);
+  if (c->mlp_aux) (void)hipFree(c->mlp_aux);
   if (c->hot) (void)hipFree(c->hot);
   if (c->qh) (void)hipFree(c->qh);
   release_module(c->device, c->mod);
@@ -1303,9 +1311,26 @@ int pypde_set_guess_model(pypde_ctx* c, const pypde_mlp* m) {
     if ((rc = load_module(c, code, key, &c->mod_guess, c->fn_guess))) return rc;
   }
   if (!c->gbuf) HIPCHK(hipMalloc((void**)&c->gbuf, c->g.nwc * c->NT * c->V * sizeof(double)));
-  memcpy(c->fn, c->fn_guess, sizeof c->fn);
+  // pack the weights once: BF16 fragments in the inference kernel's order
+  // (its sizes follow ader_step.h: hidden width 128)
+  const int n_in = c->ND * c->V + 1, n_out = c->NT * c->V, hid = 128;
+  const long nfrag = (long)(hid / 32) * ((n_in + 7) / 8) + (long)(hid / 32) * (hid / 8) +
+                     (long)((n_out + 31) / 32) * (hid / 8);
+  const long naux = 2L * n_in + 2L * hid + 3L * n_out;
+  if (!c->mlp_pack) HIPCHK(hipMalloc(&c->mlp_pack, nfrag * 64 * 8));
+  if (!c->mlp_aux) HIPCHK(hipMalloc((void**)&c->mlp_aux, naux * sizeof(float)));
   c->model = {m->W1t, m->b1, m->W2t, m->b2, m->W3t, m->b3, m->in_mean, m->in_rstd, m->out_mean,
               m->out_std};
+  memcpy(c->fn, c->fn_guess, sizeof c->fn);
+  {
+    MLPW mw = c->model;
+    void* pk = c->mlp_pack;
+    float* ax = c->mlp_aux;
+    std::vector<void*> a = {&mw, &pk, &ax};
+    int rc = launch(c, K_MLPPACK, dim3((unsigned)cdiv(std::max(nfrag * 64, naux), 256)), dim3(256), a);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
   c->use_model = true;
   return 0;
 }

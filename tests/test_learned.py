@@ -99,12 +99,14 @@ def make_solver(seed, nx=256, ny=64):
                   reflect=systems.REACTIVE_REFLECT)
 
 
-def test_learned_guess_trains_and_saves_iterations():
+def test_learned_guess_trains_infers_and_costs_iterations():
     """Train on pairs the solver generates (seeds 0, 1), evaluate on seed 99:
     the fused inference kernel reproduces the training forward pass, its
-    guesses are closer to the converged predictor than the plain guess, Newton
-    needs fewer iterations than from the plain guess, and the answer is the
-    same to the Newton tolerance scale."""
+    guesses are closer to the converged predictor than the plain guess, and
+    the answer is the same to the Newton tolerance scale.  What the network
+    changes in the Newton iteration count is asserted as measured: from its
+    guesses the lagged-flux Newton needs more iterations than from the plain
+    guess (1.5x to 2x on this channel), not fewer."""
     import ctypes as C
     from pypde_amd.learned import Dev, GuessMLP, _bind, harvest
     Xs, Ys = [], []
@@ -158,13 +160,16 @@ def test_learned_guess_trains_and_saves_iterations():
     sl = learned.step(12)
     ul = learned.state()
     learned.close()
-    # What the network gains: Newton iterations against the plain guess.  The
-    # stiff Newton lags the flux terms, so it converges linearly and only
+    # What the network changes: Newton iterations against the plain guess.
+    # The stiff Newton lags the flux terms, so it converges linearly and only
     # guesses within ~1e-4 of the solution save iterations (see the oracle
     # study in test_oracle.py); a BF16 network is far from that, and its
-    # guesses cost iterations.  Recorded, not asserted.
+    # guesses cost iterations: asserted within the band measured on the
+    # MI355X (ratio 1.63 on this channel, 1.78 on the full 1024 x 256 one).
+    ratio = sl["pred_iters"] / sp["pred_iters"]
     print("plain iters", sp["pred_iters"], "learned iters", sl["pred_iters"], "fallbacks",
-          sl["guess_fallbacks"], "ratio", sl["pred_iters"] / sp["pred_iters"])
+          sl["guess_fallbacks"], "ratio", ratio)
+    assert 1.0 < ratio < 2.5
     assert sl["pred_capped"] == 0 and sp["pred_capped"] == 0
     assert sl["pred_cells"] == sp["pred_cells"]
     # Both runs solve every predictor to the Newton tolerance (1e-12 relative)
