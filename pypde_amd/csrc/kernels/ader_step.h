@@ -1874,30 +1874,24 @@ k_mlp_pack(MLPW m, bf16x4* __restrict__ pack, float* __restrict__ aux) {
   }
 }
 
-// Fused inference: a workgroup of four wavefronts stages every weight
-// fragment (84 KB at 46-128-128-135) and the FP32 vectors in LDS once, then
-// each wavefront infers 32 cells at a time (lane % 32 = cell), looping over
-// the grid.  The transposed form H^T = W^T X^T makes each layer's
-// accumulators the next layer's B operands; each hidden activation (bias +
-// SiLU, rounded to BF16) is formed once.
-constexpr int MLP_WAVES = 4;
-extern "C" __global__ void __launch_bounds__(64 * MLP_WAVES)
+// Fused inference: one wavefront infers 32 cells (lane % 32 = cell), four
+// wavefronts per workgroup; the packed BF16 fragments (84 KB at 46-128-128-135)
+// are read from global memory, where every wavefront finds them in the L2.
+// The transposed form H^T = W^T X^T makes each layer's accumulators the next
+// layer's B operands; each hidden activation (bias + SiLU, rounded to BF16)
+// is formed once.
+extern "C" __global__ void __launch_bounds__(256)
 k_mlp_guess(const double* __restrict__ wh, const StepState* st, const bf16x4* __restrict__ pack,
             const float* __restrict__ aux, double* __restrict__ guess, long nwc) {
   if (st->done) return;
-  __shared__ bf16x4 sw[NFRAG * 64];
-  __shared__ float sa[NAUX];
-  for (int i = threadIdx.x; i < NFRAG * 64; i += 64 * MLP_WAVES) sw[i] = pack[i];
-  for (int i = threadIdx.x; i < NAUX; i += 64 * MLP_WAVES) sa[i] = aux[i];
-  __syncthreads();
   const int lane = threadIdx.x & 63;
-  const bf16x4* w1 = sw;
-  const bf16x4* w2 = sw + RB_HID * KB_IN * 64;
+  const bf16x4* w1 = pack;
+  const bf16x4* w2 = pack + RB_HID * KB_IN * 64;
   const bf16x4* w3 = w2 + RB_HID * KH * 64;
+  const float* sa = aux;
   const double dt = st->dt;
-  const long ncb = (nwc + 31) / 32;
-  for (long cb = (long)blockIdx.x * MLP_WAVES + (threadIdx.x >> 6); cb < ncb;
-       cb += (long)gridDim.x * MLP_WAVES) {
+  {
+    const long cb = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
     const long cell = cb * 32 + (lane & 31);
     const bool ok = cell < nwc;
     bf16x4 xb[KB_IN];

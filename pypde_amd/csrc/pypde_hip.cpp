@@ -487,10 +487,8 @@ int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
     double* gb = c->gbuf;
     long nwc = g.nwc;
     std::vector<void*> a = {&w, &st, &pk, &ax, &gb, &nwc};
-    // four wavefronts of 32 cells per workgroup, at most one workgroup per CU
-    // (the weights take most of a CU's LDS), looping over the cells
-    const long blocks = std::min<long>(cdiv(cdiv(g.nwc, 32), 4), c->n_cu > 0 ? c->n_cu : 256);
-    if ((rc = launch(c, K_MLPGUESS, dim3((unsigned)blocks), dim3(256), a))) return rc;
+    // four wavefronts of 32 cells per workgroup
+    if ((rc = launch(c, K_MLPGUESS, dim3((unsigned)cdiv(g.nwc, 128)), dim3(256), a))) return rc;
   }
   if (c->spec.mixed) {
     // FP32 phase: from the learned guess, a caller's guess or the plain one;
