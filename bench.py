@@ -168,6 +168,7 @@ def gpu_run(steps, warmup, profile, net=None, mixed=False):
     st = sol.step(steps)
     t1 = time.perf_counter()
     state = sol.state()
+    st["tuning"] = sol.tuning()
     sol.close()
     return st, t1 - t0, w, state
 
@@ -233,22 +234,31 @@ def window(st, w):
     return {k: st[k] - (w[k] if w else 0) for k in keys}
 
 
+PREDICTOR_KERNELS = {"split": ["k_predictor_cold", "k_predictor_hot"],
+                     "static": ["k_predictor_static"], "queue": ["k_predictor"]}
+
+
 def predictor_roofline(st_k, win, K, guess=False):
-    """Roofline of the dominant kernel, the predictor: its algorithmic FP64
-    flops over the timed steps (predictor_flops) / its HIP-event time over the
-    same steps, against MI355X's FP64 peak.  It runs on the vector ALUs, whose
-    FP64 peak equals the FP64 matrix-core peak (78.6 TFLOP/s); 'mfma' is the
-    contract's word for the compute bound."""
+    """Roofline of the dominant stage, the predictor (its kernels as the
+    autotuner chose them: the cold and hot passes of the split schedule, or one
+    kernel): its algorithmic FP64 flops over the timed steps
+    (predictor_flops) / its HIP-event time over the same steps, against
+    MI355X's FP64 peak.  It runs on the vector ALUs, whose FP64 peak equals the
+    FP64 matrix-core peak (78.6 TFLOP/s); 'mfma' is the contract's word for the
+    compute bound."""
     pred_ms = st_k["ms_kernel"]["predictor"]
     flops = predictor_flops(win, guess)
     achieved = flops / (pred_ms * 1e-3) if pred_ms > 0 else 0.0
-    traffic = profile_traffic("k_predictor_static")
+    kernels = PREDICTOR_KERNELS[st_k["tuning"]["predictor"]] if not guess else \
+        ["k_predictor_static"]
+    parts = [profile_traffic(k) for k in kernels]
+    traffic = sum(parts) if all(x is not None for x in parts) else None
     return dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK_FP64 / 1e12, unit="TFLOP/s",
                 frac=achieved / PEAK_FP64, traffic=traffic,
                 traffic_source=("profiles/reactive_channel_1024x256_kernels.md (rocprofv3 "
                                 "FETCH_SIZE + WRITE_SIZE per launch, copy-calibrated)")
                 if traffic else None,
-                kernel="k_predictor_static", pipe="fp64 vector ALU (no MFMA instructions)",
+                kernel="+".join(kernels), pipe="fp64 vector ALU (no MFMA instructions)",
                 avg_launch_ms=pred_ms / K, flops_per_launch=flops / K,
                 survey_model_flops_per_launch=survey_predictor_flops(win) / K,
                 node_corrections_full=win["pred_node_full"],
@@ -260,6 +270,10 @@ def stencil_rooflines(st_k, K, N=ORDER, D=D, V=V, nx=NX, ny=NY):
     with its algorithmic bytes per launch: every input read once, every
     output written once (w-cells include the ghost layer)."""
     ms = {k: v / K for k, v in st_k["ms_kernel"].items()}
+    tun = st_k["tuning"]
+    names = {"k_weno": "k_weno" + {"8x32": ""}.get(tun["weno_tile"], "_" + tun["weno_tile"]),
+             "k_update": {"cell": "k_update"}.get(tun["update"],
+                                                 "k_update_r" + tun["update"][-1:])}
     ncell, nwc = nx * ny, (nx + 2) * (ny + 2)
     ND = N ** D
     nf = [(nx + 1) * ny, nx * (ny + 1)]
@@ -273,10 +287,10 @@ def stencil_rooflines(st_k, K, N=ORDER, D=D, V=V, nx=NX, ny=NY):
                        ("k_flux0+k_flux1", b_flux, ms["flux0"] + ms["flux1"]),
                        ("fv_stage", b_fv, ms["flux0"] + ms["flux1"] + ms["update"])):
         bw = b / (t * 1e-3) if t > 0 else 0.0
+        kn = names.get(name, name)
         out[name] = dict(bound="hbm", achieved=bw / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s",
-                         frac=bw / PEAK_HBM, bytes_per_launch=b, avg_launch_ms=t,
-                         traffic=profile_traffic(name) if name.startswith("k_") and "+" not in name
-                         else None)
+                         frac=bw / PEAK_HBM, bytes_per_launch=b, avg_launch_ms=t, kernel=kn,
+                         traffic=profile_traffic(kn) if kn in names.values() else None)
     return out
 
 
@@ -374,6 +388,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the ensemble and the 512 x 512 Euler legs")
     ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--only", choices=["ensemble", "euler512"],
+                    help="run only that leg and print its JSON (for kernel traces of it)")
     ap.add_argument("--train-steps", type=int, default=12)
     ap.add_argument("--train-pairs", type=int, default=2_000_000)
     ap.add_argument("--epochs", type=int, default=6)
@@ -396,6 +412,12 @@ def main():
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     K, W = args.steps, args.warmup
+    if args.only == "ensemble":
+        print(json.dumps({"ensemble": ensemble_leg(K)}))
+        return
+    if args.only == "euler512":
+        print(json.dumps({"euler2d_512_order4": euler_modes_leg(K)}))
+        return
     cells = NX * NY
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     # timed run without per-kernel events; a second, profiled run of the same
