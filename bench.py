@@ -235,6 +235,7 @@ def window(st, w):
 
 
 PREDICTOR_KERNELS = {"split": ["k_predictor_cold", "k_predictor_hot"],
+                     "split3": ["k_predictor_cold3", "k_predictor_hot"],
                      "static": ["k_predictor_static"], "queue": ["k_predictor"]}
 
 

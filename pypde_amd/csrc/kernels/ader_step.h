@@ -1092,6 +1092,17 @@ k_predictor_cold(const double* __restrict__ wh, double* __restrict__ tr, double*
   predictor_static_body<1>(sm, wh, nullptr, tr, vol, qh, st, g, newton_tol, newton_maxit,
                            nullptr, hot);
 }
+// The cold pass with the register budget of three wavefronts per SIMD (168
+// registers; whatever does not fit spills), for the autotuner to weigh
+// against the two-wavefront build.
+extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3)))
+k_predictor_cold3(const double* __restrict__ wh, double* __restrict__ tr, double* __restrict__ vol,
+                  double* __restrict__ qh, StepState* st, Geom g, double newton_tol,
+                  int newton_maxit, unsigned int* __restrict__ hot) {
+  __shared__ PredLDS sm;
+  predictor_static_body<1>(sm, wh, nullptr, tr, vol, qh, st, g, newton_tol, newton_maxit,
+                           nullptr, hot);
+}
 extern "C" __global__ void __launch_bounds__(64)
 k_predictor_hot(const double* __restrict__ wh, double* __restrict__ tr, double* __restrict__ vol,
                 double* __restrict__ qh, StepState* st, Geom g, double newton_tol,

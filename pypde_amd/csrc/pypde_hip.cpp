@@ -272,7 +272,7 @@ int check_spec(const Spec& s, bool secondOrder, int FLUX) {
 enum { K_WENO, K_DT, K_DTFIN, K_PRED, K_FLUX0, K_FLUX1, K_UPDATE, K_ADVANCE, K_A2S, K_S2A,
        K_HARVEST, K_MLPGUESS, K_PRED_STATIC, K_PRED_STATIC2, K_EIG, K_P2R, K_PRED32,
        K_WENO_4x64, K_WENO_16x16, K_WENO_16x32, K_FLUX0_128, K_FLUX0_64, K_FLUX1_128, K_FLUX1_64,
-       K_UPD1, K_UPD2, K_UPD4, K_UPD8, K_PRED_COLD, K_PRED_HOT, K_MLPPACK, K_COUNT };
+       K_UPD1, K_UPD2, K_UPD4, K_UPD8, K_PRED_COLD, K_PRED_HOT, K_MLPPACK, K_PRED_COLD3, K_COUNT };
 const char* kernel_names[K_COUNT] = {"k_weno",   "k_dt",    "k_dt_finalize", "k_predictor",
                                      "k_flux0",  "k_flux1", "k_update",      "k_advance",
                                      "k_aos_to_soa", "k_soa_to_aos", "k_harvest", "k_mlp_guess",
@@ -281,14 +281,16 @@ const char* kernel_names[K_COUNT] = {"k_weno",   "k_dt",    "k_dt_finalize", "k_
                                      "k_weno_4x64", "k_weno_16x16", "k_weno_16x32",
                                      "k_flux0_128", "k_flux0_64", "k_flux1_128", "k_flux1_64",
                                      "k_update_r1", "k_update_r2", "k_update_r4", "k_update_r8",
-                                     "k_predictor_cold", "k_predictor_hot", "k_mlp_pack"};
+                                     "k_predictor_cold", "k_predictor_hot", "k_mlp_pack",
+                                     "k_predictor_cold3"};
 
 // Launch variants the autotuner chooses between (every variant of a kernel
 // computes identical results): WENO tiles, flux workgroup sizes, corrector
 // mappings (one thread per cell and all variables, or one variable and R
 // x-rows per thread), predictor schedules (static, work queue, and for the
-// stiff predictor with the plain guess the cold/hot split).
-constexpr int N_WENO_V = 4, N_FLUX_V = 3, N_UPD_V = 5, N_PRED_V = 3;
+// stiff predictor with the plain guess the cold/hot split, its cold pass
+// built for two or for three wavefronts per SIMD).
+constexpr int N_WENO_V = 4, N_FLUX_V = 3, N_UPD_V = 5, N_PRED_V = 4;
 const int WENO_TILE[N_WENO_V][2] = {{8, 32}, {4, 64}, {16, 16}, {16, 32}};
 const int WENO_KID[N_WENO_V] = {K_WENO, K_WENO_4x64, K_WENO_16x16, K_WENO_16x32};
 const int FLUX_BS[N_FLUX_V] = {256, 128, 64};
@@ -337,6 +339,7 @@ struct pypde_ctx {
   bool pred_w2 = false;     // the two-wavefronts-per-SIMD register budget
   bool pred_fixed = false;  // schedule fixed by PYPDE_PRED_SCHEDULE (not tuned)
   bool pred_split = false;  // cold pass over every cell, hot pass over the cells it hands on
+  bool pred_cold3 = false;  // ... with the cold pass built for three wavefronts per SIMD
   unsigned int* hot = nullptr;  // the hot pass's cell list (nwc entries)
   struct {
     int weno = 0, flux = 0, update = 0;  // variant indices (WENO_KID, FLUX_KID, UPD_KID)
@@ -390,7 +393,7 @@ int load_module(pypde_ctx* c, const std::vector<char>& code, uint64_t key, hipMo
     fn[k] = nullptr;
     bool optional = ((k == K_FLUX1 || k == K_FLUX1_128 || k == K_FLUX1_64) && c->D < 2) ||
                     k == K_MLPGUESS || k == K_MLPPACK || (k == K_PRED32 && !c->spec.mixed) ||
-                    k == K_PRED_COLD || k == K_PRED_HOT;
+                    k == K_PRED_COLD || k == K_PRED_HOT || k == K_PRED_COLD3;
     if (hipModuleGetFunction(&fn[k], *mod, kernel_names[k]) != hipSuccess) {
       fn[k] = nullptr;
       if (!optional) return fail(PYPDE_ERR_HIP, "kernel %s missing", kernel_names[k]);
@@ -516,7 +519,8 @@ int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
       unsigned int* hl = c->hot;
       std::vector<void*> b = {&w, &t, &v, &qh, &st, &g, &tol, &maxit, &hl};
       const unsigned blocks = (unsigned)cdiv(g.nwc, c->CPW);
-      if ((rc = launch(c, K_PRED_COLD, dim3(blocks), dim3(64), b))) return rc;
+      const int kc = (c->pred_cold3 && c->fn[K_PRED_COLD3]) ? K_PRED_COLD3 : K_PRED_COLD;
+      if ((rc = launch(c, kc, dim3(blocks), dim3(64), b))) return rc;
       if ((rc = launch(c, K_PRED_HOT, dim3(blocks), dim3(64), b))) return rc;
     } else if (c->pred_static) {
       if ((rc = launch(c, c->pred_w2 ? K_PRED_STATIC2 : K_PRED_STATIC,
@@ -859,7 +863,8 @@ void apply_tuning(pypde_ctx* c, const std::vector<int>& v) {
   c->tune.update = (v[2] >= 0 && v[2] < N_UPD_V) ? v[2] : 0;
   if (!c->pred_fixed) {
     c->pred_static = v[3] != 1;
-    c->pred_split = v[3] == 2 && c->fn[K_PRED_COLD] != nullptr;
+    c->pred_split = (v[3] == 2 || v[3] == 3) && c->fn[K_PRED_COLD] != nullptr;
+    c->pred_cold3 = v[3] == 3;
   }
 }
 
@@ -907,7 +912,7 @@ int autotune(pypde_ctx* c) {
   HIPCHK(hipMemcpyAsync(c->tmp, c->u, ub, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(st_bak, c->st, sizeof(StepState), hipMemcpyDeviceToDevice, c->stream));
   double* keep_ret = c->ret;
-  const bool keep_static = c->pred_static, keep_split = c->pred_split;
+  const bool keep_static = c->pred_static, keep_split = c->pred_split, keep_c3 = c->pred_cold3;
   c->ret = nullptr;
   const int nw = c->D == 2 ? N_WENO_V : 1;
   const int np = c->pred_fixed ? 1 : (c->fn[K_PRED_COLD] && c->hot ? N_PRED_V : 2);
@@ -922,7 +927,8 @@ int autotune(pypde_ctx* c) {
     c->tune.update = tk % N_UPD_V;
     if (!c->pred_fixed) {
       c->pred_static = (tk % np) != 1;
-      c->pred_split = (tk % np) == 2;
+      c->pred_split = (tk % np) >= 2;
+      c->pred_cold3 = (tk % np) == 3;
     }
     rc = enqueue_step(c, false, &evs[k]);
     if (!rc && hipMemcpyAsync(c->u, c->tmp, ub, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
@@ -958,6 +964,7 @@ int autotune(pypde_ctx* c) {
   c->ret = keep_ret;
   c->pred_static = keep_static;
   c->pred_split = keep_split;
+  c->pred_cold3 = keep_c3;
   if (rc) return rc;
   auto best = [](const double* t, int n) {
     int b = 0;
@@ -1406,7 +1413,7 @@ int pypde_get_tuning(pypde_ctx* c, int* choices, double* ms) {
     choices[0] = c->tune.weno;
     choices[1] = c->tune.flux;
     choices[2] = c->tune.update;
-    choices[3] = c->pred_split ? 2 : (c->pred_static ? 0 : 1);
+    choices[3] = c->pred_split ? (c->pred_cold3 ? 3 : 2) : (c->pred_static ? 0 : 1);
     choices[4] = c->tune.source;
   }
   if (ms)
