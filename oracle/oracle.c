@@ -1180,31 +1180,53 @@ int oracle_boundaries(double *ub, const double *u, const int *nX, int ndim,
 
 /* One line window d[0..2N-2] (stride `st`) -> N nodal values (weno.cpp:26-59):
  * for each stencil c = M^-1 data, p = c^T SIG c + EPS, o = LAM / p^8,
- * num += o c, den += o; result num / den. */
+ * num += o c, den += o; result num / den.  With at most three stencils the
+ * weights are multiplied through by prod_r p_r^8 (o_s' = LAM_s prod_{r != s}
+ * p_r^8, never below 1e-224) and the sums scaled by one reciprocal, which
+ * saves a division per stencil; where that product form overflows the
+ * division form is used.  Both give the reference's weights to a few ulps
+ * (held to the reference's own weno_solver to 1e-12 at N = 3). */
 static void weno_line(const Tab *T, const double *d, long st, double *out) {
-  int N = T->N;
-  double num[MAXN], den = 0.0;
-  for (int k = 0; k < N; k++) num[k] = 0.0;
-  for (int s = 0; s < T->nsten; s++) {
+  int N = T->N, NS = T->nsten;
+  double num[MAXN], den = 0.0, c[4][MAXN], P[4];
+  for (int s = 0; s < NS; s++) {
     const double *dd = d + (long)T->soff[s] * st;
-    double c[MAXN];
     for (int k = 0; k < N; k++) {
       double acc = T->winv[s][k][0] * dd[0];
       for (int j = 1; j < N; j++) acc = fma(T->winv[s][k][j], dd[(long)j * st], acc);
-      c[k] = acc;
+      c[s][k] = acc;
     }
     double p = 0.0;
     for (int k = 0; k < N; k++) {
-      double t = T->sig[k][0] * c[0];
-      for (int j = 1; j < N; j++) t = fma(T->sig[k][j], c[j], t);
-      p = (k == 0) ? c[0] * t : fma(c[k], t, p);
+      double t = T->sig[k][0] * c[s][0];
+      for (int j = 1; j < N; j++) t = fma(T->sig[k][j], c[s][j], t);
+      p = (k == 0) ? c[s][0] * t : fma(c[s][k], t, p);
     }
     p = p + WENO_EPS;
     double p2 = p * p;
     double p4 = p2 * p2;
-    double p8 = p4 * p4;
-    double o = T->slam[s] / p8;
-    for (int k = 0; k < N; k++) num[k] = fma(o, c[k], num[k]);
+    P[s] = p4 * p4;
+  }
+  if (NS <= 3) {
+    for (int k = 0; k < N; k++) num[k] = 0.0;
+    for (int s = 0; s < NS; s++) {
+      double o = T->slam[s];
+      for (int r = 0; r < NS; r++)
+        if (r != s) o = o * P[r];
+      for (int k = 0; k < N; k++) num[k] = fma(o, c[s][k], num[k]);
+      den = den + o;
+    }
+    if (den <= DBL_MAX) {
+      double rden = 1.0 / den;
+      for (int k = 0; k < N; k++) out[k] = num[k] * rden;
+      return;
+    }
+    den = 0.0;
+  }
+  for (int k = 0; k < N; k++) num[k] = 0.0;
+  for (int s = 0; s < NS; s++) {
+    double o = T->slam[s] / P[s];
+    for (int k = 0; k < N; k++) num[k] = fma(o, c[s][k], num[k]);
     den = den + o;
   }
   for (int k = 0; k < N; k++) out[k] = num[k] / den;

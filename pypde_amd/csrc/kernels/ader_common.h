@@ -44,38 +44,70 @@ DEV double dsqrt(double a) { return __builtin_sqrt(a); }
 template <int d> DEV constexpr int kdig(int idx) { return (idx / ipow_c(N, D - 1 - d)) % N; }
 
 // ------------------------------------------------------------------- WENO
-// One line window (2N-1 values, `st` apart) -> N nodal values (weno.cpp:26-59).
+// One line window (2N-1 values, `st` apart) -> N nodal values (weno.cpp:26-59):
+// per stencil c = M^-1 data, p = c^T SIG c + eps, weight o = lambda / p^8, and
+// the result sum o c / sum o.  With at most three stencils (N <= 3) the
+// weights are formed without a division per stencil: each o is multiplied
+// through by prod_r p_r^8, o_s' = lambda_s prod_{r != s} p_r^8 (at least
+// (1e-14)^16 = 1e-224, so never below the normal range), and the sums are
+// scaled by one reciprocal.  Where that product form overflows (p above
+// about 1e19) the division form is used.  Either form gives the same weights
+// to a few ulps.
 template <typename Get>
 DEV void weno_line(Get get, double* out) {
-  double num[N];
+  constexpr int NS = T_NSTEN;
+  double c[NS][N], P[NS];
 #pragma unroll
-  for (int k = 0; k < N; k++) num[k] = 0.0;
-  double den = 0.0;
-#pragma unroll
-  for (int s = 0; s < T_NSTEN; s++) {
-    double c[N];
+  for (int s = 0; s < NS; s++) {
 #pragma unroll
     for (int k = 0; k < N; k++) {
       double acc = T_WINV[s][k][0] * get(T_SOFF[s]);
 #pragma unroll
       for (int j = 1; j < N; j++) acc = dfma(T_WINV[s][k][j], get(T_SOFF[s] + j), acc);
-      c[k] = acc;
+      c[s][k] = acc;
     }
     double p = 0.0;
 #pragma unroll
     for (int k = 0; k < N; k++) {
-      double t = T_SIG[k][0] * c[0];
+      double t = T_SIG[k][0] * c[s][0];
 #pragma unroll
-      for (int j = 1; j < N; j++) t = dfma(T_SIG[k][j], c[j], t);
-      p = (k == 0) ? c[0] * t : dfma(c[k], t, p);
+      for (int j = 1; j < N; j++) t = dfma(T_SIG[k][j], c[s][j], t);
+      p = (k == 0) ? c[s][0] * t : dfma(c[s][k], t, p);
     }
     p = p + WENO_EPS;
     double p2 = p * p;
     double p4 = p2 * p2;
-    double p8 = p4 * p4;
-    double o = T_SLAM[s] / p8;
+    P[s] = p4 * p4;
+  }
+  double num[N], den = 0.0;
+  if constexpr (NS <= 3) {
 #pragma unroll
-    for (int k = 0; k < N; k++) num[k] = dfma(o, c[k], num[k]);
+    for (int k = 0; k < N; k++) num[k] = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      double o = T_SLAM[s];
+#pragma unroll
+      for (int r = 0; r < NS; r++)
+        if (r != s) o = o * P[r];
+#pragma unroll
+      for (int k = 0; k < N; k++) num[k] = dfma(o, c[s][k], num[k]);
+      den = den + o;
+    }
+    if (den <= 1.7976931348623157e308) {
+      const double rden = 1.0 / den;
+#pragma unroll
+      for (int k = 0; k < N; k++) out[k] = num[k] * rden;
+      return;
+    }
+    den = 0.0;  // product form overflowed (or NaN): the division form below
+  }
+#pragma unroll
+  for (int k = 0; k < N; k++) num[k] = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const double o = T_SLAM[s] / P[s];
+#pragma unroll
+    for (int k = 0; k < N; k++) num[k] = dfma(o, c[s][k], num[k]);
     den = den + o;
   }
 #pragma unroll

@@ -10,7 +10,8 @@ solves.npz  pde_solver end states: C0 Sod (order 3; Rusanov, Roe and Osher
             changes, 24x24 periodic 2D Euler, orders 2-4, the viscous
             path (advection-diffusion 1D, Navier-Stokes 2D) and the
             17-variable GPR Riemann problem (Picard and stiff)
-            (python tests/golden/make_golden.py gpr refreshes only those)
+            (python tests/golden/make_golden.py gpr refreshes only those,
+            ... sodflux only the Roe / Osher Sod entries)
 """
 import os
 import sys
@@ -63,17 +64,7 @@ def main():
         noises.append(to.relerr(ur2, ur))
     s["sod_final"] = ur
     s["sod_noise"] = np.array(max(noises))
-    # Roe (1) and Osher (2) fluxes on the same problem, with their own noise
-    for flux, name in ((1, "roe"), (2, "osher")):
-        ur = to.ref_sod_flux(u, flux)
-        fl_noises = []
-        for _ in range(2):
-            u2 = u.copy()
-            m = rng.random(u.shape) < 0.5
-            u2[m] = np.nextafter(u2[m], np.inf)
-            fl_noises.append(to.relerr(to.ref_sod_flux(u2, flux), ur))
-        s[f"sod_{name}_final"] = ur
-        s[f"sod_{name}_noise"] = np.array(max(fl_noises))
+    s.update(sod_flux_vectors())
     for N in [2, 3, 4]:
         ur, _ = cc.ref_solve("euler2d", to.euler2d_field(24, 24, noise=0.0), 0.05, [1, 1], 1, N,
                              nthreads=8)
@@ -85,6 +76,19 @@ def main():
     s.update(gpr_vectors())
     np.savez_compressed(os.path.join(HERE, "solves.npz"), **s)
     print("golden vectors written:", sorted(os.listdir(HERE)))
+
+
+def sod_flux_vectors():
+    """Roe (1) and Osher (2) fluxes on Sod, with the reference's own
+    sensitivity to 1-ulp input changes (to.sod_flux_noise: single entries and
+    all entries at random; the largest movement is kept)."""
+    out = {}
+    u = to.sod1d()
+    for flux, name in ((1, "roe"), (2, "osher")):
+        ur = to.ref_sod_flux(u, flux)
+        out[f"sod_{name}_final"] = ur
+        out[f"sod_{name}_noise"] = np.array(max(to.sod_flux_noise(flux, ur)))
+    return out
 
 
 def gpr_vectors():
@@ -99,7 +103,14 @@ def gpr_vectors():
 
 
 if __name__ == "__main__":
-    if sys.argv[1:] == ["gpr"]:
+    if sys.argv[1:] == ["sodflux"]:
+        # refresh only the Roe / Osher Sod entries of solves.npz
+        path = os.path.join(HERE, "solves.npz")
+        s = dict(np.load(path, allow_pickle=False))
+        s.update(sod_flux_vectors())
+        np.savez_compressed(path, **s)
+        print("Roe / Osher Sod golden vectors written")
+    elif sys.argv[1:] == ["gpr"]:
         # add or refresh only the GPR entries of solves.npz
         path = os.path.join(HERE, "solves.npz")
         s = dict(np.load(path, allow_pickle=False))
