@@ -66,7 +66,9 @@ typedef struct {
   int nan_step;             /* first step with NaNs (0: none) */
   double ms_loop;           /* wall time of the step loop */
   double ms_kernel[8];      /* profile: summed time per kernel (weno, dt, learned guess,
-                               predictor, flux0, flux1, update, bookkeeping) */
+                               predictor, flux0, flux1, update; the last slot is kept
+                               for layout and holds only the cost of one event pair,
+                               the step's bookkeeping running inside dt and update) */
   long pred_node_full;      /* stiff predictor: per-node Newton corrections that solved
                                with the source Jacobian, summed over nodes and steps */
   long pred_node_short;     /* ... and those that took the A = 0 shortcut (max |dt dS/dQ|

@@ -38,12 +38,16 @@ struct StepState {
   unsigned long long pred_nfull, pred_nshort;
   unsigned long long pred_iters32;  // mixed precision: FP32-phase iterations over cells
   unsigned long long hot_count;     // split stiff predictor: cells handed to the hot pass
+  // workgroups of k_dt and of the corrector that have finished this step: the
+  // last one of each folds the step's partial results (dt; the step's
+  // bookkeeping) and resets its ticket
+  unsigned int dt_ticket, step_ticket;
   // predictor statistics spread over 64 slots, one 128-byte line each, so the
-  // wavefronts' atomics do not queue on one address; k_advance folds them into
-  // the totals above and clears them ([0] iterations, [1] cells, [2] capped,
-  // [3] fallbacks, [4] max iterations, [5] full and [6] shortcut node
-  // corrections, [7] FP32-phase iterations; [8] k_dt's partial maximum,
-  // folded by k_dt_finalize)
+  // wavefronts' atomics do not queue on one address; the corrector's last
+  // workgroup folds them into the totals above and clears them ([0]
+  // iterations, [1] cells, [2] capped, [3] fallbacks, [4] max iterations, [5]
+  // full and [6] shortcut node corrections, [7] FP32-phase iterations; [8]
+  // k_dt's partial maximum, folded by k_dt's last workgroup)
   unsigned long long pstat[64][16];
 };
 
@@ -52,6 +56,30 @@ DEV double wave_max(double x) {
   for (int o = 32; o >= 1; o >>= 1) x = dmax(x, __shfl_xor(x, o, 64));
   return x;
 }
+// True in every thread of the workgroup that is the last of the grid to
+// reach this point; that workgroup also resets the ticket.  What the other
+// workgroups hand over must be written with device-scope atomics (performed
+// at the device's coherence point) and read back with atomics: each thread
+// waits for its own atomics to be acknowledged before the workgroup draws its
+// ticket, so the last ticket is drawn after every other workgroup's atomics
+// have completed.  No device-scope fence is used: on gfx950 every such fence
+// writes back or invalidates the XCD's L2, which costs more than a separate
+// one-workgroup launch.  Every thread of every workgroup must call it.
+DEV bool last_workgroup(unsigned int* ticket) {
+  __shared__ unsigned int is_last;
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned int n = gridDim.x * gridDim.y * gridDim.z;
+    const unsigned int prev = atomicAdd(ticket, 1u);
+    is_last = (prev == n - 1) ? 1u : 0u;
+    if (is_last) atomicExch(ticket, 0u);
+  }
+  __syncthreads();
+  return is_last != 0u;
+}
+DEV void dt_finalize(StepState* st);
+
 DEV unsigned long long wave_sum_u64(unsigned long long x) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
@@ -242,20 +270,23 @@ k_dt(const double* __restrict__ wh, StepState* st, Geom g) {
   double m = wave_max(sum);
   if ((threadIdx.x & 63) == 0 && m > 0.0)
     atomicMax(&st->pstat[blockIdx.x & 63][8], (unsigned long long)__double_as_longlong(m));
+  // the last workgroup to finish takes dt from the 64 slots
+  if (last_workgroup(&st->dt_ticket)) dt_finalize(st);
 }
 
-// dt = CFL / MAX, x0.2 while count <= 5, clamped to tf - t (stepper.cpp:67-75).
-extern "C" __global__ void k_dt_finalize(StepState* st) {
-  // max over the 64 slots k_dt spreads its atomics over (IEEE bits of values
-  // >= 0 order as integers); each slot is cleared for the next step
-  unsigned long long b = st->pstat[threadIdx.x & 63][8];
-  st->pstat[threadIdx.x & 63][8] = 0ull;
+// dt = CFL / MAX, x0.2 while count <= 5, clamped to tf - t (stepper.cpp:67-75),
+// MAX the maximum over the 64 slots k_dt spreads its atomics over (IEEE bits
+// of values >= 0 order as integers); each slot is read and cleared with one
+// atomic exchange.  Run by the first wavefront of the last workgroup.
+DEV void dt_finalize(StepState* st) {
+  if (threadIdx.x >= 64) return;
+  unsigned long long b = atomicExch(&st->pstat[threadIdx.x][8], 0ull);
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) {
     unsigned long long y = __shfl_xor(b, o, 64);
     b = b > y ? b : y;
   }
-  if (threadIdx.x != 0 || st->done) return;
+  if (threadIdx.x != 0) return;
   double MAX = __longlong_as_double((long long)b);
   double dt = st->cfl / MAX;
   if (st->count <= 5) dt = dt * 0.2;

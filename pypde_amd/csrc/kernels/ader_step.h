@@ -496,7 +496,7 @@ DEV bool newton_node(const NodeCtx& c, const double (&q)[N][V], double (&G)[N][V
 #endif
 
 // Persistent, work-queued: each wavefront claims cells from st->qhead (reset
-// by k_dt_finalize every step), iterates its CPW slots, and as soon as a slot's
+// by k_dt's last workgroup every step), iterates its CPW slots, and as soon as a slot's
 // cell has converged writes that cell's outputs and refills the slot with the
 // next unclaimed cell, so cells with many Newton iterations do not hold the
 // lanes of cells that need few.  Every cell's arithmetic is independent of
@@ -1507,10 +1507,10 @@ struct FaceBufs {
   const double* fb[2];
 };
 
-extern "C" __global__ void __launch_bounds__(256)
-k_update(double* __restrict__ u, const double* __restrict__ vol, FaceBufs fbufs,
-         double* __restrict__ ret, StepState* st, Geom g) {
-  if (st->done) return;
+DEV void advance_step(StepState* st, int ndt);
+
+DEV void update_cell(double* __restrict__ u, const double* __restrict__ vol, const FaceBufs& fbufs,
+                     double* __restrict__ ret, StepState* st, const Geom& g) {
   const long c = (long)blockIdx.x * 256 + threadIdx.x;
   if (c >= g.ncell) return;
   const double dt = st->dt;
@@ -1589,6 +1589,16 @@ k_update(double* __restrict__ u, const double* __restrict__ vol, FaceBufs fbufs,
   }
 }
 
+// Every corrector kernel ends the step: its last workgroup runs advance_step
+// once all the others have written u and raised the NaN flag.
+extern "C" __global__ void __launch_bounds__(256)
+k_update(double* __restrict__ u, const double* __restrict__ vol, FaceBufs fbufs,
+         double* __restrict__ ret, StepState* st, Geom g) {
+  if (st->done) return;
+  update_cell(u, vol, fbufs, ret, st, g);
+  if (last_workgroup(&st->step_ticket)) advance_step(st, g.ndt);
+}
+
 // The corrector with one variable per thread and R consecutive x-rows per
 // thread (thread = (v, row group, y)): each x-face flux row between two of
 // its cells is read once and kept in registers for the next row, so the face
@@ -1598,7 +1608,6 @@ k_update(double* __restrict__ u, const double* __restrict__ vol, FaceBufs fbufs,
 template <int R>
 DEV void update_rows(double* __restrict__ u, const double* __restrict__ vol, const FaceBufs& fbufs,
                      double* __restrict__ ret, StepState* st, const Geom& g) {
-  if (st->done) return;
   const int nx = g.nx[0], ny = g.nx[1];
   const int nxg = (nx + R - 1) / R;
   const long idx = (long)blockIdx.x * 256 + threadIdx.x;
@@ -1710,7 +1719,9 @@ DEV void update_rows(double* __restrict__ u, const double* __restrict__ vol, con
   extern "C" __global__ void __launch_bounds__(256)                                           \
   k_update_r##R_(double* __restrict__ u, const double* __restrict__ vol, FaceBufs fbufs,      \
                  double* __restrict__ ret, StepState* st, Geom g) {                            \
+    if (st->done) return;                                                                      \
     update_rows<R_>(u, vol, fbufs, ret, st, g);                                                \
+    if (last_workgroup(&st->step_ticket)) advance_step(st, g.ndt);                             \
   }
 PYPDE_UPDATE_ROWS(1)
 PYPDE_UPDATE_ROWS(2)
@@ -1718,11 +1729,13 @@ PYPDE_UPDATE_ROWS(4)
 PYPDE_UPDATE_ROWS(8)
 
 // t += dt; count++; snapshot row; NaN and end-of-run checks (iterator.cpp:129-147).
-extern "C" __global__ void k_advance(StepState* st, int ndt) {
+// Run by the first wavefront of the corrector's last workgroup, which also
+// folds the predictor's spread statistics into the totals (lane i takes slot
+// i, then sums and a max across the wavefront) and clears the slots.
+DEV void advance_step(StepState* st, int ndt) {
+  if (threadIdx.x >= 64) return;
   {
-    // fold the predictor's spread statistics into the totals (one wavefront:
-    // lane i takes slot i, then sums and a max across the wavefront)
-    unsigned long long* ps = st->pstat[threadIdx.x & 63];
+    unsigned long long* ps = st->pstat[threadIdx.x];
     unsigned long long a0 = wave_sum_u64(ps[0]), a1 = wave_sum_u64(ps[1]);
     unsigned long long a2 = wave_sum_u64(ps[2]), a3 = wave_sum_u64(ps[3]);
     unsigned long long a5 = wave_sum_u64(ps[5]), a6 = wave_sum_u64(ps[6]);
@@ -1746,11 +1759,13 @@ extern "C" __global__ void k_advance(StepState* st, int ndt) {
       if ((unsigned int)m4 > st->pred_maxit) st->pred_maxit = (unsigned int)m4;
     }
   }
-  if (threadIdx.x != 0 || st->done) return;
+  if (threadIdx.x != 0) return;
   st->t = st->t + st->dt;
   st->count += 1;
   if (st->push < ndt && st->t >= (double)(st->push + 1) / (double)ndt * st->tf) st->push += 1;
-  if (st->nan_flag) {
+  // raised by the corrector's other workgroups during this launch: read at
+  // the device's coherence point
+  if (atomicAdd(&st->nan_flag, 0u)) {
     st->done = 1;
     st->nan_step = (int)st->count;
   }

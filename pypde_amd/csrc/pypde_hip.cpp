@@ -81,11 +81,13 @@ struct StepState {
   unsigned long long pred_nfull, pred_nshort;
   unsigned long long pred_iters32;
   unsigned long long hot_count;
+  unsigned int dt_ticket, step_ticket;
   // predictor statistics spread over 64 slots, one 128-byte line each, so the
-  // wavefronts' atomics do not queue on one address; k_advance folds them into
-  // the totals above and clears them ([0] iterations, [1] cells, [2] capped,
-  // [3] fallbacks, [4] max iterations, [5] full and [6] shortcut node
-  // corrections; [8] k_dt's partial maximum, folded by k_dt_finalize)
+  // wavefronts' atomics do not queue on one address; the corrector's last
+  // workgroup folds them into the totals above and clears them ([0]
+  // iterations, [1] cells, [2] capped, [3] fallbacks, [4] max iterations, [5]
+  // full and [6] shortcut node corrections, [7] FP32-phase iterations; [8]
+  // k_dt's partial maximum, folded by k_dt's last workgroup)
   unsigned long long pstat[64][16];
 };
 
@@ -269,12 +271,12 @@ int check_spec(const Spec& s, bool secondOrder, int FLUX) {
   return 0;
 }
 
-enum { K_WENO, K_DT, K_DTFIN, K_PRED, K_FLUX0, K_FLUX1, K_UPDATE, K_ADVANCE, K_A2S, K_S2A,
+enum { K_WENO, K_DT, K_PRED, K_FLUX0, K_FLUX1, K_UPDATE, K_A2S, K_S2A,
        K_HARVEST, K_MLPGUESS, K_PRED_STATIC, K_PRED_STATIC2, K_EIG, K_P2R, K_PRED32,
        K_WENO_4x64, K_WENO_16x16, K_WENO_16x32, K_FLUX0_128, K_FLUX0_64, K_FLUX1_128, K_FLUX1_64,
        K_UPD1, K_UPD2, K_UPD4, K_UPD8, K_PRED_COLD, K_PRED_HOT, K_MLPPACK, K_PRED_COLD3, K_COUNT };
-const char* kernel_names[K_COUNT] = {"k_weno",   "k_dt",    "k_dt_finalize", "k_predictor",
-                                     "k_flux0",  "k_flux1", "k_update",      "k_advance",
+const char* kernel_names[K_COUNT] = {"k_weno",   "k_dt",    "k_predictor",
+                                     "k_flux0",  "k_flux1", "k_update",
                                      "k_aos_to_soa", "k_soa_to_aos", "k_harvest", "k_mlp_guess",
                                      "k_predictor_static", "k_predictor_static2", "k_eig_batch",
                                      "k_planes_to_rows", "k_predictor_f32",
@@ -477,9 +479,8 @@ int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
     const double* w = c->wh;
     StepState* st = c->st;
     std::vector<void*> a = {&w, &st, &g};
+    // its last workgroup also turns the wave-speed maximum into dt
     if ((rc = launch(c, K_DT, dim3((unsigned)cdiv(g.nwc, 256)), dim3(256), a))) return rc;
-    std::vector<void*> b = {&st};
-    if ((rc = launch(c, K_DTFIN, dim3(1), dim3(64), b))) return rc;
   }
   mark();
   if (c->use_model) {
@@ -557,12 +558,9 @@ int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
       return rc;
   }
   mark();
-  {
-    StepState* st = c->st;
-    int ndt = g.ndt;
-    std::vector<void*> a = {&st, &ndt};
-    if ((rc = launch(c, K_ADVANCE, dim3(1), dim3(64), a))) return rc;
-  }
+  // the corrector's last workgroup advances t, count and the snapshot row
+  // and folds the step's statistics: the advance stage keeps its marks and
+  // times nothing
   mark();
   if (mark_err != hipSuccess)
     return fail(PYPDE_ERR_HIP, "profiling event: %s", hipGetErrorString(mark_err));
