@@ -176,23 +176,38 @@ def gpu_run(steps, warmup, profile, net=None, mixed=False):
 def ensemble_leg(steps):
     """48 independent 2D Euler grids, sides drawn from {64,128,256,512},
     seeded four-quadrant Riemann data, order 3, transmissive walls, each grid
-    with its own dt; every timestep of all grids is one HIP graph launch.
-    Timed against the same grids run one after another."""
+    with its own dt; every timestep of all grids is one HIP graph launch:
+    batched in groups of at most 2^16 cells (the value), batched in one launch
+    per stage over all grids, or one stream per grid inside the graph.  All
+    are timed against the same grids run one after another."""
     from pypde_amd.solvers import Ensemble
     shapes = systems.ensemble_sizes(48)
     grids = [systems.riemann4(nx, ny, seed=i) for i, (nx, ny) in enumerate(shapes)]
     Ls = [[1.0, 1.0]] * len(grids)
     src = systems.source("euler2d")
-    ens = Ensemble(grids, Ls, src, order=3)
-    ens.run(1.0, max_steps=2)                      # warm-up (code objects, graph)
-    ens.close()
-    ens = Ensemble(grids, Ls, src, order=3)
-    t0 = time.perf_counter()
-    stats, _ = ens.run(1.0, max_steps=steps)
-    t1 = time.perf_counter()
-    ens.close()
     cells = sum(nx * ny for nx, ny in shapes)
-    done = sum(st["steps"] * nx * ny for st, (nx, ny) in zip(stats, shapes))
+    rates = {}
+    # groups of at most 2^16 cells (the default), one launch per stage over
+    # all grids, one stream per grid
+    for mode, env, cap in (("batched", "batched", None), ("one_launch", "batched", "0"),
+                           ("streams", "streams", None)):
+        os.environ["PYPDE_ENSEMBLE"] = env
+        if cap is None:
+            os.environ.pop("PYPDE_ENSEMBLE_CELLS", None)
+        else:
+            os.environ["PYPDE_ENSEMBLE_CELLS"] = cap
+        ens = Ensemble(grids, Ls, src, order=3)
+        ens.run(1.0, max_steps=2)                      # warm-up (code objects, graph)
+        ens.close()
+        ens = Ensemble(grids, Ls, src, order=3)
+        t0 = time.perf_counter()
+        stats, _ = ens.run(1.0, max_steps=steps)
+        t1 = time.perf_counter()
+        ens.close()
+        done = sum(st["steps"] * nx * ny for st, (nx, ny) in zip(stats, shapes))
+        rates[mode] = (done / (t1 - t0), (t1 - t0) * 1e3 / steps)
+    os.environ.pop("PYPDE_ENSEMBLE_CELLS", None)
+    os.environ.pop("PYPDE_ENSEMBLE", None)
     seq = 0.0
     for g, L_ in zip(grids, Ls):
         sol = Solver(g, L_, src, order=3)
@@ -200,9 +215,31 @@ def ensemble_leg(steps):
         sol.run(1.0, max_steps=steps)
         seq += time.perf_counter() - a
         sol.close()
-    return dict(grids=len(grids), cells=cells, steps=steps, value=done / (t1 - t0),
-                unit="cell-updates/s", ms_per_step=(t1 - t0) * 1e3 / steps,
-                sequential_value=done / seq, graph_speedup=seq / (t1 - t0))
+    seq_value = steps * cells / seq
+    # many small grids: 48 of 64 x 64 cells, grouped against one stream each
+    small = [systems.riemann4(64, 64, seed=i) for i in range(48)]
+    small_rates = {}
+    for env in ("batched", "streams"):
+        os.environ["PYPDE_ENSEMBLE"] = env
+        ens = Ensemble(small, Ls, src, order=3)
+        ens.run(1.0, max_steps=2)
+        ens.close()
+        ens = Ensemble(small, Ls, src, order=3)
+        t0 = time.perf_counter()
+        stats, _ = ens.run(1.0, max_steps=steps)
+        t1 = time.perf_counter()
+        ens.close()
+        small_rates[env] = sum(st["steps"] for st in stats) * 64 * 64 / (t1 - t0)
+    os.environ.pop("PYPDE_ENSEMBLE", None)
+    return dict(grids=len(grids), cells=cells, steps=steps, value=rates["batched"][0],
+                unit="cell-updates/s", ms_per_step=rates["batched"][1],
+                one_launch_value=rates["one_launch"][0], streams_value=rates["streams"][0],
+                sequential_value=seq_value,
+                batched_vs_streams=rates["batched"][0] / rates["streams"][0],
+                batched_vs_sequential=rates["batched"][0] / seq_value,
+                small_48x64=dict(batched_value=small_rates["batched"],
+                                 streams_value=small_rates["streams"],
+                                 batched_vs_streams=small_rates["batched"] / small_rates["streams"]))
 
 
 def euler_modes_leg(steps, warmup=3):

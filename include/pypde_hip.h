@@ -222,6 +222,11 @@ int pypde_device_count(void);
  * secondOrder as in pde_solver_hip. */
 int pypde_hip_compile(const char *src, bool useF, bool useB, bool useS, int ndim, int N, int V,
                       bool STIFF, int FLUX, bool secondOrder, const pypde_options *opts);
+/* The same for the code object of ensemble members, which also holds the batched
+ * ensemble kernels (one launch per stage over every member). */
+int pypde_hip_compile_ensemble(const char *src, bool useF, bool useB, bool useS, int ndim, int N,
+                               int V, bool STIFF, int FLUX, bool secondOrder,
+                               const pypde_options *opts);
 /* Tables exactly as the kernels see them (for checks). */
 int pypde_hip_tables(int N, double *nodes, double *wghts, double *endvals, double *dervals,
                      double *weno_inv, double *sig, double *dgmat, double *dginv, double *fl_c,

@@ -82,6 +82,7 @@ def lib():
         L.pypde_hip_compile.argtypes = [C.c_char_p, C.c_bool, C.c_bool, C.c_bool, C.c_int,
                                         C.c_int, C.c_int, C.c_bool, C.c_int, C.c_bool,
                                         C.POINTER(Options)]
+        L.pypde_hip_compile_ensemble.argtypes = L.pypde_hip_compile.argtypes
         L.pypde_hip_tables.argtypes = [C.c_int] + [PD] * 10
         L.pypde_ensemble_create.restype = C.c_void_p
         L.pypde_ensemble_create.argtypes = [C.c_int, C.c_char_p, C.c_bool, C.c_bool, C.c_bool, PI,
@@ -148,10 +149,12 @@ def make_options(newton_tol=1e-12, newton_maxit=100, max_steps=0, reflect=None, 
 
 
 def compile_only(src, useF, useB, useS, ndim, N, V, stiff=False, use_M=False, b_quirk=0, flux=0,
-                 second=False, mixed=False):
+                 second=False, mixed=False, ensemble=False):
+    """Compile (or fetch from the code-object cache) without a GPU; ensemble =
+    True gives the code object of ensemble members (with the batched kernels)."""
     o = make_options(use_M=use_M, b_quirk=b_quirk, mixed=mixed)
-    rc = lib().pypde_hip_compile(src.encode(), useF, useB, useS, ndim, N, V, stiff, int(flux),
-                                 bool(second), C.byref(o))
+    fn = lib().pypde_hip_compile_ensemble if ensemble else lib().pypde_hip_compile
+    rc = fn(src.encode(), useF, useB, useS, ndim, N, V, stiff, int(flux), bool(second), C.byref(o))
     check(rc, f"hipRTC compile (ndim={ndim}, N={N}, V={V})")
 
 

@@ -51,6 +51,16 @@ struct StepState {
   unsigned long long pstat[64][16];
 };
 
+// A workgroup's coordinates within one grid's launch, and the number of
+// workgroups of that launch: the hardware's own for a single grid, or one
+// member's share of a batched ensemble launch.
+struct Blk {
+  unsigned int x, y, z, n;
+};
+DEV Blk hw_blk() {
+  return {blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x * gridDim.y * gridDim.z};
+}
+
 DEV double wave_max(double x) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) x = dmax(x, __shfl_xor(x, o, 64));
@@ -64,13 +74,13 @@ DEV double wave_max(double x) {
 // ticket, so the last ticket is drawn after every other workgroup's atomics
 // have completed.  No device-scope fence is used: on gfx950 every such fence
 // writes back or invalidates the XCD's L2, which costs more than a separate
-// one-workgroup launch.  Every thread of every workgroup must call it.
-DEV bool last_workgroup(unsigned int* ticket) {
+// one-workgroup launch.  n is the number of workgroups drawing from the
+// ticket; every thread of each of them must call it.
+DEV bool last_workgroup(unsigned int* ticket, unsigned int n) {
   __shared__ unsigned int is_last;
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
-    const unsigned int n = gridDim.x * gridDim.y * gridDim.z;
     const unsigned int prev = atomicAdd(ticket, 1u);
     is_last = (prev == n - 1) ? 1u : 0u;
     if (is_last) atomicExch(ticket, 0u);
@@ -98,20 +108,20 @@ DEV unsigned int wave_max_u32(unsigned int x) {
 // Boundary ghosts fused into the load, both sweeps staged in LDS
 // (boundaries.cpp:25-53, weno.cpp:61-121).  ghost = false gives the
 // reference's weno_solver semantics (no ghost extension).
-// 2D: one variable per workgroup (blockIdx.z), a TX x TY tile of output cells
+// 2D: one variable per workgroup (coordinate z), a TX x TY tile of output cells
 // per workgroup (256 threads); sweep 1 along axis 0 over the TX + 2N - 2 rows
 // and TY + 2N - 2 columns the tile needs, then sweep 2 along axis 1 for every
 // coefficient index k0.  The autotuner picks the tile per problem.
 template <int TX, int TY>
 DEV void weno_2d(const double* __restrict__ u, double* __restrict__ wh, const Geom& g, int ghost,
-                 int nin0, int nin1, long out_stride) {
+                 int nin0, int nin1, long out_stride, const Blk& b) {
   constexpr int H = 2 * N - 2;
   const int tid = threadIdx.x;
   constexpr int RX = TX + H, RY = TY + H, RYP = RY + 1;
   __shared__ double s0[RX][RYP];
   __shared__ double s1[N][TX][RYP];
-  const int v = blockIdx.z;
-  const int i0 = blockIdx.y * TX, j0 = blockIdx.x * TY;
+  const int v = b.z;
+  const int i0 = b.y * TX, j0 = b.x * TY;
   const int nb0 = ghost ? g.nb[0] : nin0, nb1 = ghost ? g.nb[1] : nin1;
   const int no0 = nb0 - H, no1 = nb1 - H;
   const long in_stride = ghost ? g.ncell : (long)nin0 * nin1;
@@ -157,13 +167,13 @@ DEV void weno_2d(const double* __restrict__ u, double* __restrict__ wh, const Ge
 
 // 1D: 256 output cells per workgroup, all variables.
 DEV void weno_1d(const double* __restrict__ u, double* __restrict__ wh, const Geom& g, int ghost,
-                 int nin0, long out_stride) {
+                 int nin0, long out_stride, const Blk& b) {
   constexpr int H = 2 * N - 2;
   const int tid = threadIdx.x;
   constexpr int TX = 256;
   constexpr int RX = TX + H;
   __shared__ double s0[V][RX];
-  const int i0 = blockIdx.x * TX;
+  const int i0 = b.x * TX;
   const int nb0 = ghost ? g.nb[0] : nin0;
   const int no0 = nb0 - H;
   for (int e = tid; e < V * RX; e += 256) {
@@ -198,9 +208,9 @@ DEV void weno_1d(const double* __restrict__ u, double* __restrict__ wh, const Ge
        Geom g, int ghost, int nin0, int nin1, long out_stride) {                               \
     if (st && st->done) return;                                                                \
     if constexpr (D == 1)                                                                      \
-      weno_1d(u, wh, g, ghost, nin0, out_stride);                                              \
+      weno_1d(u, wh, g, ghost, nin0, out_stride, hw_blk());                                    \
     else                                                                                       \
-      weno_2d<TX, TY>(u, wh, g, ghost, nin0, nin1, out_stride);                                \
+      weno_2d<TX, TY>(u, wh, g, ghost, nin0, nin1, out_stride, hw_blk());                      \
   }
 PYPDE_WENO_KERNEL(k_weno, 8, 32)
 PYPDE_WENO_KERNEL(k_weno_4x64, 4, 64)
@@ -225,10 +235,8 @@ DEV void cell_deriv(const double (&w)[ND][V], int idx, double rdx, double* out) 
   }
 }
 
-extern "C" __global__ void __launch_bounds__(256)
-k_dt(const double* __restrict__ wh, StepState* st, Geom g) {
-  if (st->done) return;
-  const long c = (long)blockIdx.x * 256 + threadIdx.x;
+DEV void dt_body(const double* __restrict__ wh, StepState* st, const Geom& g, const Blk& b) {
+  const long c = (long)b.x * 256 + threadIdx.x;
   double sum = 0.0;
   if (c < g.nwc) {
     double w[ND][V];
@@ -269,9 +277,15 @@ k_dt(const double* __restrict__ wh, StepState* st, Geom g) {
   }
   double m = wave_max(sum);
   if ((threadIdx.x & 63) == 0 && m > 0.0)
-    atomicMax(&st->pstat[blockIdx.x & 63][8], (unsigned long long)__double_as_longlong(m));
+    atomicMax(&st->pstat[b.x & 63][8], (unsigned long long)__double_as_longlong(m));
   // the last workgroup to finish takes dt from the 64 slots
-  if (last_workgroup(&st->dt_ticket)) dt_finalize(st);
+  if (last_workgroup(&st->dt_ticket, b.n)) dt_finalize(st);
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+k_dt(const double* __restrict__ wh, StepState* st, Geom g) {
+  if (st->done) return;
+  dt_body(wh, st, g, hw_blk());
 }
 
 // dt = CFL / MAX, x0.2 while count <= 5, clamped to tf - t (stepper.cpp:67-75),

@@ -793,9 +793,10 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
                                const double* __restrict__ guess, double* __restrict__ tr,
                                double* __restrict__ vol, double* __restrict__ qh, StepState* st,
                                const Geom& g, double newton_tol, int newton_maxit,
-                               double* __restrict__ trd, unsigned int* __restrict__ hot = nullptr) {
+                               double* __restrict__ trd, const Blk& b,
+                               unsigned int* __restrict__ hot = nullptr) {
   if (st->done) return;
-  if (MODE == 2 && (unsigned long long)blockIdx.x * CPW >= st->hot_count) return;
+  if (MODE == 2 && (unsigned long long)b.x * CPW >= st->hot_count) return;
   const double dt = st->dt;
   const int lane = threadIdx.x;
   NodeCtx c;
@@ -803,11 +804,11 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
   long cell;
   bool valid;
   if constexpr (MODE == 2) {
-    const unsigned long long i = (unsigned long long)blockIdx.x * CPW + c.slot;
+    const unsigned long long i = (unsigned long long)b.x * CPW + c.slot;
     valid = (c.slot < CPW) && (i < st->hot_count);
     cell = valid ? (long)hot[i] : 0;
   } else {
-    cell = (long)blockIdx.x * CPW + c.slot;
+    cell = (long)b.x * CPW + c.slot;
     valid = (c.slot < CPW) && (cell < g.nwc);
   }
   double w[V], q[N][V], rhs[N][V], dqs[N][D][V];
@@ -1060,7 +1061,7 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
   mx = wave_max_u32(mx);
   const unsigned long long w_full = wave_sum_u64(n_full), w_short = wave_sum_u64(n_short);
   if (lane == 0) {
-    unsigned long long* ps = st->pstat[blockIdx.x & 63];
+    unsigned long long* ps = st->pstat[b.x & 63];
     atomicAdd(&ps[0], mine);
     atomicAdd(&ps[1], cnt);
     if (cap) atomicAdd(&ps[2], cap);
@@ -1077,7 +1078,8 @@ k_predictor_static(const double* __restrict__ wh, const double* __restrict__ gue
                    StepState* st, Geom g, double newton_tol, int newton_maxit,
                    double* __restrict__ trd) {
   __shared__ PredLDS sm;
-  predictor_static_body(sm, wh, guess, tr, vol, qh, st, g, newton_tol, newton_maxit, trd);
+  predictor_static_body(sm, wh, guess, tr, vol, qh, st, g, newton_tol, newton_maxit, trd,
+                        hw_blk());
 }
 
 #if PDE_STIFF && !PDE_GUESS
@@ -1090,7 +1092,7 @@ k_predictor_cold(const double* __restrict__ wh, double* __restrict__ tr, double*
                  int newton_maxit, unsigned int* __restrict__ hot) {
   __shared__ PredLDS sm;
   predictor_static_body<1>(sm, wh, nullptr, tr, vol, qh, st, g, newton_tol, newton_maxit,
-                           nullptr, hot);
+                           nullptr, hw_blk(), hot);
 }
 // The cold pass with the register budget of three wavefronts per SIMD (168
 // registers; whatever does not fit spills), for the autotuner to weigh
@@ -1101,7 +1103,7 @@ k_predictor_cold3(const double* __restrict__ wh, double* __restrict__ tr, double
                   int newton_maxit, unsigned int* __restrict__ hot) {
   __shared__ PredLDS sm;
   predictor_static_body<1>(sm, wh, nullptr, tr, vol, qh, st, g, newton_tol, newton_maxit,
-                           nullptr, hot);
+                           nullptr, hw_blk(), hot);
 }
 extern "C" __global__ void __launch_bounds__(64)
 k_predictor_hot(const double* __restrict__ wh, double* __restrict__ tr, double* __restrict__ vol,
@@ -1109,7 +1111,7 @@ k_predictor_hot(const double* __restrict__ wh, double* __restrict__ tr, double* 
                 int newton_maxit, unsigned int* __restrict__ hot) {
   __shared__ PredLDS sm;
   predictor_static_body<2>(sm, wh, nullptr, tr, vol, qh, st, g, newton_tol, newton_maxit,
-                           nullptr, hot);
+                           nullptr, hw_blk(), hot);
 }
 #endif
 
@@ -1121,7 +1123,8 @@ k_predictor_static2(const double* __restrict__ wh, const double* __restrict__ gu
                     StepState* st, Geom g, double newton_tol, int newton_maxit,
                    double* __restrict__ trd) {
   __shared__ PredLDS sm;
-  predictor_static_body(sm, wh, guess, tr, vol, qh, st, g, newton_tol, newton_maxit, trd);
+  predictor_static_body(sm, wh, guess, tr, vol, qh, st, g, newton_tol, newton_maxit, trd,
+                        hw_blk());
 }
 
 #if PDE_MIXED
@@ -1369,9 +1372,10 @@ DEV void face_cells(const Geom& g, long f, long* wl, long* wr) {
 // (fluxes.cpp:103-116), from the two cells' face traces.
 template <int d, int BS = 256>
 DEV void flux_dir(const double* __restrict__ tr, double* __restrict__ ff, double* __restrict__ fb,
-                  const StepState* st, const Geom& g, const double* __restrict__ trd) {
+                  const StepState* st, const Geom& g, const double* __restrict__ trd,
+                  const Blk& b) {
   const long nf = g.nf[d];
-  const long item = (long)blockIdx.x * BS + threadIdx.x;
+  const long item = (long)b.x * BS + threadIdx.x;
   if (item >= nf * N * ND1) return;
   const long f = item % nf;
   const int j = (int)((item / nf) % ND1);
@@ -1485,7 +1489,7 @@ DEV void flux_dir(const double* __restrict__ tr, double* __restrict__ ff, double
   NAME(const double* __restrict__ tr, double* __restrict__ ff, double* __restrict__ fb,        \
        const StepState* st, Geom g, const double* __restrict__ trd) {                          \
     if (st->done) return;                                                                      \
-    flux_dir<DIR, BS>(tr, ff, fb, st, g, trd);                                                 \
+    flux_dir<DIR, BS>(tr, ff, fb, st, g, trd, hw_blk());                                       \
   }
 PYPDE_FLUX_KERNEL(k_flux0, 0, 256)
 PYPDE_FLUX_KERNEL(k_flux0_128, 0, 128)
@@ -1510,8 +1514,8 @@ struct FaceBufs {
 DEV void advance_step(StepState* st, int ndt);
 
 DEV void update_cell(double* __restrict__ u, const double* __restrict__ vol, const FaceBufs& fbufs,
-                     double* __restrict__ ret, StepState* st, const Geom& g) {
-  const long c = (long)blockIdx.x * 256 + threadIdx.x;
+                     double* __restrict__ ret, StepState* st, const Geom& g, const Blk& b) {
+  const long c = (long)b.x * 256 + threadIdx.x;
   if (c >= g.ncell) return;
   const double dt = st->dt;
   int cc[2];
@@ -1595,8 +1599,9 @@ extern "C" __global__ void __launch_bounds__(256)
 k_update(double* __restrict__ u, const double* __restrict__ vol, FaceBufs fbufs,
          double* __restrict__ ret, StepState* st, Geom g) {
   if (st->done) return;
-  update_cell(u, vol, fbufs, ret, st, g);
-  if (last_workgroup(&st->step_ticket)) advance_step(st, g.ndt);
+  const Blk b = hw_blk();
+  update_cell(u, vol, fbufs, ret, st, g, b);
+  if (last_workgroup(&st->step_ticket, b.n)) advance_step(st, g.ndt);
 }
 
 // The corrector with one variable per thread and R consecutive x-rows per
@@ -1607,10 +1612,10 @@ k_update(double* __restrict__ u, const double* __restrict__ vol, FaceBufs fbufs,
 // results are identical; the autotuner picks R per problem.
 template <int R>
 DEV void update_rows(double* __restrict__ u, const double* __restrict__ vol, const FaceBufs& fbufs,
-                     double* __restrict__ ret, StepState* st, const Geom& g) {
+                     double* __restrict__ ret, StepState* st, const Geom& g, const Blk& b) {
   const int nx = g.nx[0], ny = g.nx[1];
   const int nxg = (nx + R - 1) / R;
-  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  const long idx = (long)b.x * 256 + threadIdx.x;
   if (idx >= (long)V * nxg * ny) return;
   const int cy = (int)(idx % ny);
   const long rest = idx / ny;
@@ -1720,8 +1725,9 @@ DEV void update_rows(double* __restrict__ u, const double* __restrict__ vol, con
   k_update_r##R_(double* __restrict__ u, const double* __restrict__ vol, FaceBufs fbufs,      \
                  double* __restrict__ ret, StepState* st, Geom g) {                            \
     if (st->done) return;                                                                      \
-    update_rows<R_>(u, vol, fbufs, ret, st, g);                                                \
-    if (last_workgroup(&st->step_ticket)) advance_step(st, g.ndt);                             \
+    const Blk b = hw_blk();                                                                    \
+    update_rows<R_>(u, vol, fbufs, ret, st, g, b);                                             \
+    if (last_workgroup(&st->step_ticket, b.n)) advance_step(st, g.ndt);                        \
   }
 PYPDE_UPDATE_ROWS(1)
 PYPDE_UPDATE_ROWS(2)
@@ -1991,4 +1997,154 @@ k_mlp_guess(const double* __restrict__ wh, const StepState* st, const bf16x4* __
     }
   }
 }
+#endif
+
+#ifndef PDE_BATCH
+#define PDE_BATCH 0
+#endif
+#if PDE_BATCH
+// ------------------------------------------------------- batched ensembles
+// Every member of an ensemble (grids of one system, order and variable
+// count, each with its own size, spacing, walls and state) advances in one
+// launch per stage.  Member m owns the workgroups [map[m].start,
+// map[m + 1].start) of a launch, laid out as its own single-grid launch of
+// gx x gy x gz workgroups, and runs the single-grid kernel body on its own
+// buffers, so each member computes exactly what it computes alone.  Both
+// tables are read through the constant address space (nothing writes them
+// during a launch), so their uniform loads go to scalar registers.
+struct Member {
+  Geom g;
+  double *u, *wh, *tr, *vol, *trd;
+  double *ff[2], *fb[2];
+  StepState* st;
+  unsigned int* hot;
+};
+struct BlkMap {
+  unsigned int start, gx, gy, gz;
+};
+#define PDE_CONST_AS __attribute__((address_space(4)))
+
+// The member owning workgroup blk: map[nm].start is the launch's total.
+DEV int batch_member(const PDE_CONST_AS BlkMap* map, int nm, unsigned int blk) {
+  int lo = 0, hi = nm;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (map[mid].start <= blk) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Resolves the launch's workgroup to (member, its coordinates in the
+// member's own launch); declares M (the member) and b (the coordinates).
+#define PYPDE_BATCH_MEMBER                                                                     \
+  const PDE_CONST_AS BlkMap* cmap = (const PDE_CONST_AS BlkMap*)map;                           \
+  const int m_ = batch_member(cmap, nm, blockIdx.x);                                           \
+  const unsigned int loc_ = blockIdx.x - cmap[m_].start;                                       \
+  const unsigned int gx_ = cmap[m_].gx, gy_ = cmap[m_].gy;                                     \
+  const Blk b = {loc_ % gx_, (loc_ / gx_) % gy_, loc_ / (gx_ * gy_), gx_ * gy_ * cmap[m_].gz}; \
+  const Member& M = *(const Member*)&((const PDE_CONST_AS Member*)ms)[m_];
+
+#define PYPDE_WENO_BATCH(NAME, TX, TY)                                                         \
+  extern "C" __global__ void __launch_bounds__(256)                                            \
+  NAME(const Member* ms, const BlkMap* map, int nm) {                                          \
+    PYPDE_BATCH_MEMBER                                                                         \
+    if (M.st->done) return;                                                                    \
+    if constexpr (D == 1)                                                                      \
+      weno_1d(M.u, M.wh, M.g, 1, 0, M.g.nwc, b);                                               \
+    else                                                                                       \
+      weno_2d<TX, TY>(M.u, M.wh, M.g, 1, 0, 0, M.g.nwc, b);                                    \
+  }
+PYPDE_WENO_BATCH(k_weno_b, 8, 32)
+PYPDE_WENO_BATCH(k_weno_4x64_b, 4, 64)
+PYPDE_WENO_BATCH(k_weno_16x16_b, 16, 16)
+PYPDE_WENO_BATCH(k_weno_16x32_b, 16, 32)
+
+extern "C" __global__ void __launch_bounds__(256)
+k_dt_b(const Member* ms, const BlkMap* map, int nm) {
+  PYPDE_BATCH_MEMBER
+  if (M.st->done) return;
+  dt_body(M.wh, M.st, M.g, b);
+}
+
+extern "C" __global__ void __launch_bounds__(64)
+k_predictor_static_b(const Member* ms, const BlkMap* map, int nm, double newton_tol,
+                     int newton_maxit) {
+  __shared__ PredLDS sm;
+  PYPDE_BATCH_MEMBER
+  predictor_static_body(sm, M.wh, nullptr, M.tr, M.vol, nullptr, M.st, M.g, newton_tol,
+                        newton_maxit, M.trd, b);
+}
+extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_predictor_static2_b(const Member* ms, const BlkMap* map, int nm, double newton_tol,
+                      int newton_maxit) {
+  __shared__ PredLDS sm;
+  PYPDE_BATCH_MEMBER
+  predictor_static_body(sm, M.wh, nullptr, M.tr, M.vol, nullptr, M.st, M.g, newton_tol,
+                        newton_maxit, M.trd, b);
+}
+#if PDE_STIFF && !PDE_GUESS
+extern "C" __global__ void __launch_bounds__(64)
+k_predictor_cold_b(const Member* ms, const BlkMap* map, int nm, double newton_tol,
+                   int newton_maxit) {
+  __shared__ PredLDS sm;
+  PYPDE_BATCH_MEMBER
+  predictor_static_body<1>(sm, M.wh, nullptr, M.tr, M.vol, nullptr, M.st, M.g, newton_tol,
+                           newton_maxit, nullptr, b, M.hot);
+}
+extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3)))
+k_predictor_cold3_b(const Member* ms, const BlkMap* map, int nm, double newton_tol,
+                    int newton_maxit) {
+  __shared__ PredLDS sm;
+  PYPDE_BATCH_MEMBER
+  predictor_static_body<1>(sm, M.wh, nullptr, M.tr, M.vol, nullptr, M.st, M.g, newton_tol,
+                           newton_maxit, nullptr, b, M.hot);
+}
+extern "C" __global__ void __launch_bounds__(64)
+k_predictor_hot_b(const Member* ms, const BlkMap* map, int nm, double newton_tol,
+                  int newton_maxit) {
+  __shared__ PredLDS sm;
+  PYPDE_BATCH_MEMBER
+  predictor_static_body<2>(sm, M.wh, nullptr, M.tr, M.vol, nullptr, M.st, M.g, newton_tol,
+                           newton_maxit, nullptr, b, M.hot);
+}
+#endif
+
+#define PYPDE_FLUX_BATCH(NAME, DIR, BS)                                                        \
+  extern "C" __global__ void __launch_bounds__(BS)                                             \
+  NAME(const Member* ms, const BlkMap* map, int nm) {                                          \
+    PYPDE_BATCH_MEMBER                                                                         \
+    if (M.st->done) return;                                                                    \
+    flux_dir<DIR, BS>(M.tr, M.ff[DIR], M.fb[DIR], M.st, M.g, M.trd, b);                        \
+  }
+PYPDE_FLUX_BATCH(k_flux0_b, 0, 256)
+PYPDE_FLUX_BATCH(k_flux0_128_b, 0, 128)
+PYPDE_FLUX_BATCH(k_flux0_64_b, 0, 64)
+#if PDE_D >= 2
+PYPDE_FLUX_BATCH(k_flux1_b, 1, 256)
+PYPDE_FLUX_BATCH(k_flux1_128_b, 1, 128)
+PYPDE_FLUX_BATCH(k_flux1_64_b, 1, 64)
+#endif
+
+extern "C" __global__ void __launch_bounds__(256)
+k_update_b(const Member* ms, const BlkMap* map, int nm) {
+  PYPDE_BATCH_MEMBER
+  if (M.st->done) return;
+  const FaceBufs fbufs = {{M.ff[0], M.ff[1]}, {M.fb[0], M.fb[1]}};
+  update_cell(M.u, M.vol, fbufs, nullptr, M.st, M.g, b);
+  if (last_workgroup(&M.st->step_ticket, b.n)) advance_step(M.st, M.g.ndt);
+}
+#define PYPDE_UPDATE_ROWS_BATCH(R_)                                                            \
+  extern "C" __global__ void __launch_bounds__(256)                                           \
+  k_update_r##R_##_b(const Member* ms, const BlkMap* map, int nm) {                            \
+    PYPDE_BATCH_MEMBER                                                                         \
+    if (M.st->done) return;                                                                    \
+    const FaceBufs fbufs = {{M.ff[0], M.ff[1]}, {M.fb[0], M.fb[1]}};                           \
+    update_rows<R_>(M.u, M.vol, fbufs, nullptr, M.st, M.g, b);                                 \
+    if (last_workgroup(&M.st->step_ticket, b.n)) advance_step(M.st, M.g.ndt);                  \
+  }
+PYPDE_UPDATE_ROWS_BATCH(1)
+PYPDE_UPDATE_ROWS_BATCH(2)
+PYPDE_UPDATE_ROWS_BATCH(4)
+PYPDE_UPDATE_ROWS_BATCH(8)
 #endif

@@ -65,6 +65,12 @@ def standard_configs(full=False):
         s = SYSTEMS[name]
         out.append(dict(src=source(name), useF=s["F"], useB=s["B"], useS=s["S"], ndim=ndim, N=N,
                         V=s["V"], stiff=stiff, use_M=s["M"], flux=0, mixed=True))
+    # ensemble members (code objects with the batched ensemble kernels): 2D Euler,
+    # order 3
+    for name, ndim, N, stiff in ([("euler2d", 2, 3, False)] if full else []):
+        s = SYSTEMS[name]
+        out.append(dict(src=source(name), useF=s["F"], useB=s["B"], useS=s["S"], ndim=ndim, N=N,
+                        V=s["V"], stiff=stiff, use_M=s["M"], flux=0, ensemble=True))
     # 17-variable GPR (1D, order 2, the reference's B.dQ evaluation); each takes
     # a few minutes to compile
     for stiff in ((False, True) if full else ()):

@@ -299,12 +299,18 @@ def test_graph_replay_equals_eager():
 
 
 @pytest.mark.gpu
-def test_ensemble_members_equal_single_runs():
+@pytest.mark.parametrize("mode,cap", [("batched", None), ("batched", "1500"), ("streams", None)])
+def test_ensemble_members_equal_single_runs(mode, cap, monkeypatch):
     """Ensemble: independent grids of different sizes with seeded 4-quadrant
     Riemann data, advanced together (one graph launch per step, each grid
-    with its own dt); every member equals its own single-grid run bit for
-    bit, and the smallest ones equal the oracle."""
+    with its own dt), batched (one launch per stage over every grid, or over
+    groups of at most 1500 reconstructed cells) or one stream per grid; every
+    member equals its own single-grid run bit for bit, and the smallest ones
+    equal the oracle."""
     from pypde_amd.solvers import Ensemble
+    monkeypatch.setenv("PYPDE_ENSEMBLE", mode)
+    if cap is not None:
+        monkeypatch.setenv("PYPDE_ENSEMBLE_CELLS", cap)
     shapes = systems.ensemble_sizes(6, sizes=(16, 24, 32, 48))
     grids = [systems.riemann4(nx, ny, seed=i) for i, (nx, ny) in enumerate(shapes)]
     Ls = [[1.0, 1.0]] * len(grids)
