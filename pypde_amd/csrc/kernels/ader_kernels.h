@@ -61,6 +61,15 @@ DEV Blk hw_blk() {
   return {blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x * gridDim.y * gridDim.z};
 }
 
+// Position of workgroup i of n in an order that gives each of the 8 XCDs
+// one contiguous run of positions: workgroups are dispatched to the XCDs
+// round-robin (i to XCD i mod 8), so neighbouring positions, whose cells
+// share cache lines, meet in one XCD's L2.  A bijection of [0, n).
+DEV unsigned int xcd_order(unsigned int i, unsigned int n) {
+  const unsigned int q = n >> 3, r = n & 7u, x = i & 7u, k = i >> 3;
+  return (x < r ? x * (q + 1u) : r * (q + 1u) + (x - r) * q) + k;
+}
+
 DEV double wave_max(double x) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) x = dmax(x, __shfl_xor(x, o, 64));

@@ -808,7 +808,7 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
     valid = (c.slot < CPW) && (i < st->hot_count);
     cell = valid ? (long)hot[i] : 0;
   } else {
-    cell = (long)b.x * CPW + c.slot;
+    cell = (long)xcd_order(b.x, b.n) * CPW + c.slot;
     valid = (c.slot < CPW) && (cell < g.nwc);
   }
   double w[V], q[N][V], rhs[N][V], dqs[N][D][V];
