@@ -177,7 +177,7 @@ def ensemble_leg(steps):
     """48 independent 2D Euler grids, sides drawn from {64,128,256,512},
     seeded four-quadrant Riemann data, order 3, transmissive walls, each grid
     with its own dt; every timestep of all grids is one HIP graph launch:
-    batched in groups of at most 2^16 cells (the value), batched in one launch
+    batched in groups of at most 2^18 cells (the value), batched in one launch
     per stage over all grids, or one stream per grid inside the graph.  All
     are timed against the same grids run one after another."""
     from pypde_amd.solvers import Ensemble
@@ -187,7 +187,7 @@ def ensemble_leg(steps):
     src = systems.source("euler2d")
     cells = sum(nx * ny for nx, ny in shapes)
     rates = {}
-    # groups of at most 2^16 cells (the default), one launch per stage over
+    # groups of at most 2^18 cells (the default), one launch per stage over
     # all grids, one stream per grid
     for mode, env, cap in (("batched", "batched", None), ("one_launch", "batched", "0"),
                            ("streams", "streams", None)):
@@ -273,6 +273,8 @@ def window(st, w):
 
 PREDICTOR_KERNELS = {"split": ["k_predictor_cold", "k_predictor_hot"],
                      "split3": ["k_predictor_cold3", "k_predictor_hot"],
+                     "split_hot2": ["k_predictor_cold", "k_predictor_hot2"],
+                     "split3_hot2": ["k_predictor_cold3", "k_predictor_hot2"],
                      "static": ["k_predictor_static"], "queue": ["k_predictor"]}
 
 

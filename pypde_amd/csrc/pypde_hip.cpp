@@ -293,7 +293,7 @@ enum { K_WENO, K_DT, K_PRED, K_FLUX0, K_FLUX1, K_UPDATE, K_A2S, K_S2A,
        K_B_WENO, K_B_WENO_4x64, K_B_WENO_16x16, K_B_WENO_16x32, K_B_DT, K_B_PRED_STATIC,
        K_B_PRED_STATIC2, K_B_PRED_COLD, K_B_PRED_COLD3, K_B_PRED_HOT, K_B_FLUX0, K_B_FLUX0_128,
        K_B_FLUX0_64, K_B_FLUX1, K_B_FLUX1_128, K_B_FLUX1_64, K_B_UPDATE, K_B_UPD1, K_B_UPD2,
-       K_B_UPD4, K_B_UPD8, K_COUNT };
+       K_B_UPD4, K_B_UPD8, K_PRED_HOT2, K_B_PRED_HOT2, K_COUNT };
 const char* kernel_names[K_COUNT] = {"k_weno",   "k_dt",    "k_predictor",
                                      "k_flux0",  "k_flux1", "k_update",
                                      "k_aos_to_soa", "k_soa_to_aos", "k_harvest", "k_mlp_guess",
@@ -310,15 +310,16 @@ const char* kernel_names[K_COUNT] = {"k_weno",   "k_dt",    "k_predictor",
                                      "k_flux0_b", "k_flux0_128_b", "k_flux0_64_b",
                                      "k_flux1_b", "k_flux1_128_b", "k_flux1_64_b",
                                      "k_update_b", "k_update_r1_b", "k_update_r2_b", "k_update_r4_b",
-                                     "k_update_r8_b"};
+                                     "k_update_r8_b", "k_predictor_hot2", "k_predictor_hot2_b"};
 
 // Launch variants the autotuner chooses between (every variant of a kernel
 // computes identical results): WENO tiles, flux workgroup sizes, corrector
 // mappings (one thread per cell and all variables, or one variable and R
 // x-rows per thread), predictor schedules (static, work queue, and for the
 // stiff predictor with the plain guess the cold/hot split, its cold pass
-// built for two or for three wavefronts per SIMD).
-constexpr int N_WENO_V = 4, N_FLUX_V = 3, N_UPD_V = 5, N_PRED_V = 4;
+// built for two or for three wavefronts per SIMD, its hot pass for one or
+// two: 2 and 3 with the one-wavefront hot pass, 4 and 5 with the other).
+constexpr int N_WENO_V = 4, N_FLUX_V = 3, N_UPD_V = 5, N_PRED_V = 6;
 const int WENO_TILE[N_WENO_V][2] = {{8, 32}, {4, 64}, {16, 16}, {16, 32}};
 const int WENO_KID[N_WENO_V] = {K_WENO, K_WENO_4x64, K_WENO_16x16, K_WENO_16x32};
 const int FLUX_BS[N_FLUX_V] = {256, 128, 64};
@@ -372,6 +373,7 @@ struct pypde_ctx {
   bool pred_fixed = false;  // schedule fixed by PYPDE_PRED_SCHEDULE (not tuned)
   bool pred_split = false;  // cold pass over every cell, hot pass over the cells it hands on
   bool pred_cold3 = false;  // ... with the cold pass built for three wavefronts per SIMD
+  bool pred_hot2 = false;   // ... with the hot pass built for two wavefronts per SIMD
   unsigned int* hot = nullptr;  // the hot pass's cell list (nwc entries)
   struct {
     int weno = 0, flux = 0, update = 0;  // variant indices (WENO_KID, FLUX_KID, UPD_KID)
@@ -426,6 +428,7 @@ int load_module(pypde_ctx* c, const std::vector<char>& code, uint64_t key, hipMo
     bool optional = ((k == K_FLUX1 || k == K_FLUX1_128 || k == K_FLUX1_64) && c->D < 2) ||
                     k == K_MLPGUESS || k == K_MLPPACK || (k == K_PRED32 && !c->spec.mixed) ||
                     k == K_PRED_COLD || k == K_PRED_HOT || k == K_PRED_COLD3 || k >= K_B_WENO;
+    // (K_PRED_HOT2 and K_B_PRED_HOT2 come after K_B_WENO: optional as well)
     if (hipModuleGetFunction(&fn[k], *mod, kernel_names[k]) != hipSuccess) {
       fn[k] = nullptr;
       if (!optional) return fail(PYPDE_ERR_HIP, "kernel %s missing", kernel_names[k]);
@@ -552,7 +555,8 @@ int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
       const unsigned blocks = (unsigned)cdiv(g.nwc, c->CPW);
       const int kc = (c->pred_cold3 && c->fn[K_PRED_COLD3]) ? K_PRED_COLD3 : K_PRED_COLD;
       if ((rc = launch(c, kc, dim3(blocks), dim3(64), b))) return rc;
-      if ((rc = launch(c, K_PRED_HOT, dim3(blocks), dim3(64), b))) return rc;
+      const int kh = (c->pred_hot2 && c->fn[K_PRED_HOT2]) ? K_PRED_HOT2 : K_PRED_HOT;
+      if ((rc = launch(c, kh, dim3(blocks), dim3(64), b))) return rc;
     } else if (c->pred_static) {
       if ((rc = launch(c, c->pred_w2 ? K_PRED_STATIC2 : K_PRED_STATIC,
                        dim3((unsigned)cdiv(g.nwc, c->CPW)), dim3(64), a)))
@@ -916,8 +920,9 @@ void apply_tuning(pypde_ctx* c, const std::vector<int>& v) {
   c->tune.update = (v[2] >= 0 && v[2] < N_UPD_V) ? v[2] : 0;
   if (!c->pred_fixed) {
     c->pred_static = v[3] != 1;
-    c->pred_split = (v[3] == 2 || v[3] == 3) && c->fn[K_PRED_COLD] != nullptr;
-    c->pred_cold3 = v[3] == 3;
+    c->pred_split = v[3] >= 2 && v[3] < N_PRED_V && c->fn[K_PRED_COLD] != nullptr;
+    c->pred_cold3 = v[3] == 3 || v[3] == 5;
+    c->pred_hot2 = v[3] >= 4;
   }
 }
 
@@ -965,7 +970,8 @@ int autotune(pypde_ctx* c) {
   HIPCHK(hipMemcpyAsync(c->tmp, c->u, ub, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(st_bak, c->st, sizeof(StepState), hipMemcpyDeviceToDevice, c->stream));
   double* keep_ret = c->ret;
-  const bool keep_static = c->pred_static, keep_split = c->pred_split, keep_c3 = c->pred_cold3;
+  const bool keep_static = c->pred_static, keep_split = c->pred_split, keep_c3 = c->pred_cold3,
+             keep_h2 = c->pred_hot2;
   c->ret = nullptr;
   const int nw = c->D == 2 ? N_WENO_V : 1;
   const int np = c->pred_fixed ? 1 : (c->fn[K_PRED_COLD] && c->hot ? N_PRED_V : 2);
@@ -981,7 +987,8 @@ int autotune(pypde_ctx* c) {
     if (!c->pred_fixed) {
       c->pred_static = (tk % np) != 1;
       c->pred_split = (tk % np) >= 2;
-      c->pred_cold3 = (tk % np) == 3;
+      c->pred_cold3 = (tk % np) == 3 || (tk % np) == 5;
+      c->pred_hot2 = (tk % np) >= 4;
     }
     rc = enqueue_step(c, false, &evs[k]);
     if (!rc && hipMemcpyAsync(c->u, c->tmp, ub, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
@@ -1018,6 +1025,7 @@ int autotune(pypde_ctx* c) {
   c->pred_static = keep_static;
   c->pred_split = keep_split;
   c->pred_cold3 = keep_c3;
+  c->pred_hot2 = keep_h2;
   if (rc) return rc;
   auto best = [](const double* t, int n) {
     int b = 0;
@@ -1226,7 +1234,7 @@ struct EnsGroup {
   // launch variants: those the autotuner chose for the group's largest member
   // (the work queue replaced by the static schedule)
   int weno = 0, flux = 0, update = 0;
-  int pred = 0;  // 0 static, 1 static (two-wavefront budget), 2 split, 3 split (cold pass x3)
+  int pred = 0;  // 0 static, 1 static (two-wavefront budget), 2-5 split (as the tuner's choice)
 };
 
 struct pypde_ensemble {
@@ -1254,13 +1262,12 @@ void pypde_ensemble_destroy(pypde_ensemble* e) {
 // are taken largest first and packed greedily into groups of at most
 // `cap` reconstructed cells (a larger member forms its own group, which runs
 // the member's own single-grid step).  Measured on 2D Euler, order 3: one
-// launch per stage over all 48 benchmark grids (3.1M cells) is 1.4x slower
-// than one stream per grid, because between two stages of such a launch
+// launch per stage over all 48 benchmark grids (3.1M cells) is 6 % slower
+// than groups of at most 2^18 cells, as between two stages of one launch
 // the data of every grid passes through HBM instead of staying in the
-// last-level cache; groups of at most 2^16 cells keep that locality and
-// still need far fewer launches than one stream per member (48 grids of
-// 64 x 64 cells: 1.9x the rate of one stream per grid).  Each group runs
-// with the variants the autotuner chose for its largest member.
+// last-level cache, while on 48 grids of 64 x 64 cells (one group either
+// way) batching runs at 2.1x one stream per grid.  Each group runs with the
+// variants the autotuner chose for its largest member.
 static int batch_setup(pypde_ensemble* e, long cap) {
   const int nm = (int)e->m.size();
   std::vector<int>& order = e->order;
@@ -1283,7 +1290,8 @@ static int batch_setup(pypde_ensemble* e, long cap) {
       gr.flux = big->tune.flux;
       gr.update = big->tune.update;
       if (big->pred_split && big->hot && big->fn[K_B_PRED_COLD] && big->fn[K_B_PRED_HOT])
-        gr.pred = (big->pred_cold3 && big->fn[K_B_PRED_COLD3]) ? 3 : 2;
+        gr.pred = ((big->pred_cold3 && big->fn[K_B_PRED_COLD3]) ? 3 : 2) +
+                  ((big->pred_hot2 && big->fn[K_B_PRED_HOT2]) ? 2 : 0);
       else if (big->pred_static && big->pred_w2)
         gr.pred = 1;
       e->groups.push_back(gr);
@@ -1394,8 +1402,9 @@ static int enqueue_group_step(pypde_ensemble* e, const EnsGroup& gr) {
   if ((rc = go(B_WENO_KID[gr.weno], BS_WENO, 256, false))) return rc;
   if ((rc = go(K_B_DT, BS_DT, 256, false))) return rc;
   if (gr.pred >= 2) {
-    if ((rc = go(gr.pred == 3 ? K_B_PRED_COLD3 : K_B_PRED_COLD, BS_PRED, 64, true))) return rc;
-    if ((rc = go(K_B_PRED_HOT, BS_PRED, 64, true))) return rc;
+    const bool c3 = gr.pred == 3 || gr.pred == 5;
+    if ((rc = go(c3 ? K_B_PRED_COLD3 : K_B_PRED_COLD, BS_PRED, 64, true))) return rc;
+    if ((rc = go(gr.pred >= 4 ? K_B_PRED_HOT2 : K_B_PRED_HOT, BS_PRED, 64, true))) return rc;
   } else if ((rc = go(gr.pred == 1 ? K_B_PRED_STATIC2 : K_B_PRED_STATIC, BS_PRED, 64, true))) {
     return rc;
   }
@@ -1516,7 +1525,7 @@ int pypde_ensemble_run(pypde_ensemble* e, double tf, long max_steps, pypde_stats
   // size cap (reconstructed cells; 0: one group)
   const char* mode = getenv("PYPDE_ENSEMBLE");
   const bool batched = c0->fn[K_B_DT] && !(mode && strcmp(mode, "streams") == 0);
-  long cap = 1L << 16;
+  long cap = 1L << 18;
   if (const char* v = getenv("PYPDE_ENSEMBLE_CELLS")) cap = atol(v);
   auto t0 = std::chrono::steady_clock::now();
   hipGraphExec_t exec = nullptr;
@@ -1718,7 +1727,8 @@ int pypde_get_tuning(pypde_ctx* c, int* choices, double* ms) {
     choices[0] = c->tune.weno;
     choices[1] = c->tune.flux;
     choices[2] = c->tune.update;
-    choices[3] = c->pred_split ? (c->pred_cold3 ? 3 : 2) : (c->pred_static ? 0 : 1);
+    choices[3] = c->pred_split ? (c->pred_cold3 ? 3 : 2) + (c->pred_hot2 ? 2 : 0)
+                               : (c->pred_static ? 0 : 1);
     choices[4] = c->tune.source;
   }
   if (ms)
