@@ -66,8 +66,9 @@ def standard_configs(full=False):
         out.append(dict(src=source(name), useF=s["F"], useB=s["B"], useS=s["S"], ndim=ndim, N=N,
                         V=s["V"], stiff=stiff, use_M=s["M"], flux=0, mixed=True))
     # ensemble members (code objects with the batched ensemble kernels): 2D Euler,
-    # order 3
-    for name, ndim, N, stiff in ([("euler2d", 2, 3, False)] if full else []):
+    # the stiff reactive channel and 1D Euler, order 3
+    for name, ndim, N, stiff in ([("euler2d", 2, 3, False), ("reactive2d", 2, 3, True),
+                                  ("euler1d", 1, 3, False)] if full else []):
         s = SYSTEMS[name]
         out.append(dict(src=source(name), useF=s["F"], useB=s["B"], useS=s["S"], ndim=ndim, N=N,
                         V=s["V"], stiff=stiff, use_M=s["M"], flux=0, ensemble=True))

@@ -334,6 +334,47 @@ def test_ensemble_members_equal_single_runs(mode, cap, monkeypatch):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("case", ["reactive_stiff", "sod_1d"])
+def test_batched_ensemble_stiff_and_1d(case, monkeypatch):
+    """The batched ensemble kernels beyond 2D Euler: the stiff predictor's
+    cold/hot split (reactive channel windows, reflective walls, each grid's
+    hot cells listed and solved within the batched launches) and the 1D
+    kernels (Sod tubes of different lengths).  Every member equals its own
+    single-grid run bit for bit; the first also matches the oracle."""
+    from pypde_amd.solvers import Ensemble
+    monkeypatch.setenv("PYPDE_ENSEMBLE", "batched")
+    if case == "reactive_stiff":
+        shapes = [(40, 16), (24, 24), (32, 12), (16, 40)]
+        grids = [systems.reactive_channel(nx, ny, seed=i + 3) for i, (nx, ny) in enumerate(shapes)]
+        Ls = [[nx / 1024, ny / 1024] for nx, ny in shapes]
+        kw = dict(boundaryTypes="reflective", stiff=True, reflect=systems.REACTIVE_REFLECT)
+        system, N, steps, bcode = "reactive2d", 3, 8, 2
+    else:
+        grids = [systems.sod1d(n) for n in (50, 120, 200, 77, 150)]
+        Ls = [[1.0]] * len(grids)
+        kw = dict(boundaryTypes="transitive", stiff=False)
+        system, N, steps, bcode = "euler1d", 3, 25, 0
+    src = systems.source(system)
+    ens = Ensemble(grids, Ls, src, order=N, **kw)
+    stats, _ = ens.run(1.0, max_steps=steps)
+    outs = [ens.state(i) for i in range(len(grids))]
+    ens.close()
+    for i, g in enumerate(grids):
+        sol = Solver(g, Ls[i], src, order=N, **kw)
+        st = sol.run(1.0, max_steps=steps)
+        single = sol.state()
+        sol.close()
+        assert stats[i]["steps"] == st["steps"] == steps, i
+        assert stats[i]["pred_iters"] == st["pred_iters"], i
+        assert np.array_equal(outs[i], single), i
+    refl = kw.get("reflect")
+    uo, _, sto, rc = cc.oracle_solve(system, grids[0], 1.0, Ls[0], bcode, N,
+                                     stiff=kw["stiff"], max_steps=steps,
+                                     reflect=None if refl is None else refl.ravel())
+    assert rc == 0 and sto.steps == steps and relerr(outs[0], uo) <= TOL
+
+
+@pytest.mark.gpu
 def test_predictor_schedules_agree():
     """The static (one wavefront per cell group), its two-wavefront register
     budget variant and the persistent work-queued predictor give the same bits
