@@ -273,8 +273,6 @@ def window(st, w):
 
 PREDICTOR_KERNELS = {"split": ["k_predictor_cold", "k_predictor_hot"],
                      "split3": ["k_predictor_cold3", "k_predictor_hot"],
-                     "split_hot2": ["k_predictor_cold", "k_predictor_hot2"],
-                     "split3_hot2": ["k_predictor_cold3", "k_predictor_hot2"],
                      "static": ["k_predictor_static"], "queue": ["k_predictor"]}
 
 

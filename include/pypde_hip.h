@@ -175,8 +175,7 @@ int pypde_harvest_step(pypde_ctx *c, float *X, float *Y, long ldx, long col0, py
  * 256, 1 128, 2 64), corrector (0 one thread per cell, 1..4 one variable and
  * 1, 2, 4, 8 x-rows per thread), predictor (0 static, 1 work queue, 2 the
  * stiff predictor's cold/hot split, 3 the same with the cold pass built for
- * three wavefronts per SIMD, 4 and 5 as 2 and 3 with the hot pass built for
- * two wavefronts per SIMD), source (0 defaults, 1 measured, 2 tuning table,
+ * three wavefronts per SIMD), source (0 defaults, 1 measured, 2 tuning table,
  * 3 PYPDE_TUNE_FORCE); ms[4]: the chosen WENO, predictor, flux (both
  * directions) and corrector trial times. */
 int pypde_get_tuning(pypde_ctx *c, int *choices, double *ms);

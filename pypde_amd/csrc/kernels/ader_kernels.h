@@ -62,12 +62,18 @@ DEV Blk hw_blk() {
 }
 
 // Position of workgroup i of n in an order that gives each of the 8 XCDs
-// one contiguous run of positions: workgroups are dispatched to the XCDs
-// round-robin (i to XCD i mod 8), so neighbouring positions, whose cells
-// share cache lines, meet in one XCD's L2.  A bijection of [0, n).
+// runs of XCD_RUN consecutive positions: workgroups are dispatched to the
+// XCDs round-robin (i to XCD i mod 8), so neighbouring positions, whose
+// cells share cache lines, meet in one XCD's L2, while costly regions of a
+// grid (a reaction zone) still spread over every XCD.  Runs of
+// 8 x XCD_RUN positions go round the XCDs; the n mod (8 x XCD_RUN) last
+// positions keep their order.  A bijection of [0, n).
+constexpr unsigned int XCD_RUN = 16;
 DEV unsigned int xcd_order(unsigned int i, unsigned int n) {
-  const unsigned int q = n >> 3, r = n & 7u, x = i & 7u, k = i >> 3;
-  return (x < r ? x * (q + 1u) : r * (q + 1u) + (x - r) * q) + k;
+  constexpr unsigned int SPAN = 8u * XCD_RUN;
+  if (i >= n - n % SPAN) return i;
+  const unsigned int x = i & 7u, k = i >> 3;
+  return (k / XCD_RUN) * SPAN + x * XCD_RUN + k % XCD_RUN;
 }
 
 DEV double wave_max(double x) {

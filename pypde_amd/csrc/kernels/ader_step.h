@@ -1113,17 +1113,6 @@ k_predictor_hot(const double* __restrict__ wh, double* __restrict__ tr, double* 
   predictor_static_body<2>(sm, wh, nullptr, tr, vol, qh, st, g, newton_tol, newton_maxit,
                            nullptr, hw_blk(), hot);
 }
-// The hot pass with the register budget of two wavefronts per SIMD (256
-// registers; the rest spills), for the autotuner to weigh against the
-// one-wavefront build.
-extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_predictor_hot2(const double* __restrict__ wh, double* __restrict__ tr, double* __restrict__ vol,
-                 double* __restrict__ qh, StepState* st, Geom g, double newton_tol,
-                 int newton_maxit, unsigned int* __restrict__ hot) {
-  __shared__ PredLDS sm;
-  predictor_static_body<2>(sm, wh, nullptr, tr, vol, qh, st, g, newton_tol, newton_maxit,
-                           nullptr, hw_blk(), hot);
-}
 #endif
 
 // The same with the register budget of two wavefronts per SIMD (spilling the
@@ -2119,14 +2108,7 @@ k_predictor_hot_b(const Member* ms, const BlkMap* map, int nm, double newton_tol
   predictor_static_body<2>(sm, M.wh, nullptr, M.tr, M.vol, nullptr, M.st, M.g, newton_tol,
                            newton_maxit, nullptr, b, M.hot);
 }
-extern "C" __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_predictor_hot2_b(const Member* ms, const BlkMap* map, int nm, double newton_tol,
-                   int newton_maxit) {
-  __shared__ PredLDS sm;
-  PYPDE_BATCH_MEMBER
-  predictor_static_body<2>(sm, M.wh, nullptr, M.tr, M.vol, nullptr, M.st, M.g, newton_tol,
-                           newton_maxit, nullptr, b, M.hot);
-}
+
 #endif
 
 #define PYPDE_FLUX_BATCH(NAME, DIR, BS)                                                        \

@@ -293,7 +293,7 @@ enum { K_WENO, K_DT, K_PRED, K_FLUX0, K_FLUX1, K_UPDATE, K_A2S, K_S2A,
        K_B_WENO, K_B_WENO_4x64, K_B_WENO_16x16, K_B_WENO_16x32, K_B_DT, K_B_PRED_STATIC,
        K_B_PRED_STATIC2, K_B_PRED_COLD, K_B_PRED_COLD3, K_B_PRED_HOT, K_B_FLUX0, K_B_FLUX0_128,
        K_B_FLUX0_64, K_B_FLUX1, K_B_FLUX1_128, K_B_FLUX1_64, K_B_UPDATE, K_B_UPD1, K_B_UPD2,
-       K_B_UPD4, K_B_UPD8, K_PRED_HOT2, K_B_PRED_HOT2, K_COUNT };
+       K_B_UPD4, K_B_UPD8, K_COUNT };
 const char* kernel_names[K_COUNT] = {"k_weno",   "k_dt",    "k_predictor",
                                      "k_flux0",  "k_flux1", "k_update",
                                      "k_aos_to_soa", "k_soa_to_aos", "k_harvest", "k_mlp_guess",
@@ -310,16 +310,15 @@ const char* kernel_names[K_COUNT] = {"k_weno",   "k_dt",    "k_predictor",
                                      "k_flux0_b", "k_flux0_128_b", "k_flux0_64_b",
                                      "k_flux1_b", "k_flux1_128_b", "k_flux1_64_b",
                                      "k_update_b", "k_update_r1_b", "k_update_r2_b", "k_update_r4_b",
-                                     "k_update_r8_b", "k_predictor_hot2", "k_predictor_hot2_b"};
+                                     "k_update_r8_b"};
 
 // Launch variants the autotuner chooses between (every variant of a kernel
 // computes identical results): WENO tiles, flux workgroup sizes, corrector
 // mappings (one thread per cell and all variables, or one variable and R
 // x-rows per thread), predictor schedules (static, work queue, and for the
 // stiff predictor with the plain guess the cold/hot split, its cold pass
-// built for two or for three wavefronts per SIMD, its hot pass for one or
-// two: 2 and 3 with the one-wavefront hot pass, 4 and 5 with the other).
-constexpr int N_WENO_V = 4, N_FLUX_V = 3, N_UPD_V = 5, N_PRED_V = 6;
+// built for two or for three wavefronts per SIMD).
+constexpr int N_WENO_V = 4, N_FLUX_V = 3, N_UPD_V = 5, N_PRED_V = 4;
 const int WENO_TILE[N_WENO_V][2] = {{8, 32}, {4, 64}, {16, 16}, {16, 32}};
 const int WENO_KID[N_WENO_V] = {K_WENO, K_WENO_4x64, K_WENO_16x16, K_WENO_16x32};
 const int FLUX_BS[N_FLUX_V] = {256, 128, 64};
@@ -346,6 +345,7 @@ struct pypde_ctx {
   Geom g;
   int D, N, V, ND, NT, CPW;
   hipModule_t mod = nullptr;
+  uint64_t code_key = 0;  // the code object's cache key (kernel source and options)
   hipFunction_t fn[K_COUNT] = {};
   hipStream_t stream = nullptr;
   double *u = nullptr, *wh = nullptr, *tr = nullptr, *vol = nullptr, *ff[2] = {}, *fb[2] = {},
@@ -373,7 +373,6 @@ struct pypde_ctx {
   bool pred_fixed = false;  // schedule fixed by PYPDE_PRED_SCHEDULE (not tuned)
   bool pred_split = false;  // cold pass over every cell, hot pass over the cells it hands on
   bool pred_cold3 = false;  // ... with the cold pass built for three wavefronts per SIMD
-  bool pred_hot2 = false;   // ... with the hot pass built for two wavefronts per SIMD
   unsigned int* hot = nullptr;  // the hot pass's cell list (nwc entries)
   struct {
     int weno = 0, flux = 0, update = 0;  // variant indices (WENO_KID, FLUX_KID, UPD_KID)
@@ -428,7 +427,6 @@ int load_module(pypde_ctx* c, const std::vector<char>& code, uint64_t key, hipMo
     bool optional = ((k == K_FLUX1 || k == K_FLUX1_128 || k == K_FLUX1_64) && c->D < 2) ||
                     k == K_MLPGUESS || k == K_MLPPACK || (k == K_PRED32 && !c->spec.mixed) ||
                     k == K_PRED_COLD || k == K_PRED_HOT || k == K_PRED_COLD3 || k >= K_B_WENO;
-    // (K_PRED_HOT2 and K_B_PRED_HOT2 come after K_B_WENO: optional as well)
     if (hipModuleGetFunction(&fn[k], *mod, kernel_names[k]) != hipSuccess) {
       fn[k] = nullptr;
       if (!optional) return fail(PYPDE_ERR_HIP, "kernel %s missing", kernel_names[k]);
@@ -555,8 +553,7 @@ int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
       const unsigned blocks = (unsigned)cdiv(g.nwc, c->CPW);
       const int kc = (c->pred_cold3 && c->fn[K_PRED_COLD3]) ? K_PRED_COLD3 : K_PRED_COLD;
       if ((rc = launch(c, kc, dim3(blocks), dim3(64), b))) return rc;
-      const int kh = (c->pred_hot2 && c->fn[K_PRED_HOT2]) ? K_PRED_HOT2 : K_PRED_HOT;
-      if ((rc = launch(c, kh, dim3(blocks), dim3(64), b))) return rc;
+      if ((rc = launch(c, K_PRED_HOT, dim3(blocks), dim3(64), b))) return rc;
     } else if (c->pred_static) {
       if ((rc = launch(c, c->pred_w2 ? K_PRED_STATIC2 : K_PRED_STATIC,
                        dim3((unsigned)cdiv(g.nwc, c->CPW)), dim3(64), a)))
@@ -771,6 +768,7 @@ static pypde_ctx* create_ctx(const char* src, bool useF, bool useB, bool useS, c
   if ((rc = compile_code(s, &code, &code_key))) return nullptr;
   pypde_ctx* c = new pypde_ctx();
   c->spec = s;
+  c->code_key = code_key;
   c->opts = opts;
   c->D = ndim;
   c->N = N;
@@ -908,7 +906,8 @@ uint64_t tune_key(pypde_ctx* c) {
   std::ostringstream k;
   k << c->spec.src << "|" << c->D << c->N << c->V << c->spec.useF << c->spec.useB << c->spec.useS
     << c->spec.useM << c->spec.stiff << c->spec.guess << c->spec.mixed << c->spec.flux
-    << c->spec.second << c->spec.b_quirk << "|" << c->g.nx[0] << "x" << c->g.nx[1] << "|";
+    << c->spec.second << c->spec.b_quirk << "|" << c->g.nx[0] << "x" << c->g.nx[1] << "|"
+    << std::hex << c->code_key << std::dec << "|";
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) k << prop.gcnArchName;
   return fnv1a(k.str());
@@ -920,9 +919,8 @@ void apply_tuning(pypde_ctx* c, const std::vector<int>& v) {
   c->tune.update = (v[2] >= 0 && v[2] < N_UPD_V) ? v[2] : 0;
   if (!c->pred_fixed) {
     c->pred_static = v[3] != 1;
-    c->pred_split = v[3] >= 2 && v[3] < N_PRED_V && c->fn[K_PRED_COLD] != nullptr;
-    c->pred_cold3 = v[3] == 3 || v[3] == 5;
-    c->pred_hot2 = v[3] >= 4;
+    c->pred_split = (v[3] == 2 || v[3] == 3) && c->fn[K_PRED_COLD] != nullptr;
+    c->pred_cold3 = v[3] == 3;
   }
 }
 
@@ -969,9 +967,29 @@ int autotune(pypde_ctx* c) {
   HIPCHK(hipMalloc((void**)&st_bak, sizeof(StepState)));
   HIPCHK(hipMemcpyAsync(c->tmp, c->u, ub, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(st_bak, c->st, sizeof(StepState), hipMemcpyDeviceToDevice, c->stream));
+  // every trial step starts from the run's state, but with the step count
+  // past the first steps' dt ramp (x0.2 while count <= 5, stepper.cpp:69-71):
+  // at the ramped dt the stiff predictor has fewer full Newton solves and
+  // iterations than for the rest of the run, which would favour the wrong
+  // variants
+  StepState trial;
+  HIPCHK(hipMemcpyAsync(&trial, c->st, sizeof(StepState), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (trial.count < 6) trial.count = 6;
+  StepState* st_trial = nullptr;
+  if (hipMalloc((void**)&st_trial, sizeof(StepState)) != hipSuccess) {
+    (void)hipFree(st_bak);
+    return fail(PYPDE_ERR_HIP, "autotune: allocation failed");
+  }
+  if (hipMemcpy(st_trial, &trial, sizeof(StepState), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpyAsync(c->st, st_trial, sizeof(StepState), hipMemcpyDeviceToDevice, c->stream) !=
+          hipSuccess) {
+    (void)hipFree(st_bak);
+    (void)hipFree(st_trial);
+    return fail(PYPDE_ERR_HIP, "autotune: trial state upload failed");
+  }
   double* keep_ret = c->ret;
-  const bool keep_static = c->pred_static, keep_split = c->pred_split, keep_c3 = c->pred_cold3,
-             keep_h2 = c->pred_hot2;
+  const bool keep_static = c->pred_static, keep_split = c->pred_split, keep_c3 = c->pred_cold3;
   c->ret = nullptr;
   const int nw = c->D == 2 ? N_WENO_V : 1;
   const int np = c->pred_fixed ? 1 : (c->fn[K_PRED_COLD] && c->hot ? N_PRED_V : 2);
@@ -987,18 +1005,21 @@ int autotune(pypde_ctx* c) {
     if (!c->pred_fixed) {
       c->pred_static = (tk % np) != 1;
       c->pred_split = (tk % np) >= 2;
-      c->pred_cold3 = (tk % np) == 3 || (tk % np) == 5;
-      c->pred_hot2 = (tk % np) >= 4;
+      c->pred_cold3 = (tk % np) == 3;
     }
     rc = enqueue_step(c, false, &evs[k]);
     if (!rc && hipMemcpyAsync(c->u, c->tmp, ub, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
       rc = fail(PYPDE_ERR_HIP, "autotune: state restore failed");
-    if (!rc && hipMemcpyAsync(c->st, st_bak, sizeof(StepState), hipMemcpyDeviceToDevice,
+    if (!rc && hipMemcpyAsync(c->st, st_trial, sizeof(StepState), hipMemcpyDeviceToDevice,
                               c->stream) != hipSuccess)
       rc = fail(PYPDE_ERR_HIP, "autotune: state restore failed");
   }
+  if (!rc && hipMemcpyAsync(c->st, st_bak, sizeof(StepState), hipMemcpyDeviceToDevice,
+                            c->stream) != hipSuccess)
+    rc = fail(PYPDE_ERR_HIP, "autotune: state restore failed");
   if (!rc && hipStreamSynchronize(c->stream) != hipSuccess)
     rc = fail(PYPDE_ERR_HIP, "autotune: trial steps failed");
+  (void)hipFree(st_trial);
   // per variant, the fastest of its trials; event marks per step:
   // weno | dt | guess | predictor | flux0 | flux1 | update | advance
   double tw[N_WENO_V], tf[N_FLUX_V], tu[N_UPD_V], tp[N_PRED_V];
@@ -1025,7 +1046,6 @@ int autotune(pypde_ctx* c) {
   c->pred_static = keep_static;
   c->pred_split = keep_split;
   c->pred_cold3 = keep_c3;
-  c->pred_hot2 = keep_h2;
   if (rc) return rc;
   auto best = [](const double* t, int n) {
     int b = 0;
@@ -1234,7 +1254,7 @@ struct EnsGroup {
   // launch variants: those the autotuner chose for the group's largest member
   // (the work queue replaced by the static schedule)
   int weno = 0, flux = 0, update = 0;
-  int pred = 0;  // 0 static, 1 static (two-wavefront budget), 2-5 split (as the tuner's choice)
+  int pred = 0;  // 0 static, 1 static (two-wavefront budget), 2 split, 3 split (cold pass x3)
 };
 
 struct pypde_ensemble {
@@ -1290,8 +1310,7 @@ static int batch_setup(pypde_ensemble* e, long cap) {
       gr.flux = big->tune.flux;
       gr.update = big->tune.update;
       if (big->pred_split && big->hot && big->fn[K_B_PRED_COLD] && big->fn[K_B_PRED_HOT])
-        gr.pred = ((big->pred_cold3 && big->fn[K_B_PRED_COLD3]) ? 3 : 2) +
-                  ((big->pred_hot2 && big->fn[K_B_PRED_HOT2]) ? 2 : 0);
+        gr.pred = (big->pred_cold3 && big->fn[K_B_PRED_COLD3]) ? 3 : 2;
       else if (big->pred_static && big->pred_w2)
         gr.pred = 1;
       e->groups.push_back(gr);
@@ -1402,9 +1421,8 @@ static int enqueue_group_step(pypde_ensemble* e, const EnsGroup& gr) {
   if ((rc = go(B_WENO_KID[gr.weno], BS_WENO, 256, false))) return rc;
   if ((rc = go(K_B_DT, BS_DT, 256, false))) return rc;
   if (gr.pred >= 2) {
-    const bool c3 = gr.pred == 3 || gr.pred == 5;
-    if ((rc = go(c3 ? K_B_PRED_COLD3 : K_B_PRED_COLD, BS_PRED, 64, true))) return rc;
-    if ((rc = go(gr.pred >= 4 ? K_B_PRED_HOT2 : K_B_PRED_HOT, BS_PRED, 64, true))) return rc;
+    if ((rc = go(gr.pred == 3 ? K_B_PRED_COLD3 : K_B_PRED_COLD, BS_PRED, 64, true))) return rc;
+    if ((rc = go(K_B_PRED_HOT, BS_PRED, 64, true))) return rc;
   } else if ((rc = go(gr.pred == 1 ? K_B_PRED_STATIC2 : K_B_PRED_STATIC, BS_PRED, 64, true))) {
     return rc;
   }
@@ -1727,8 +1745,7 @@ int pypde_get_tuning(pypde_ctx* c, int* choices, double* ms) {
     choices[0] = c->tune.weno;
     choices[1] = c->tune.flux;
     choices[2] = c->tune.update;
-    choices[3] = c->pred_split ? (c->pred_cold3 ? 3 : 2) + (c->pred_hot2 ? 2 : 0)
-                               : (c->pred_static ? 0 : 1);
+    choices[3] = c->pred_split ? (c->pred_cold3 ? 3 : 2) : (c->pred_static ? 0 : 1);
     choices[4] = c->tune.source;
   }
   if (ms)

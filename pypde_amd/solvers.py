@@ -183,7 +183,7 @@ class Solver:
         weno = ["8x32", "4x64", "16x16", "16x32"][ch[0]]
         flux = [256, 128, 64][ch[1]]
         upd = ["cell", "rows1", "rows2", "rows4", "rows8"][ch[2]]
-        pred = ["static", "queue", "split", "split3", "split_hot2", "split3_hot2"][ch[3]]
+        pred = ["static", "queue", "split", "split3"][ch[3]]
         src = ["defaults", "measured", "table", "forced"][ch[4]]
         return dict(weno_tile=weno, flux_workgroup=flux, update=upd, predictor=pred, source=src,
                     ms=dict(zip(["weno", "predictor", "flux", "update"], ms.tolist())))
