@@ -13,6 +13,11 @@ static_assert(CPW >= 1, "N^D must not exceed 64");
 #define PDE_GUESS 0
 #endif
 
+template <bool B>
+struct BConst {
+  static constexpr bool value = B;
+};
+
 DEV double tw(int i) {
   double r = 0.0;
 #pragma unroll
@@ -833,10 +838,15 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
   int learned = (PDE_GUESS && guess) ? 1 : 0;
   int fellback = 0;
   double r0 = 0.0;
-  int trip = 0;
   unsigned int n_full = 0u, n_short = 0u;  // node corrections of this lane
   bool cell_hot = false;                   // MODE 1: the cell needs the full solve
-  while (__ballot(active) != 0ull) {
+  // One iteration for every cell still active; FIRST: the first iteration
+  // from the plain guess, whose iterate is constant in time (the wave-uniform
+  // q = w), so the right-hand side needs F, S and the derivatives once.  It is
+  // peeled off the loop so that the loop holds one right-hand side only
+  // (the stiff cold pass then needs 208 registers instead of 234).
+  auto iterate = [&](auto firstc) {
+    constexpr bool FIRST = decltype(firstc)::value;
     if (active) {
 #pragma unroll
       for (int t = 0; t < N; t++)
@@ -844,12 +854,10 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
         for (int v = 0; v < V; v++) sm.q[c.slot][t][c.idx][v] = q[t][v];
     }
     __syncthreads();
-    // first trip from the plain guess: q is constant in time (wave-uniform)
-    if (trip == 0 && !(PDE_GUESS && guess))
+    if constexpr (FIRST)
       node_rhs_const(c, sm, q, w, dt, g, active, rhs, dqs);
     else
       node_rhs(c, sm, q, w, dt, g, active, rhs, dqs);
-    trip++;
     bool more = false;
     bool skip = false;
 #if PDE_STIFF
@@ -947,7 +955,10 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
       if ((mb & cellmask) == 0ull || it >= maxit) active = false;
     }
     __syncthreads();
-  }
+  };
+  if (!(PDE_GUESS && guess) && __ballot(active) != 0ull) iterate(BConst<true>{});
+  while (__ballot(active) != 0ull) iterate(BConst<false>{});
+
   // ---- epilogue: traces, volume term, optional full qh
   if constexpr (MODE == 1) {
     // hand hot cells to the hot pass: one atomic per wavefront, slots in order
