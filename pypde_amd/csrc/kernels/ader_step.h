@@ -33,47 +33,70 @@ DEV double tder(int i, int j) {
   return r;
 }
 
+// Per-node constants of the space-time system, kept in LDS (one table per
+// wavefront) rather than in registers, where they would hold some 40 per
+// lane: the DERVALS matrix, and per spatial node W[k_d], prod_d W[k_d] (WP),
+// 1 / WP and 1 / (W_t WP).
+constexpr int NT_W = 0, NT_WP = D, NT_IWP = D + 1, NT_ISC = D + 2, NT_COUNT = D + 2 + N;
+struct NodeTab {
+  double der[N][N];
+  double node[ND][NT_COUNT];
+};
+
 struct NodeCtx {
   int slot, idx;
   int kd[D];
   int base[D];
-  double der[D][N];   // DERVALS row k_d
-  double wk[D];       // W[k_d]
-  double WP, IWP;     // prod_d W[k_d], 1 / WP
-  double ct[N];       // W_t prod W (dg.cpp:105-116)
-  double isc[N];      // 1 / (W_t prod W)
-  double c0[N];       // psi_t(0) prod W (dg.cpp:156-171)
+  const NodeTab* nt;
+  DEV double der(int d, int j) const { return nt->der[kd[d]][j]; }  // DERVALS row k_d
+  DEV double wk(int d) const { return nt->node[idx][NT_W + d]; }    // W[k_d]
+  DEV double WP() const { return nt->node[idx][NT_WP]; }
+  DEV double IWP() const { return nt->node[idx][NT_IWP]; }
+  DEV double isc(int t) const { return nt->node[idx][NT_ISC + t]; }
+  // W_t prod W (dg.cpp:105-116) and psi_t(0) prod W (dg.cpp:156-171), the
+  // products taken in the order d = 0, 1
+  DEV double ct(int t) const {
+    double a = T_W[t];
+#pragma unroll
+    for (int d = 0; d < D; d++) a = a * wk(d);
+    return a;
+  }
+  DEV double c0(int t) const {
+    double b = T_END0[t];
+#pragma unroll
+    for (int d = 0; d < D; d++) b = b * wk(d);
+    return b;
+  }
 };
 
-DEV void node_ctx(NodeCtx& c, int lane) {
+// Lane indices and the LDS table (written by the first ND lanes, then a
+// workgroup barrier: every thread of the workgroup must call it).
+DEV void node_ctx(NodeCtx& c, int lane, NodeTab& nt) {
   c.slot = lane / ND;
   c.idx = lane % ND;
+  c.nt = &nt;
+  double wk[D];
 #pragma unroll
   for (int d = 0; d < D; d++) {
     int stride = ipow_c(N, D - 1 - d);
     c.kd[d] = (c.idx / stride) % N;
     c.base[d] = c.idx - c.kd[d] * stride;
-    c.wk[d] = tw(c.kd[d]);
-#pragma unroll
-    for (int j = 0; j < N; j++) c.der[d][j] = tder(c.kd[d], j);
+    wk[d] = tw(c.kd[d]);
   }
-  double p = c.wk[0];
+  if (lane < N * N) nt.der[lane / N][lane % N] = tder(lane / N, lane % N);
+  if (lane < ND) {
+    double p = wk[0];
 #pragma unroll
-  for (int d = 1; d < D; d++) p = p * c.wk[d];
-  c.WP = p;
-  c.IWP = 1.0 / p;
+    for (int d = 1; d < D; d++) p = p * wk[d];
+    double* row = nt.node[lane];
 #pragma unroll
-  for (int t = 0; t < N; t++) {
-    double a = T_W[t], b = T_END0[t];
+    for (int d = 0; d < D; d++) row[NT_W + d] = wk[d];
+    row[NT_WP] = p;
+    row[NT_IWP] = 1.0 / p;
 #pragma unroll
-    for (int d = 0; d < D; d++) {
-      a = a * c.wk[d];
-      b = b * c.wk[d];
-    }
-    c.ct[t] = a;
-    c.c0[t] = b;
-    c.isc[t] = 1.0 / (T_W[t] * p);
+    for (int t = 0; t < N; t++) row[NT_ISC + t] = 1.0 / (T_W[t] * p);
   }
+  __syncthreads();
 }
 
 // derivative along d at this lane's node of buffer buf[slot][t][node][v]
@@ -82,9 +105,9 @@ DEV void lds_deriv(const NodeCtx& c, const double (*buf)[NTB][ND][V], int t, dou
   constexpr int stride = ipow_c(N, D - 1 - d);
 #pragma unroll
   for (int v = 0; v < V; v++) {
-    double acc = c.der[d][0] * buf[c.slot][t][c.base[d]][v];
+    double acc = c.der(d, 0) * buf[c.slot][t][c.base[d]][v];
 #pragma unroll
-    for (int j = 1; j < N; j++) acc = dfma(c.der[d][j], buf[c.slot][t][c.base[d] + j * stride][v], acc);
+    for (int j = 1; j < N; j++) acc = dfma(c.der(d, j), buf[c.slot][t][c.base[d] + j * stride][v], acc);
     out[v] = acc * rdx;
   }
 }
@@ -106,7 +129,10 @@ struct PredLDS {
 #if PDE_USE_F
   double f[D][CPW][NTF][ND][V];
 #endif
-  double red[64];
+#if PDE_GUESS
+  double red[64];  // per-lane residuals of the learned-guess check
+#endif
+  NodeTab nt;
 };
 
 #if PDE_SECOND
@@ -207,8 +233,8 @@ DEV void node_rhs(const NodeCtx& c, PredLDS& sm, const double (&q)[N][V], const 
     });
 #pragma unroll
     for (int v = 0; v < V; v++) {
-      double rr = r[v] * c.ct[t];
-      rhs[t][v] = rr * dt + c.c0[t] * w[v];
+      double rr = r[v] * c.ct(t);
+      rhs[t][v] = rr * dt + c.c0(t) * w[v];
     }
   };
   if constexpr (NTF == N) {
@@ -286,8 +312,8 @@ DEV void node_rhs_const(const NodeCtx& c, PredLDS& sm, const double (&q)[N][V], 
     for (int t = 0; t < N; t++)
 #pragma unroll
       for (int v = 0; v < V; v++) {
-        double rr = r[v] * c.ct[t];
-        rhs[t][v] = rr * dt + c.c0[t] * w[v];
+        double rr = r[v] * c.ct(t);
+        rhs[t][v] = rr * dt + c.c0(t) * w[v];
       }
 #pragma unroll
     for (int t = 1; t < N; t++)
@@ -312,7 +338,7 @@ DEV bool newton_node(const NodeCtx& c, const double (&q)[N][V], double (&G)[N][V
 #pragma unroll
   for (int t = 0; t < N; t++)
 #pragma unroll
-    for (int v = 0; v < V; v++) g[t][v] = G[t][v] * c.isc[t];
+    for (int v = 0; v < V; v++) g[t][v] = G[t][v] * c.isc(t);
   double qbar[V];
 #pragma unroll
   for (int v = 0; v < V; v++) {
@@ -516,7 +542,7 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
   const double dt = st->dt;
   const int lane = threadIdx.x;
   NodeCtx c;
-  node_ctx(c, lane);
+  node_ctx(c, lane, sm.nt);
   const bool lane_ok = c.slot < CPW;
   const unsigned long long cellmask =
       (ND == 64 ? ~0ull : ((1ull << ND) - 1ull)) << (lane_ok ? c.slot * ND : 0);
@@ -582,9 +608,9 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
       for (int t = 0; t < N; t++)
 #pragma unroll
         for (int v = 0; v < V; v++) {
-          double acc = (T_DGMAT[t][0] * c.WP) * q[0][v];
+          double acc = (T_DGMAT[t][0] * c.WP()) * q[0][v];
 #pragma unroll
-          for (int s2 = 1; s2 < N; s2++) acc = dfma(T_DGMAT[t][s2] * c.WP, q[s2][v], acc);
+          for (int s2 = 1; s2 < N; s2++) acc = dfma(T_DGMAT[t][s2] * c.WP(), q[s2][v], acc);
           rhs[t][v] = rhs[t][v] - acc;
           double a = dabs(rhs[t][v]);
           rmax = (a > rmax || a != a) ? a : rmax;  // keeps a NaN
@@ -643,7 +669,7 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
           double acc = T_DGINV[t][0] * rhs[0][v];
 #pragma unroll
           for (int s2 = 1; s2 < N; s2++) acc = dfma(T_DGINV[t][s2], rhs[s2][v], acc);
-          double q1 = acc * c.IWP;
+          double q1 = acc * c.IWP();
           if (dabs(q1 - q[t][v]) > DG_TOL * (1.0 + dabs(q[t][v]))) more = true;
           q[t][v] = q1;
         }
@@ -721,7 +747,7 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
 #endif
         double tmp = dt * T_W[t];
 #pragma unroll
-        for (int d = 0; d < D; d++) tmp = tmp * c.wk[d];
+        for (int d = 0; d < D; d++) tmp = tmp * c.wk(d);
 #pragma unroll
         for (int v = 0; v < V; v++) part[v] = (t == 0) ? tmp * sv[v] : dfma(tmp, sv[v], part[v]);
       }
@@ -805,7 +831,7 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
   const double dt = st->dt;
   const int lane = threadIdx.x;
   NodeCtx c;
-  node_ctx(c, lane);
+  node_ctx(c, lane, sm.nt);
   long cell;
   bool valid;
   if constexpr (MODE == 2) {
@@ -868,9 +894,9 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
       for (int t = 0; t < N; t++)
 #pragma unroll
         for (int v = 0; v < V; v++) {
-          double acc = (T_DGMAT[t][0] * c.WP) * q[0][v];
+          double acc = (T_DGMAT[t][0] * c.WP()) * q[0][v];
 #pragma unroll
-          for (int s = 1; s < N; s++) acc = dfma(T_DGMAT[t][s] * c.WP, q[s][v], acc);
+          for (int s = 1; s < N; s++) acc = dfma(T_DGMAT[t][s] * c.WP(), q[s][v], acc);
           rhs[t][v] = rhs[t][v] - acc;
           double a = dabs(rhs[t][v]);
           rmax = (a > rmax || a != a) ? a : rmax;  // keeps a NaN
@@ -943,7 +969,7 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
           double acc = T_DGINV[t][0] * rhs[0][v];
 #pragma unroll
           for (int s = 1; s < N; s++) acc = dfma(T_DGINV[t][s], rhs[s][v], acc);
-          double q1 = acc * c.IWP;
+          double q1 = acc * c.IWP();
           if (dabs(q1 - q[t][v]) > DG_TOL * (1.0 + dabs(q[t][v]))) more = true;
           q[t][v] = q1;
         }
@@ -1033,7 +1059,7 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
 #endif
       double tmp = dt * T_W[t];
 #pragma unroll
-      for (int d = 0; d < D; d++) tmp = tmp * c.wk[d];
+      for (int d = 0; d < D; d++) tmp = tmp * c.wk(d);
 #pragma unroll
       for (int v = 0; v < V; v++) part[v] = (t == 0) ? tmp * s[v] : dfma(tmp, s[v], part[v]);
     }
@@ -1153,6 +1179,7 @@ struct PredLDS32 {
 #if PDE_USE_F
   float f[D][CPW][N][ND][V];
 #endif
+  NodeTab nt;
 };
 
 DEV void F32(float* out, const float* q, const float* dq, int d) {
@@ -1185,10 +1212,10 @@ DEV void lds_deriv32(const NodeCtx& c, const float (*buf)[N][ND][V], int t, floa
   constexpr int stride = ipow_c(N, D - 1 - d);
 #pragma unroll
   for (int v = 0; v < V; v++) {
-    float acc = (float)c.der[d][0] * buf[c.slot][t][c.base[d]][v];
+    float acc = (float)c.der(d, 0) * buf[c.slot][t][c.base[d]][v];
 #pragma unroll
     for (int j = 1; j < N; j++)
-      acc = __builtin_fmaf((float)c.der[d][j], buf[c.slot][t][c.base[d] + j * stride][v], acc);
+      acc = __builtin_fmaf((float)c.der(d, j), buf[c.slot][t][c.base[d] + j * stride][v], acc);
     out[v] = acc * rdx;
   }
 }
@@ -1240,7 +1267,7 @@ DEV void node_rhs32(const NodeCtx& c, PredLDS32& sm, const float (&q)[N][V], con
     for (int t = 0; t < N; t++)
 #pragma unroll
       for (int v = 0; v < V; v++)
-        rhs[t][v] = (r[CONST ? 0 : t][v] * (float)c.ct[t]) * dt + (float)c.c0[t] * w[v];
+        rhs[t][v] = (r[CONST ? 0 : t][v] * (float)c.ct(t)) * dt + (float)c.c0(t) * w[v];
   }
 }
 
@@ -1253,7 +1280,7 @@ k_predictor_f32(const double* __restrict__ wh, const double* __restrict__ gin,
   const double dtd = st->dt;
   const int lane = threadIdx.x;
   NodeCtx c;
-  node_ctx(c, lane);
+  node_ctx(c, lane, sm.nt);
   const long cell = (long)blockIdx.x * CPW + c.slot;
   const bool valid = (c.slot < CPW) && (cell < g.nwc);
   float rdx[D];
@@ -1290,7 +1317,7 @@ k_predictor_f32(const double* __restrict__ wh, const double* __restrict__ gin,
 #if PDE_STIFF
       // residual G = rhs - (DG_MAT (x) W) q, then the FP64 Newton correction
       double qd[N][V], G[N][V];
-      const float wp = (float)c.WP;
+      const float wp = (float)c.WP();
 #pragma unroll
       for (int t = 0; t < N; t++)
 #pragma unroll
@@ -1312,7 +1339,7 @@ k_predictor_f32(const double* __restrict__ wh, const double* __restrict__ gin,
         }
 #else
       // Picard: q1 = DG_MAT^-1 rhs / prod W
-      const float iwp = (float)c.IWP;
+      const float iwp = (float)c.IWP();
 #pragma unroll
       for (int t = 0; t < N; t++)
 #pragma unroll
