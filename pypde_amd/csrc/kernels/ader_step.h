@@ -33,10 +33,15 @@ DEV double tder(int i, int j) {
   return r;
 }
 
-// Per-node constants of the space-time system, kept in LDS (one table per
-// wavefront) rather than in registers, where they would hold some 40 per
-// lane: the DERVALS matrix, and per spatial node W[k_d], prod_d W[k_d] (WP),
-// 1 / WP and 1 / (W_t WP).
+// Per-node constants of the space-time system: the DERVALS matrix, and per
+// spatial node W[k_d], prod_d W[k_d] (WP), 1 / WP and 1 / (W_t WP).  The
+// stiff predictor keeps them in LDS (one table per wavefront) rather than in
+// registers, where they would hold some 40 per lane and cost it a wavefront
+// per SIMD; the Picard predictor, far from its register limit, keeps them in
+// registers (reading them from LDS in its short iterations costs 8 % there).
+#ifndef PDE_NODE_TAB_LDS
+#define PDE_NODE_TAB_LDS PDE_STIFF
+#endif
 constexpr int NT_W = 0, NT_WP = D, NT_IWP = D + 1, NT_ISC = D + 2, NT_COUNT = D + 2 + N;
 struct NodeTab {
   double der[N][N];
@@ -47,12 +52,21 @@ struct NodeCtx {
   int slot, idx;
   int kd[D];
   int base[D];
+#if PDE_NODE_TAB_LDS
   const NodeTab* nt;
   DEV double der(int d, int j) const { return nt->der[kd[d]][j]; }  // DERVALS row k_d
   DEV double wk(int d) const { return nt->node[idx][NT_W + d]; }    // W[k_d]
   DEV double WP() const { return nt->node[idx][NT_WP]; }
   DEV double IWP() const { return nt->node[idx][NT_IWP]; }
   DEV double isc(int t) const { return nt->node[idx][NT_ISC + t]; }
+#else
+  double der_[D][N], wk_[D], WP_, IWP_, isc_[N];
+  DEV double der(int d, int j) const { return der_[d][j]; }
+  DEV double wk(int d) const { return wk_[d]; }
+  DEV double WP() const { return WP_; }
+  DEV double IWP() const { return IWP_; }
+  DEV double isc(int t) const { return isc_[t]; }
+#endif
   // W_t prod W (dg.cpp:105-116) and psi_t(0) prod W (dg.cpp:156-171), the
   // products taken in the order d = 0, 1
   DEV double ct(int t) const {
@@ -69,12 +83,12 @@ struct NodeCtx {
   }
 };
 
-// Lane indices and the LDS table (written by the first ND lanes, then a
-// workgroup barrier: every thread of the workgroup must call it).
-DEV void node_ctx(NodeCtx& c, int lane, NodeTab& nt) {
+// Lane indices and the per-node constants; with the LDS table, the first
+// ND lanes write it and a workgroup barrier follows (every thread of the
+// workgroup must call it).
+DEV void node_ctx(NodeCtx& c, int lane, NodeTab* nt) {
   c.slot = lane / ND;
   c.idx = lane % ND;
-  c.nt = &nt;
   double wk[D];
 #pragma unroll
   for (int d = 0; d < D; d++) {
@@ -83,12 +97,14 @@ DEV void node_ctx(NodeCtx& c, int lane, NodeTab& nt) {
     c.base[d] = c.idx - c.kd[d] * stride;
     wk[d] = tw(c.kd[d]);
   }
-  if (lane < N * N) nt.der[lane / N][lane % N] = tder(lane / N, lane % N);
-  if (lane < ND) {
-    double p = wk[0];
+  double p = wk[0];
 #pragma unroll
-    for (int d = 1; d < D; d++) p = p * wk[d];
-    double* row = nt.node[lane];
+  for (int d = 1; d < D; d++) p = p * wk[d];
+#if PDE_NODE_TAB_LDS
+  c.nt = nt;
+  if (lane < N * N) nt->der[lane / N][lane % N] = tder(lane / N, lane % N);
+  if (lane < ND) {
+    double* row = nt->node[lane];
 #pragma unroll
     for (int d = 0; d < D; d++) row[NT_W + d] = wk[d];
     row[NT_WP] = p;
@@ -97,6 +113,19 @@ DEV void node_ctx(NodeCtx& c, int lane, NodeTab& nt) {
     for (int t = 0; t < N; t++) row[NT_ISC + t] = 1.0 / (T_W[t] * p);
   }
   __syncthreads();
+#else
+  (void)nt;
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    c.wk_[d] = wk[d];
+#pragma unroll
+    for (int j = 0; j < N; j++) c.der_[d][j] = tder(c.kd[d], j);
+  }
+  c.WP_ = p;
+  c.IWP_ = 1.0 / p;
+#pragma unroll
+  for (int t = 0; t < N; t++) c.isc_[t] = 1.0 / (T_W[t] * p);
+#endif
 }
 
 // derivative along d at this lane's node of buffer buf[slot][t][node][v]
@@ -132,8 +161,15 @@ struct PredLDS {
 #if PDE_GUESS
   double red[64];  // per-lane residuals of the learned-guess check
 #endif
+#if PDE_NODE_TAB_LDS
   NodeTab nt;
+#endif
 };
+#if PDE_NODE_TAB_LDS
+#define PRED_NODE_TAB(sm) (&(sm).nt)
+#else
+#define PRED_NODE_TAB(sm) ((NodeTab*)nullptr)
+#endif
 
 #if PDE_SECOND
 // Gradient traces for the viscous flux (fv.cpp:87-122): the derivatives of
@@ -542,7 +578,7 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
   const double dt = st->dt;
   const int lane = threadIdx.x;
   NodeCtx c;
-  node_ctx(c, lane, sm.nt);
+  node_ctx(c, lane, PRED_NODE_TAB(sm));
   const bool lane_ok = c.slot < CPW;
   const unsigned long long cellmask =
       (ND == 64 ? ~0ull : ((1ull << ND) - 1ull)) << (lane_ok ? c.slot * ND : 0);
@@ -831,7 +867,7 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
   const double dt = st->dt;
   const int lane = threadIdx.x;
   NodeCtx c;
-  node_ctx(c, lane, sm.nt);
+  node_ctx(c, lane, PRED_NODE_TAB(sm));
   long cell;
   bool valid;
   if constexpr (MODE == 2) {
@@ -1179,7 +1215,9 @@ struct PredLDS32 {
 #if PDE_USE_F
   float f[D][CPW][N][ND][V];
 #endif
+#if PDE_NODE_TAB_LDS
   NodeTab nt;
+#endif
 };
 
 DEV void F32(float* out, const float* q, const float* dq, int d) {
@@ -1280,7 +1318,7 @@ k_predictor_f32(const double* __restrict__ wh, const double* __restrict__ gin,
   const double dtd = st->dt;
   const int lane = threadIdx.x;
   NodeCtx c;
-  node_ctx(c, lane, sm.nt);
+  node_ctx(c, lane, PRED_NODE_TAB(sm));
   const long cell = (long)blockIdx.x * CPW + c.slot;
   const bool valid = (c.slot < CPW) && (cell < g.nwc);
   float rdx[D];
