@@ -38,9 +38,11 @@ DEV double tder(int i, int j) {
 // stiff predictor keeps them in LDS (one table per wavefront) rather than in
 // registers, where they would hold some 40 per lane and cost it a wavefront
 // per SIMD; the Picard predictor, far from its register limit, keeps them in
-// registers (reading them from LDS in its short iterations costs 8 % there).
+// registers (reading them from LDS in its short iterations costs 8 % there),
+// and so does the learned-guess variant, which stays at one wavefront per
+// SIMD either way (the LDS reads cost it 6 %).
 #ifndef PDE_NODE_TAB_LDS
-#define PDE_NODE_TAB_LDS PDE_STIFF
+#define PDE_NODE_TAB_LDS (PDE_STIFF && !PDE_GUESS)
 #endif
 constexpr int NT_W = 0, NT_WP = D, NT_IWP = D + 1, NT_ISC = D + 2, NT_COUNT = D + 2 + N;
 struct NodeTab {
