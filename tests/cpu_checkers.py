@@ -204,7 +204,7 @@ def oracle_solve(sysname, u0, tf, L, bc, N, stiff=False, cfl=0.9, ndt=1, max_ste
     return u, ret.reshape((ndt,) + u.shape), st, rc
 
 
-def ref_solve(sysname, u0, tf, L, bc, N, stiff=False, cfl=0.9, ndt=1, nthreads=1):
+def ref_solve(sysname, u0, tf, L, bc, N, stiff=False, cfl=0.9, ndt=1, nthreads=1, flux=0):
     """Run the reference's pde_solver (stdout of its step loop suppressed)."""
     s = system(sysname)
     u = np.ascontiguousarray(u0, dtype=np.float64).copy()
@@ -215,7 +215,7 @@ def ref_solve(sysname, u0, tf, L, bc, N, stiff=False, cfl=0.9, ndt=1, nthreads=1
     ret = np.zeros(ndt * u.size)
     with quiet_stdout():
         ref().pde_solver(s.F, s.B, s.S, s.F is not None, s.B is not None, s.S is not None,
-                         dp(u), tf, ip(nX), ndim, dp(dX), cfl, ip(bcs), bool(stiff), 0, N, V,
+                         dp(u), tf, ip(nX), ndim, dp(dX), cfl, ip(bcs), bool(stiff), int(flux), N, V,
                          ndt, False, dp(ret), nthreads)
     return u, ret.reshape((ndt,) + u.shape)
 
