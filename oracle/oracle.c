@@ -374,11 +374,11 @@ static double house(const double *x, int m, double *v) {
     for (int i = 1; i < m; i++) v[i] = 0.0;
     return 0.0;
   }
-  double mu = sqrt(x[0] * x[0] + sigma);
+  double mu = sqrt(fma(x[0], x[0], sigma));
   double v0 = (x[0] <= 0.0) ? x[0] - mu : -sigma / (x[0] + mu);
   v[0] = v0;
   for (int i = 1; i < m; i++) v[i] = x[i];
-  return 2.0 / (v0 * v0 + sigma);
+  return 2.0 / fma(v0, v0, sigma);
 }
 
 /* rows r0..r0+m-1, columns c0..c1: H = P H */
@@ -415,7 +415,7 @@ static void hess_charpoly(const double *H, int n, double *c) {
     double hkk = H[(k - 1) * n + (k - 1)];
     for (int i = 0; i <= k; i++) {
       double v = (i >= 1) ? D[k - 1][i - 1] : 0.0;
-      if (i <= k - 1) v = v - hkk * D[k - 1][i];
+      if (i <= k - 1) v = fma(-hkk, D[k - 1][i], v);
       D[k][i] = v;
     }
     double prod = 1.0;
@@ -423,44 +423,54 @@ static void hess_charpoly(const double *H, int n, double *c) {
       /* prod = h_{k,k-1} h_{k-1,k-2} ... h_{k-m+1,k-m} (1-based) */
       prod = prod * H[(k - m) * n + (k - m - 1)];
       double t = H[(k - m - 1) * n + (k - 1)] * prod;
-      for (int i = 0; i <= k - m - 1; i++) D[k][i] = D[k][i] - t * D[k - m - 1][i];
+      for (int i = 0; i <= k - m - 1; i++) D[k][i] = fma(-t, D[k - m - 1][i], D[k][i]);
     }
   }
   for (int i = 0; i <= n; i++) c[i] = D[n][i];
 }
 
-/* One real root of the monic degree-n polynomial c (n >= 2) by Laguerre's
- * method from x0 (cubic convergence at simple roots; from above the largest
- * real root of a real-rooted polynomial it decreases monotonically onto it).
- * The step n / (G +- sqrt((n-1)(n H - G^2))), G = p'/p, H = G^2 - p''/p, is
- * evaluated multiplied through by p: n p / (p' +- sqrt((n-1)((n-1) p'^2 -
- * n p p''))), with f = p''/2 from the Horner recurrence, whose first two
- * steps are written out (exact for c[n] = 1 and finite x).  Converged:
- * |step| <= 4 eps |x|, or |step| <= 1e-8 |x| with |step| |x| <= |previous
- * step|^2 (superlinear convergence seen: the remaining error is below
- * |step|^2.5).  *dp receives p' where p was last evaluated.  Returns 0
- * when it does not converge in LAG_IT steps. */
+/* One real root of the monic degree-n polynomial c (n >= 2) from x0, by
+ * Halley's method for n <= 5 and by Laguerre's method above (both converge
+ * cubically at simple roots and, from above the largest real root of a
+ * real-rooted polynomial, decrease monotonically onto it; Halley's step needs
+ * no square root, Laguerre's lands near the roots at once from a start far
+ * outside them).  Halley: p p' / (p'^2 - p p''/2).  Laguerre: n / (G +-
+ * sqrt((n-1)(n H - G^2))), G = p'/p, H = G^2 - p''/p, evaluated multiplied
+ * through by p: n p / (p' +- sqrt((n-1)((n-1) p'^2 - n p p''))).  f = p''/2
+ * comes from the fused Horner recurrence, whose first two steps are written
+ * out (exact for c[n] = 1 and finite x).  Converged: |step| <= 4 eps |x|, or
+ * |step| <= 1e-8 |x| with |step| |x| <= |previous step|^2 (superlinear
+ * convergence seen: the remaining error is below |step|^2.5).  *dp receives
+ * p' where p was last evaluated.  Returns 0 when it does not converge in
+ * LAG_IT steps. */
 #define LAG_IT 40
-static int laguerre_real(const double *c, int n, double x0, double *root, double *dp) {
+static int real_root(const double *c, int n, double x0, double *root, double *dp) {
   double x = x0, prev = 0.0;
   for (int it = 0; it < LAG_IT; it++) {
     double b1 = x + c[n - 1];
-    double f = 1.0, d = x + b1, b = x * b1 + c[n - 2];
+    double f = 1.0, d = x + b1, b = fma(x, b1, c[n - 2]);
     for (int j = n - 3; j >= 0; j--) {
-      f = x * f + d;
-      d = x * d + b;
-      b = x * b + c[j];
+      f = fma(x, f, d);
+      d = fma(x, d, b);
+      b = fma(x, b, c[j]);
     }
     *dp = d;
     if (b == 0.0) {
       *root = x;
       return 1;
     }
-    double r = (double)(n - 1) * ((double)(n - 1) * (d * d) - (double)(2 * n) * (f * b));
-    double sq = r > 0.0 ? sqrt(r) : 0.0;
-    double den = (d >= 0.0) ? d + sq : d - sq;
+    double num, den;
+    if (n <= 5) {
+      num = b * d;
+      den = fma(d, d, -(b * f));
+    } else {
+      double r = (double)(n - 1) * ((double)(n - 1) * (d * d) - (double)(2 * n) * (f * b));
+      double sq = r > 0.0 ? sqrt(r) : 0.0;
+      num = (double)n * b;
+      den = (d >= 0.0) ? d + sq : d - sq;
+    }
     if (!(den != 0.0)) return 0;
-    double step = ((double)n * b) / den;
+    double step = num / den;
     x = x - step;
     double as = fabs(step), ax = fabs(x);
     if (!(as > 4.0 * DBL_EPSILON * ax) || as <= DBL_MIN ||
@@ -475,9 +485,9 @@ static int laguerre_real(const double *c, int n, double x0, double *root, double
 
 /* max |lambda| of an upper Hessenberg H (n >= 3, entries scaled to <= 1) from
  * its characteristic polynomial: the largest and smallest real roots by
- * Laguerre, checked for accuracy against the rounding error of evaluating
- * p, then a certificate that every other root is smaller in modulus -- the remaining factor q = p / ((x - l1)(x - l2)) is
- * centred on the mean of its roots, alpha, and its roots bounded by
+ * real_root, checked for accuracy against the rounding error of evaluating
+ * p, then a certificate that every other root is smaller in modulus -- the
+ * remaining factor q = p / ((x - l1)(x - l2)) is centred on the mean of its roots, alpha, and its roots bounded by
  * |z - alpha| <= R, R = 2^(E+1) >= 2 max_k |t_k|^(1/(m-k)) (Fujiwara's bound
  * on the roots of t(y) = q(y + alpha), in powers of two: |t_k| < 2^e_k gives
  * |t_k|^(1/j) < 2^ceil(e_k/j)).  Returns 0 (caller falls back to QR) when a
@@ -492,8 +502,9 @@ static int charpoly_max_abs(const double *H, int n, double *out) {
    * = sum a_i^2, at or above the largest a_i^2 -- or +-||H||_inf when sigma^2
    * is not positive.  Any start is safe: the certificate decides. */
   double xhi, xlo;
-  double mu = -(c[n - 1] / (double)n);
-  double s2 = (c[n - 1] * c[n - 1] - 2.0 * c[n - 2]) / (double)n - mu * mu;
+  double inv_n = 1.0 / (double)n;
+  double mu = -(c[n - 1] * inv_n);
+  double s2 = fma(-mu, mu, fma(c[n - 1], c[n - 1], -2.0 * c[n - 2]) * inv_n);
   if (s2 > 0.0 && s2 <= 1e300) {
     double w = 1.001 * sqrt(s2 * (0.5 * (double)n));
     xhi = mu + w;
@@ -509,7 +520,7 @@ static int charpoly_max_abs(const double *H, int n, double *out) {
     xlo = -U;
   }
   double l1, l2, d1, d2;
-  if (!laguerre_real(c, n, xhi, &l1, &d1) || !laguerre_real(c, n, xlo, &l2, &d2)) return 0;
+  if (!real_root(c, n, xhi, &l1, &d1) || !real_root(c, n, xlo, &l2, &d2)) return 0;
   double M = fmax(fabs(l1), fabs(l2));
   /* two distinct roots: from both ends the iterations can reach the one real
    * root, and deflating p by it twice would leave no certificate */
@@ -520,8 +531,8 @@ static int charpoly_max_abs(const double *H, int n, double *out) {
   {
     double a1 = fabs(l1), a2 = fabs(l2), E1 = 1.0, E2 = 1.0;
     for (int j = n - 1; j >= 0; j--) {
-      E1 = a1 * E1 + fabs(c[j]);
-      E2 = a2 * E2 + fabs(c[j]);
+      E1 = fma(a1, E1, fabs(c[j]));
+      E2 = fma(a2, E2, fabs(c[j]));
     }
     if (!((double)(2 * n) * DBL_EPSILON * E1 <= 1e-11 * M * fabs(d1))) return 0;
     if (!((double)(2 * n) * DBL_EPSILON * E2 <= 1e-11 * M * fabs(d2))) return 0;
@@ -533,16 +544,16 @@ static int charpoly_max_abs(const double *H, int n, double *out) {
   for (int k = 0; k < 2; k++) {
     double nq[MAXV + 1];
     nq[deg - 1] = q[deg];
-    for (int j = deg - 2; j >= 0; j--) nq[j] = q[j + 1] + rr[k] * nq[j + 1];
+    for (int j = deg - 2; j >= 0; j--) nq[j] = fma(rr[k], nq[j + 1], q[j + 1]);
     deg -= 1;
     for (int j = 0; j <= deg; j++) q[j] = nq[j];
   }
   int m = deg;
-  double alpha = -(q[m - 1] / (double)m);
+  double alpha = -(q[m - 1] * (1.0 / (double)m));
   /* Taylor shift t(y) = q(y + alpha) */
   for (int i = 0; i <= m; i++) t[i] = q[i];
   for (int k = 0; k < m; k++)
-    for (int j = m - 1; j >= k; j--) t[j] = t[j] + alpha * t[j + 1];
+    for (int j = m - 1; j >= k; j--) t[j] = fma(alpha, t[j + 1], t[j]);
   double tsum = 0.0;
   for (int k = 0; k < m; k++) tsum = tsum + fabs(t[k]);
   if (!(tsum <= DBL_MAX)) return 0;

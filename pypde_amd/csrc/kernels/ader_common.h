@@ -150,12 +150,12 @@ DEV double house(const double* x, double* v) {
     for (int i = 1; i < M; i++) v[i] = 0.0;
     return 0.0;
   }
-  double mu = dsqrt(x[0] * x[0] + sigma);
+  double mu = dsqrt(dfma(x[0], x[0], sigma));
   double v0 = (x[0] <= 0.0) ? x[0] - mu : -sigma / (x[0] + mu);
   v[0] = v0;
 #pragma unroll
   for (int i = 1; i < M; i++) v[i] = x[i];
-  return 2.0 / (v0 * v0 + sigma);
+  return 2.0 / dfma(v0, v0, sigma);
 }
 
 // H = P H on rows r0..r0+M-1 for the columns j with lo_c <= j <= hi_c.
@@ -215,7 +215,7 @@ DEV void hess_charpoly(const double* H, double* c) {
 #pragma unroll
     for (int i = 0; i <= k; i++) {
       double v = (i >= 1) ? D[k - 1][i - 1] : 0.0;
-      if (i <= k - 1) v = v - hkk * D[k - 1][i];
+      if (i <= k - 1) v = dfma(-hkk, D[k - 1][i], v);
       D[k][i] = v;
     }
     double prod = 1.0;
@@ -224,30 +224,37 @@ DEV void hess_charpoly(const double* H, double* c) {
       prod = prod * H[(k - m) * V + (k - m - 1)];
       const double t = H[(k - m - 1) * V + (k - 1)] * prod;
 #pragma unroll
-      for (int i = 0; i <= k - m - 1; i++) D[k][i] = D[k][i] - t * D[k - m - 1][i];
+      for (int i = 0; i <= k - m - 1; i++) D[k][i] = dfma(-t, D[k - m - 1][i], D[k][i]);
     }
   }
 #pragma unroll
   for (int i = 0; i <= V; i++) c[i] = D[V][i];
 }
 
-// K real roots of monic degree-V polynomials by Laguerre's method (cubic
-// convergence at simple roots; from above the largest real root of a
-// real-rooted polynomial it decreases monotonically onto it), root k of c[k]
-// from x0[k].  The K iterations are independent and advance in one loop;
-// each stops at its own convergence exactly as it would alone.  The step
-// n / (G +- sqrt((n-1)(n H - G^2))), G = p'/p, H = G^2 - p''/p, is evaluated
-// multiplied through by p: n p / (p' +- sqrt((n-1)((n-1) p'^2 - n p p''))),
-// one division.  The Horner recurrence for p, p' and p''/2 starts after its
-// first two steps, which are exact for the monic c (c[V] = 1, finite x).
-// Converged: |step| <= 4 eps |x|, or |step| <= 1e-8 |x| with |step| |x| <=
-// |previous step|^2 (superlinear convergence seen, so the remaining error
-// is below |step|^2.5).  ok[k] is false when root k does not converge in
-// 40 steps; dp[k] is p' where p was last evaluated (at the root, or one step
-// of at most 1e-8 |root| before it).
+// K real roots of monic degree-V polynomials, root k of c[k] from x0[k]: by
+// Halley's method for V <= 5, by Laguerre's method above.  Both converge
+// cubically at simple roots and, from above the largest real root of a
+// real-rooted polynomial, decrease monotonically onto it (Halley's step
+// p p' / (p'^2 - p p''/2) = 2G / (G^2 + H), G = p'/p = sum 1/(x - r_i), H =
+// G^2 - p''/p = sum 1/(x - r_i)^2, stays below the distance to the nearest
+// root).  Halley's step needs no square root; from a start far outside the
+// spectrum it moves only about 2/(V+1) of the way per step, where Laguerre's
+// step lands near the roots at once, which is why the larger systems keep
+// Laguerre.  The Laguerre step n / (G +- sqrt((n-1)(n H - G^2))) is evaluated
+// multiplied through by p: n p / (p' +- sqrt((n-1)((n-1) p'^2 - n p p''))).
+// The K iterations are independent and advance in one loop; each
+// stops at its own convergence exactly as it would alone.  The fused Horner
+// recurrence for p, p' and p''/2 starts after its first two steps, which are
+// exact for the monic c (c[V] = 1, finite x).  Converged: |step| <= 4 eps
+// |x|, or |step| <= 1e-8 |x| with |step| |x| <= |previous step|^2
+// (superlinear convergence seen, so the remaining error is below
+// |step|^2.5).  ok[k] is false when root k does not converge in 40 steps;
+// dp[k] is p' where p was last evaluated (at the root, or one step of at most
+// 1e-8 |root| before it).
+constexpr bool ROOT_HALLEY = V <= 5;
 template <int K>
-DEV void laguerre_multi(const double (&c)[K][V + 1], const double (&x0)[K], double (&root)[K],
-                        double (&dp)[K], bool (&ok)[K]) {
+DEV void real_roots(const double (&c)[K][V + 1], const double (&x0)[K], double (&root)[K],
+                    double (&dp)[K], bool (&ok)[K]) {
   double x[K], prev[K];
   int state[K];  // 0 running, 1 converged, 2 failed
 #pragma unroll
@@ -267,17 +274,25 @@ DEV void laguerre_multi(const double (&c)[K][V + 1], const double (&x0)[K], doub
     for (int k = 0; k < K; k++) {
       const double xk = x[k];
       const double b1 = xk + c[k][V - 1];
-      double f = 1.0, d = xk + b1, b = xk * b1 + c[k][V - 2];
+      double f = 1.0, d = xk + b1, b = dfma(xk, b1, c[k][V - 2]);
 #pragma unroll
       for (int j = V - 3; j >= 0; j--) {
-        f = xk * f + d;
-        d = xk * d + b;
-        b = xk * b + c[k][j];
+        f = dfma(xk, f, d);
+        d = dfma(xk, d, b);
+        b = dfma(xk, b, c[k][j]);
       }
-      const double r = (double)(V - 1) * ((double)(V - 1) * (d * d) - (double)(2 * V) * (f * b));
-      const double sq = r > 0.0 ? dsqrt(r) : 0.0;
-      const double den = (d >= 0.0) ? d + sq : d - sq;
-      const double step = ((double)V * b) / den;
+      double num, den;
+      if constexpr (ROOT_HALLEY) {
+        num = b * d;
+        den = dfma(d, d, -(b * f));
+      } else {
+        const double r =
+            (double)(V - 1) * ((double)(V - 1) * (d * d) - (double)(2 * V) * (f * b));
+        const double sq = r > 0.0 ? dsqrt(r) : 0.0;
+        num = (double)V * b;
+        den = (d >= 0.0) ? d + sq : d - sq;
+      }
+      const double step = num / den;
       const double xn = xk - step;
       const double as = dabs(step), ax = dabs(xn);
       const bool conv = !(as > 4.0 * 2.220446049250313e-16 * ax) ||
@@ -302,7 +317,7 @@ DEV void laguerre_multi(const double (&c)[K][V + 1], const double (&x0)[K], doub
 }
 
 // Characteristic polynomial c of the scaled upper Hessenberg H and the
-// starting points of the Laguerre iterations for its largest and smallest
+// starting points of the root iterations for its largest and smallest
 // real roots: mu +- 1.001 sigma sqrt(V / 2), mu and sigma^2 the mean and
 // variance of the roots from c[V-1] and c[V-2] -- for a spectrum of pairs
 // u +- a_i around roots at u (the waves of a hyperbolic system) sigma^2 V / 2
@@ -310,8 +325,9 @@ DEV void laguerre_multi(const double (&c)[K][V + 1], const double (&x0)[K], doub
 // is not positive.  Any start is safe: the certificate decides.
 DEV void charpoly_setup(const double* H, double* c, double* xhi, double* xlo) {
   hess_charpoly(H, c);
-  const double mu = -(c[V - 1] / (double)V);
-  const double s2 = (c[V - 1] * c[V - 1] - 2.0 * c[V - 2]) / (double)V - mu * mu;
+  constexpr double inv_v = 1.0 / (double)V;
+  const double mu = -(c[V - 1] * inv_v);
+  const double s2 = dfma(-mu, mu, dfma(c[V - 1], c[V - 1], -2.0 * c[V - 2]) * inv_v);
   if (s2 > 0.0 && s2 <= 1e300) {
     const double w = 1.001 * dsqrt(s2 * (0.5 * (double)V));
     *xhi = mu + w;
@@ -353,8 +369,8 @@ DEV bool charpoly_certify(const double* c, double l1, double l2, double d1, doub
     double E1 = 1.0, E2 = 1.0;
 #pragma unroll
     for (int j = V - 1; j >= 0; j--) {
-      E1 = a1 * E1 + dabs(c[j]);
-      E2 = a2 * E2 + dabs(c[j]);
+      E1 = dfma(a1, E1, dabs(c[j]));
+      E2 = dfma(a2, E2, dabs(c[j]));
     }
     if (!((double)(2 * V) * 2.220446049250313e-16 * E1 <= 1e-11 * M * dabs(d1))) return false;
     if (!((double)(2 * V) * 2.220446049250313e-16 * E2 <= 1e-11 * M * dabs(d2))) return false;
@@ -363,18 +379,19 @@ DEV bool charpoly_certify(const double* c, double l1, double l2, double d1, doub
   double q1[V], q[V - 1];
   q1[V - 1] = c[V];
 #pragma unroll
-  for (int j = V - 2; j >= 0; j--) q1[j] = c[j + 1] + l1 * q1[j + 1];
+  for (int j = V - 2; j >= 0; j--) q1[j] = dfma(l1, q1[j + 1], c[j + 1]);
   q[V - 2] = q1[V - 1];
 #pragma unroll
-  for (int j = V - 3; j >= 0; j--) q[j] = q1[j + 1] + l2 * q[j + 1];
-  const double alpha = -(q[m - 1] / (double)m);
+  for (int j = V - 3; j >= 0; j--) q[j] = dfma(l2, q[j + 1], q1[j + 1]);
+  constexpr double inv_m = 1.0 / (double)m;
+  const double alpha = -(q[m - 1] * inv_m);
   double t[m + 1];
 #pragma unroll
   for (int i = 0; i <= m; i++) t[i] = q[i];
 #pragma unroll
   for (int k = 0; k < m; k++)
 #pragma unroll
-    for (int j = m - 1; j >= k; j--) t[j] = t[j] + alpha * t[j + 1];
+    for (int j = m - 1; j >= k; j--) t[j] = dfma(alpha, t[j + 1], t[j]);
   double tsum = 0.0;
 #pragma unroll
   for (int k = 0; k < m; k++) tsum = tsum + dabs(t[k]);
@@ -585,7 +602,7 @@ DEV double max_abs_eig(double* H) {
     charpoly_setup(H, c[0], &x0[0], &x0[1]);
 #pragma unroll
     for (int j = 0; j <= V; j++) c[1][j] = c[0][j];
-    laguerre_multi<2>(c, x0, root, dp, ok);
+    real_roots<2>(c, x0, root, dp, ok);
     if (ok[0] && ok[1] && charpoly_certify(c[0], root[0], root[1], dp[0], dp[1], &r))
       return __builtin_ldexp(r, e2);
   }

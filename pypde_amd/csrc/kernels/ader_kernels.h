@@ -22,7 +22,20 @@ struct Geom {
   double dX[2];
   double refl[2][17];  // reflective-wall multipliers per axis (V <= 17)
   int ndt;
+  // Flux-item index arithmetic without run-time division: n / nf[d] and n /
+  // (faces per row along d: nx[1] for d = 0, nx[1] + 1 for d = 1) as
+  // (n * mul) >> sh, exact for n < 2^31 (udiv); idx32 is 1 when every flux
+  // item index and every w-cell index is below 2^31.
+  unsigned fmul[2], fsh[2], rmul[2], rsh[2];
+  int idx32;
 };
+
+// n / d for n < 2^31 from d's multiplier m = ceil(2^sh / d), sh = 31 +
+// ceil(log2 d): with n = q d + r, n m / 2^sh = q + (r + n e / 2^sh) / d, e =
+// m d - 2^sh in [0, d), and n e < 2^31 d <= 2^sh keeps the bracket below d.
+DEV unsigned udiv(unsigned n, unsigned m, unsigned sh) {
+  return (unsigned)(((unsigned long long)n * m) >> sh);
+}
 
 struct StepState {
   double t, dt, tf, cfl;

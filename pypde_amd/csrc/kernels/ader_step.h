@@ -1453,13 +1453,37 @@ DEV void flux_dir(const double* __restrict__ tr, double* __restrict__ ff, double
                   const StepState* st, const Geom& g, const double* __restrict__ trd,
                   const Blk& b) {
   const long nf = g.nf[d];
-  const long item = (long)b.x * BS + threadIdx.x;
-  if (item >= nf * N * ND1) return;
-  const long f = item % nf;
-  const int j = (int)((item / nf) % ND1);
-  const int t = (int)(item / (nf * ND1));
-  long wl, wr;
-  face_cells<d>(g, f, &wl, &wr);
+  long f, wl, wr;
+  int j, t;
+  if (g.idx32) {
+    // 32-bit indices, divisions by nf and by the faces per row as
+    // multiply-shifts (nf N ND1 and nwc below 2^31)
+    const unsigned item = b.x * (unsigned)BS + threadIdx.x;
+    if (item >= (unsigned)nf * (unsigned)(N * ND1)) return;
+    const unsigned rest = udiv(item, g.fmul[d], g.fsh[d]);
+    const unsigned f32 = item - rest * (unsigned)nf;
+    j = (int)(rest % ND1);
+    t = (int)(rest / ND1);
+    f = f32;
+    if constexpr (D == 1) {
+      wl = f32;
+      wr = f32 + 1;
+    } else {
+      const unsigned row = d == 0 ? (unsigned)g.nx[1] : (unsigned)g.nx[1] + 1;
+      const unsigned f0 = udiv(f32, g.rmul[d], g.rsh[d]), f1 = f32 - f0 * row;
+      const unsigned nw1 = (unsigned)g.nw[1];
+      const unsigned l = d == 0 ? f0 * nw1 + (f1 + 1) : (f0 + 1) * nw1 + f1;
+      wl = l;
+      wr = d == 0 ? l + nw1 : l + 1;
+    }
+  } else {
+    const long item = (long)b.x * BS + threadIdx.x;
+    if (item >= nf * N * ND1) return;
+    f = item % nf;
+    j = (int)((item / nf) % ND1);
+    t = (int)(item / (nf * ND1));
+    face_cells<d>(g, f, &wl, &wr);
+  }
   double qL[V], qR[V];
 #pragma unroll
   for (int v = 0; v < V; v++) {

@@ -67,7 +67,18 @@ struct Geom {
   double dX[2];
   double refl[2][MAXV];
   int ndt;
+  unsigned fmul[2], fsh[2], rmul[2], rsh[2];
+  int idx32;
 };
+
+// Multiplier and shift of the device's udiv for a divisor 1 <= d < 2^31:
+// sh = 31 + ceil(log2 d), mul = ceil(2^sh / d) (below 2^32).
+static void udiv_magic(long d, unsigned* mul, unsigned* sh) {
+  int L = 0;
+  while ((1L << L) < d) L++;
+  *sh = (unsigned)(31 + L);
+  *mul = (unsigned)(((1ULL << (31 + L)) + (unsigned long long)d - 1) / (unsigned long long)d);
+}
 
 struct StepState {
   double t, dt, tf, cfl;
@@ -808,6 +819,17 @@ static pypde_ctx* create_ctx(const char* src, bool useF, bool useB, bool useS, c
     g.nf[d] = 1;
     for (int e = 0; e < ndim; e++) g.nf[d] *= (e == d) ? nX[e] + 1 : nX[e];
   }
+  g.idx32 = g.nwc < (1L << 31);
+  for (int d = 0; d < 2; d++) {
+    long nf = d < ndim ? g.nf[d] : 1;
+    udiv_magic(nf, &g.fmul[d], &g.fsh[d]);
+    udiv_magic(d == 0 ? g.nx[1] : g.nx[1] + 1, &g.rmul[d], &g.rsh[d]);
+    if (nf * N * (c->ND / N) >= (1L << 31)) g.idx32 = 0;
+  }
+  // PYPDE_FLUX_IDX64=1: the flux kernels' 64-bit index path on any grid (for
+  // testing the path that grids of 2^31 flux items or w-cells take)
+  if (const char* e = getenv("PYPDE_FLUX_IDX64"))
+    if (e[0] == '1') g.idx32 = 0;
   auto bail = [&](int code_) {
     rc = code_;
     pypde_destroy(c);

@@ -70,6 +70,31 @@ def test_no_cpu_fallback_without_gpu():
                    ndt=1, stiff=False)
 
 
+def test_flux_index_multiply_shift_is_exact():
+    """The flux kernel divides item indices n < 2^31 by nf and by the faces
+    per row as (n * m) >> sh, m = ceil(2^sh / d), sh = 31 + ceil(log2 d).
+    Exhaustive over small divisors and n, and at the extremes of n and d
+    (powers of two and their neighbours, where the error term m d - 2^sh is
+    largest relative to d)."""
+    def magic(d):
+        L = (d - 1).bit_length()
+        sh = 31 + L
+        return -(-(1 << sh) // d), sh
+
+    ds = list(range(1, 600)) + [(1 << k) + o for k in range(10, 31) for o in (-1, 0, 1)
+                                if 1 <= (1 << k) + o < (1 << 31)] + [(1 << 31) - 1]
+    rng = np.random.default_rng(7)
+    top = np.uint64((1 << 31) - 1)
+    for d in ds:
+        m, sh = magic(d)
+        assert m < (1 << 32)
+        n = np.concatenate([np.arange(0, 3000, dtype=np.uint64),
+                            top - np.arange(0, 3000, dtype=np.uint64),
+                            rng.integers(0, 1 << 31, 3000, dtype=np.uint64)])
+        q = (n * np.uint64(m)) >> np.uint64(sh)
+        assert np.array_equal(q, n // np.uint64(d)), d
+
+
 # ------------------------------------------------------------ GPU parity
 
 def oracle_run(name, u0, tf, L, bc, N, stiff, **kw):
@@ -148,6 +173,28 @@ def test_mixed_boundaries_bitwise_equal_to_oracle(bcs, N):
     print(f"mixed bcs {bcs} N={N}: steps {sg['steps']} oracle {st.steps}")
     assert rc == 0 and sg["rc"] == 0
     assert sg["steps"] == st.steps and sg["pred_iters"] == st.pred_iters
+    assert np.array_equal(ug, uo)
+
+
+@pytest.mark.gpu
+def test_flux_64bit_index_path_bitwise_equal_to_oracle(monkeypatch):
+    """The flux kernels' 64-bit index path (taken when a grid's flux items or
+    w-cells reach 2^31; forced here by PYPDE_FLUX_IDX64=1 on small grids) gives
+    the oracle's results bitwise: 2D Euler with periodic and reflective axes,
+    and 1D Sod."""
+    monkeypatch.setenv("PYPDE_FLUX_IDX64", "1")
+    u0 = systems.euler2d_modes(40, seed=5)[:, :24].copy()
+    L = [1.0, 24 / 40]
+    uo, ro, st, rc = oracle_run("euler2d", u0, 0.04, L, [1, 2], 3, False,
+                                reflect=EULER2D_REFLECT.ravel())
+    ug, rg, sg = gpu_run("euler2d", u0, 0.04, L, ["periodic", "reflective"], 3, False,
+                         reflect=EULER2D_REFLECT)
+    assert rc == 0 and sg["rc"] == 0 and sg["steps"] == st.steps
+    assert np.array_equal(ug, uo)
+    u0 = systems.sod1d(133)
+    uo, ro, st, rc = oracle_run("euler1d", u0, 0.05, [1.0], 0, 3, False)
+    ug, rg, sg = gpu_run("euler1d", u0, 0.05, [1.0], "transitive", 3, False)
+    assert rc == 0 and sg["steps"] == st.steps
     assert np.array_equal(ug, uo)
 
 
