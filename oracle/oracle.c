@@ -381,6 +381,31 @@ static double house(const double *x, int m, double *v) {
   return 2.0 / fma(v0, v0, sigma);
 }
 
+/* The same reflector with one division, for the Hessenberg reduction of a
+ * matrix scaled to entries <= 1: v^T v = 2 mu (mu - x0), so beta = 1 / (mu (mu
+ * - x0)) when x0 <= 0 (v0 = x0 - mu); when x0 > 0 (v0 = -sigma / (x0 + mu))
+ * beta = (x0 + mu) / (mu sigma), both quotients from r = 1 / ((x0 + mu) mu
+ * sigma); *mu_out = mu (P x = mu e_1).  A tail below 2^-500 in square norm
+ * counts as reduced, which keeps the product above 2^-1000. */
+#define HOUSE1_TINY 0x1p-500
+static double house1(const double *x, int m, double *v, double *mu_out) {
+  double sigma = 0.0;
+  for (int i = 1; i < m; i++) sigma = fma(x[i], x[i], sigma);
+  if (sigma <= HOUSE1_TINY) {
+    v[0] = 1.0;
+    for (int i = 1; i < m; i++) v[i] = 0.0;
+    return 0.0;
+  }
+  double mu = sqrt(fma(x[0], x[0], sigma));
+  *mu_out = mu;
+  int pos = x[0] > 0.0;
+  double s = x[0] + mu;
+  double r = 1.0 / (pos ? (s * mu) * sigma : mu * (mu - x[0]));
+  v[0] = pos ? -sigma * ((r * mu) * sigma) : x[0] - mu;
+  for (int i = 1; i < m; i++) v[i] = x[i];
+  return pos ? s * (r * s) : r;
+}
+
 /* rows r0..r0+m-1, columns c0..c1: H = P H */
 static void house_left(double *H, int n, const double *v, int m, double beta,
                        int r0, int c0, int c1) {
@@ -599,9 +624,13 @@ double oracle_max_abs_eig(const double *A, int n) {
     int m = n - k - 1;
     double vv[MAXV];
     for (int i = 0; i < m; i++) x[i] = H[(k + 1 + i) * n + k];
-    double beta = house(x, m, vv);
+    double mu = 0.0;
+    double beta = house1(x, m, vv, &mu);
     if (beta != 0.0) {
-      house_left(H, n, vv, m, beta, k + 1, k, n - 1);
+      /* column k is x: P x = mu e_1, set rather than computed */
+      house_left(H, n, vv, m, beta, k + 1, k + 1, n - 1);
+      H[(k + 1) * n + k] = mu;
+      for (int i = k + 2; i < n; i++) H[i * n + k] = 0.0;
       house_right(H, n, vv, m, beta, 0, n - 1, k + 1);
     }
   }
@@ -718,10 +747,31 @@ double oracle_max_abs_eig(const double *A, int n) {
   return ldexp(best, e2);
 }
 
+/* 1 / h_i for the forward-difference steps h_i = max(sqrt(eps)|q_i|,
+ * sqrt(eps)) = 2^-26 m_i, m_i = max(|q_i|, 1) (exact: sqrt(eps) = 2^-26), with
+ * one division for all of them: 1 / h_i = (2^26 / prod m) (prod_{j<i} m_j
+ * prod_{j>i} m_j).  When prod m passes 2^1000 each is divided on its own. */
+static void fd_recips(const double *q, int V, double *rh) {
+  double m[MAXV], pre[MAXV + 1], suf[MAXV + 1];
+  pre[0] = 1.0;
+  for (int i = 0; i < V; i++) {
+    m[i] = fmax(fabs(q[i]), 1.0);
+    pre[i + 1] = pre[i] * m[i];
+  }
+  if (!(pre[V] <= 0x1p1000)) {
+    for (int i = 0; i < V; i++) rh[i] = 1.0 / fmax(FD_EPS * fabs(q[i]), FD_EPS);
+    return;
+  }
+  suf[V] = 1.0;
+  for (int i = V - 1; i >= 0; i--) suf[i] = suf[i + 1] * m[i];
+  double r = 0x1p26 / pre[V];
+  for (int i = 0; i < V; i++) rh[i] = r * (pre[i] * suf[i + 1]);
+}
+
 /* A = dF/dQ by forward differences + B (system.cpp:6-26, NumericalDiff.h:6-44):
  * h_i = max(sqrt(eps)|q_i|, sqrt(eps)), column i = (F(q + h_i e_i) - F(q)) (1 / h_i)
- * (one division per column; the reference divides every entry, which differs
- * by at most one rounding).
+ * with the reciprocals from fd_recips (the reference divides every entry by
+ * h_i, which differs by a few roundings).
  * f0 (optional) receives F(q). */
 static void system_matrix(oracle_F F, oracle_B B, const double *q,
                           const double *dq, int d, int V, double *A,
@@ -730,15 +780,16 @@ static void system_matrix(oracle_F F, oracle_B B, const double *q,
   memcpy(qq, q, V * sizeof(double));
   memcpy(dqq, dq, MAXD * V * sizeof(double));
   if (F) {
+    double rh[MAXV];
     F(v1, qq, dqq, d);
+    fd_recips(qq, V, rh);
     for (int i = 0; i < V; i++) {
       double qi0 = qq[i];
       double h = fmax(FD_EPS * fabs(qi0), FD_EPS);
       qq[i] = qi0 + h;
       F(v2, qq, dqq, d);
       qq[i] = qi0;
-      double rh = 1.0 / h;
-      for (int r = 0; r < V; r++) A[r * V + i] = (v2[r] - v1[r]) * rh;
+      for (int r = 0; r < V; r++) A[r * V + i] = (v2[r] - v1[r]) * rh[i];
     }
     if (f0) memcpy(f0, v1, V * sizeof(double));
   } else {

@@ -158,6 +158,36 @@ DEV double house(const double* x, double* v) {
   return 2.0 / dfma(v0, v0, sigma);
 }
 
+// The same reflector with one division: v^T v = 2 mu (mu - x0), so beta =
+// 1 / (mu (mu - x0)) when x0 <= 0 (v0 = x0 - mu); when x0 > 0 (v0 = -sigma /
+// (x0 + mu)) beta = (x0 + mu) / (mu sigma), and both quotients come from r =
+// 1 / ((x0 + mu) mu sigma); *mu_out = mu (P x = mu e_1).  For the
+// Hessenberg reduction of a matrix scaled to entries <= 1 (every |x_i| <= V):
+// a tail below 2^-500 in square norm counts as reduced, which keeps the
+// product above 2^-1000.
+constexpr double HOUSE1_TINY = 0x1p-500;
+template <int M>
+DEV double house1(const double* x, double* v, double* mu_out) {
+  double sigma = 0.0;
+#pragma unroll
+  for (int i = 1; i < M; i++) sigma = dfma(x[i], x[i], sigma);
+  if (sigma <= HOUSE1_TINY) {
+    v[0] = 1.0;
+#pragma unroll
+    for (int i = 1; i < M; i++) v[i] = 0.0;
+    return 0.0;
+  }
+  const double mu = dsqrt(dfma(x[0], x[0], sigma));
+  *mu_out = mu;
+  const bool pos = x[0] > 0.0;
+  const double s = x[0] + mu;
+  const double r = 1.0 / (pos ? (s * mu) * sigma : mu * (mu - x[0]));
+  v[0] = pos ? -sigma * ((r * mu) * sigma) : x[0] - mu;
+#pragma unroll
+  for (int i = 1; i < M; i++) v[i] = x[i];
+  return pos ? s * (r * s) : r;
+}
+
 // H = P H on rows r0..r0+M-1 for the columns j with lo_c <= j <= hi_c.
 template <int M, int R0>
 DEV void house_left(double* H, const double* v, double beta, int lo_c, int hi_c) {
@@ -437,9 +467,14 @@ DEV bool eig_prepare(double* H, int* e2out, double* early) {
     double x[m], vv[m];
 #pragma unroll
     for (int i = 0; i < m; i++) x[i] = H[(k + 1 + i) * V + k];
-    double beta = house<m>(x, vv);
+    double mu = 0.0;
+    double beta = house1<m>(x, vv, &mu);
     if (beta != 0.0) {
-      house_left<m, k + 1>(H, vv, beta, k, V - 1);
+      // column k is x: P x = mu e_1, set rather than computed
+      house_left<m, k + 1>(H, vv, beta, k + 1, V - 1);
+      H[(k + 1) * V + k] = mu;
+#pragma unroll
+      for (int i = k + 2; i < V; i++) H[i * V + k] = 0.0;
       house_right<m, k + 1>(H, vv, beta, 0, V - 1);
     }
   });
@@ -620,16 +655,42 @@ DEV void callF(double* out, const double* q, const double* dq, int d) {
 #endif
 }
 
+// 1 / h_i for the forward-difference steps h_i = max(FD_EPS |q_i|, FD_EPS) =
+// 2^-26 m_i, m_i = max(|q_i|, 1) (exact: FD_EPS = 2^-26) with one division
+// for all of them: 1 / h_i = (2^26 / prod m) (prod_{j<i} m_j prod_{j>i} m_j).
+// When prod m passes 2^1000 each is divided on its own.
+DEV void fd_recips(const double* q, double* rh) {
+  double m[V], pre[V + 1], suf[V + 1];
+  pre[0] = 1.0;
+#pragma unroll
+  for (int i = 0; i < V; i++) {
+    m[i] = dmax(dabs(q[i]), 1.0);
+    pre[i + 1] = pre[i] * m[i];
+  }
+  if (!(pre[V] <= 0x1p1000)) {
+#pragma unroll
+    for (int i = 0; i < V; i++) rh[i] = 1.0 / dmax(FD_EPS * dabs(q[i]), FD_EPS);
+    return;
+  }
+  suf[V] = 1.0;
+#pragma unroll
+  for (int i = V - 1; i >= 0; i--) suf[i] = suf[i + 1] * m[i];
+  const double r = 0x1p26 / pre[V];
+#pragma unroll
+  for (int i = 0; i < V; i++) rh[i] = r * (pre[i] * suf[i + 1]);
+}
+
 // A = dF/dQ by forward differences + B(q, d) (system.cpp:6-26,
-// NumericalDiff.h:6-44), column i scaled by 1 / h_i (one division per column);
-// f0 receives F(q).
+// NumericalDiff.h:6-44), column i scaled by 1 / h_i (fd_recips; the reference
+// divides every entry by h_i); f0 receives F(q).
 DEV void system_matrix(const double* q_in, const double* dq, int d, double* A, double* f0) {
   double q[V];
 #pragma unroll
   for (int i = 0; i < V; i++) q[i] = q_in[i];
 #if PDE_USE_F
-  double v1[V], v2[V];
+  double v1[V], v2[V], rh[V];
   F(v1, q, dq, d);
+  fd_recips(q, rh);
 #pragma unroll
   for (int i = 0; i < V; i++) {
     double qi0 = q[i];
@@ -637,9 +698,8 @@ DEV void system_matrix(const double* q_in, const double* dq, int d, double* A, d
     q[i] = qi0 + h;
     F(v2, q, dq, d);
     q[i] = qi0;
-    double rh = 1.0 / h;
 #pragma unroll
-    for (int r = 0; r < V; r++) A[r * V + i] = (v2[r] - v1[r]) * rh;
+    for (int r = 0; r < V; r++) A[r * V + i] = (v2[r] - v1[r]) * rh[i];
   }
 #pragma unroll
   for (int i = 0; i < V; i++) f0[i] = v1[i];
