@@ -789,10 +789,9 @@ static pypde_ctx* create_ctx(const char* src, bool useF, bool useB, bool useS, c
   c->NT = c->ND * N;
   c->CPW = 64 / c->ND;
   c->device = opts.device;
-  // Predictor schedule: static (one wavefront per group of CPW cells) unless
-  // PYPDE_PRED_SCHEDULE=queue asks for the persistent work-queued kernel.  On
-  // the 1024 x 256 reactive channel (order 3, 3.76 Newton iterations per cell)
-  // the static kernel takes 1.50 ms per step and the work queue 2.62 ms.
+  // Predictor schedule: the autotuner picks among the static one (one
+  // wavefront per group of CPW cells), the persistent work-queued kernel and
+  // the stiff cold/hot split; PYPDE_PRED_SCHEDULE fixes one instead.
   c->pred_static = true;
   if (const char* e = getenv("PYPDE_PRED_SCHEDULE")) {
     c->pred_static = strcmp(e, "queue") != 0;
