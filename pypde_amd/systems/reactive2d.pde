@@ -5,7 +5,7 @@
  * uses discrete-ignition kinetics in 1D; this system keeps its structure
  * (F advects every variable, S acts on rho*lambda only) with Arrhenius rates.
  * M is the analytic Jacobian dS/dQ used by the stiff Newton predictor.
- * 1 / rho is formed once per state and multiplied in. */
+ * F and M form 1 / rho once per state and multiply it in; S needs none. */
 #define PDE_HAS_F 1
 #define PDE_HAS_S 1
 #define PDE_HAS_M 1
@@ -27,7 +27,6 @@ PDE_FN double reactive_pressure_ri(const double *Q, double ri) {
   double ke = 0.5 * ((ru * ru + rv * rv) * ri);
   return (PDE_GAMMA - 1.0) * (E - ke - PDE_QC * rl);
 }
-PDE_FN double reactive_pressure(const double *Q) { return reactive_pressure_ri(Q, 1.0 / Q[0]); }
 PDE_FN void F(double *out, const double *Q, const double *dQ, int d) {
   double ru = Q[1], rv = Q[2], E = Q[3], rl = Q[4];
   double ri = 1.0 / Q[0];
@@ -39,13 +38,19 @@ PDE_FN void F(double *out, const double *Q, const double *dQ, int d) {
   out[3] = (E + p) * vn;
   out[4] = rl * vn;
 }
+/* S from X = rho p / (g-1) = rho (E - Qc rho lambda) - |rho v|^2 / 2, which
+ * needs no 1 / rho: p > 0 is X > 0 (rho > 0), and the Arrhenius exponent
+ * -Ea rho / p = -Ea rho^2 / ((g-1) X) takes one division. */
 PDE_FN void S(double *out, const double *Q) {
-  double p = reactive_pressure(Q);
+  double r = Q[0], ru = Q[1], rv = Q[2], E = Q[3], rl = Q[4];
+  double X = r * (E - PDE_QC * rl) - 0.5 * (ru * ru + rv * rv);
   out[0] = 0.0;
   out[1] = 0.0;
   out[2] = 0.0;
   out[3] = 0.0;
-  out[4] = (p > 0.0) ? -PDE_K * Q[4] * pde_exp(-PDE_EA * Q[0] / p) : 0.0;
+  out[4] = (X > 0.0 && r > 0.0)
+               ? -PDE_K * rl * pde_exp(-PDE_EA * (r * r) / ((PDE_GAMMA - 1.0) * X))
+               : 0.0;
 }
 PDE_FN void M(double *out, const double *Q) {
   for (int i = 0; i < 25; i++) out[i] = 0.0;
