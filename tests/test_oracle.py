@@ -173,6 +173,52 @@ def test_max_abs_eig_reference():
             assert abs(a - b) <= 1e-13 * b
 
 
+@pytest.mark.parametrize("name,V", [("euler2d", 4), ("reactive2d", 5)])
+def test_system_matrix_against_reference_formula(name, V):
+    """The finite-difference Jacobian (system.cpp:6-26, NumericalDiff.h:6-44:
+    column i = (F(q + h_i e_i) - F(q)) / h_i, h_i = max(sqrt(eps)|q_i|,
+    sqrt(eps))) from the reference's formula in Python against the oracle's,
+    which multiplies by reciprocals of the h_i formed with one division for all
+    columns, or one division each when the product of max(|q_i|, 1) passes
+    2^1000 (states of size 1e80 here): entrywise within a few ulps."""
+    s = cc.system(name)
+    s.lib.sys_F.argtypes = [cc.PD, cc.PD, cc.PD, C.c_int]
+    s.lib.sys_F.restype = None
+    eps = np.sqrt(np.finfo(float).eps)
+    rng = np.random.default_rng(9)
+    dq = np.zeros(2 * V)
+
+    def F(q, d):
+        out = np.zeros(V)
+        s.lib.sys_F(cc.dp(out), cc.dp(np.ascontiguousarray(q)), cc.dp(dq), d)
+        return out
+
+    states = []
+    for _ in range(200):
+        q = rng.uniform(0.5, 2.0, V) * rng.choice([1e-3, 1.0, 10.0, 1e3])
+        q[V - 1 if name == "euler2d" else 3] += 50.0 * q[0]   # energy: positive pressure
+        states.append(q)
+    for _ in range(20):
+        states.append(rng.uniform(0.5, 2.0, V) * 1e80)        # product > 2^1000
+    fallback = 0
+    for q in states:
+        fallback += np.sum(np.log2(np.maximum(np.abs(q), 1.0))) > 1000
+        for d in (0, 1):
+            want = np.zeros((V, V))
+            f0 = F(q, d)
+            for i in range(V):
+                h = max(eps * abs(q[i]), eps)
+                q2 = q.copy()
+                q2[i] = q[i] + h
+                want[:, i] = (F(q2, d) - f0) / h
+            got = np.zeros(V * V)
+            cc.oracle().oracle_system_matrix(s.F, None, cc.dp(np.ascontiguousarray(q)),
+                                             cc.dp(dq), d, 2, V, cc.dp(got))
+            got = got.reshape(V, V)
+            assert np.all(np.abs(got - want) <= 8 * np.finfo(float).eps * np.abs(want)), (q, d)
+    assert fallback == 20
+
+
 def test_exp_within_one_ulp():
     rng = np.random.default_rng(2)
     xs = np.concatenate([rng.uniform(-700, 700, 20000), rng.uniform(-3, 3, 20000),
