@@ -446,7 +446,7 @@ def main():
         print(json.dumps({"calibrate_copy_bytes": n * 8, "launches": 3}))
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
+        print("bench.py measures one GPU: there is no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     K, W = args.steps, args.warmup

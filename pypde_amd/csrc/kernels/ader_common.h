@@ -114,8 +114,9 @@ DEV void weno_line(Get get, double* out) {
   for (int k = 0; k < N; k++) out[k] = num[k] / den;
 }
 
-// Ghost index map (boundaries.cpp:7-23); code 2 = reflective wall (this
-// build's addition).  Returns the interior index, sets *flip for mirrors.
+// Ghost index map (boundaries.cpp:7-23); code 2 = reflective wall (an
+// addition to the reference's codes).  Returns the interior index, sets
+// *flip for mirrors.
 DEV int ghost_src(int i, int type, int n, bool* flip) {
   int r = i - N;
   *flip = false;

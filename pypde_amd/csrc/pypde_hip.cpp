@@ -187,7 +187,8 @@ std::map<uint64_t, std::vector<char>> g_codes;
 // the source text, the compile options and the hipRTC version (in memory, then
 // <cache_dir>/<key>.hsaco, written to a temporary name and renamed into place).
 // PYPDE_HIP_RECOMPILE=1 compiles even when a cached code object exists (the
-// build's compile check); the result still replaces the cache entry.
+// compile check of `python -m pypde_amd.build`); the result still replaces the
+// cache entry.
 int rtc_compile(const std::string& src, const char* what, std::vector<char>* code,
                 uint64_t* key_out = nullptr) {
   std::string key = src;

@@ -2,7 +2,8 @@
 
 The kernels issue the oracle's floating-point operations in the same order, so
 on identical inputs the expected difference is zero; every test states the
-tolerance it enforces (north star: max relative L-inf <= 1e-10 for FP64).
+tolerance it enforces (at most a relative L-inf of 1e-10 for FP64, where not
+bitwise).
 CPU-only tests here check the C ABI itself (exports, tables, errors)."""
 import ctypes as C
 import os
@@ -160,7 +161,7 @@ EULER2D_REFLECT = np.array([[1, -1, 1, 1], [1, 1, -1, 1]], dtype=np.float64)
 @pytest.mark.parametrize("bcs,N", [((1, 2), 3), ((0, 1), 2), ((2, 0), 4)])
 def test_mixed_boundaries_bitwise_equal_to_oracle(bcs, N):
     """One boundary type per axis (boundaries.cpp:7-53 takes them per axis;
-    reflective is this build's addition): 2D Euler on a 40 x 24 window,
+    reflective is an addition to the reference's codes): 2D Euler on a 40 x 24 window,
     reflective walls flip the wall-normal momentum.  Tolerance: bitwise, with
     equal step and Picard iteration counts."""
     u0 = systems.euler2d_modes(40, seed=5)[:, :24].copy()
