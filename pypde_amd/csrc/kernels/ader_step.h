@@ -130,6 +130,13 @@ DEV void node_ctx(NodeCtx& c, int lane, NodeTab* nt) {
 #endif
 }
 
+// Whether a predictor term reads the iterate's spatial gradient: a viscous F
+// (second order) or B.  A first-order F gets dQ = 0, as the face fluxes pass
+// zero face gradients: in the reference a first-order F has no gradient
+// argument (cfuncs.py:11-27 choose its arity), so none can read it; the
+// iterate then need not be staged in LDS every iteration.
+constexpr bool PRED_DQ = PDE_SECOND || PDE_USE_B;
+
 // derivative along d at this lane's node of buffer buf[slot][t][node][v]
 template <int d, int NTB>
 DEV void lds_deriv(const NodeCtx& c, const double (*buf)[NTB][ND][V], int t, double rdx, double* out) {
@@ -225,7 +232,8 @@ DEV void grad_traces(const NodeCtx& c, PredLDS& sm, const Geom& g, long cell, bo
 
 // rhs of the space-time system at this lane's node for all t (dg.cpp:50-126):
 // r = W_t prod W (S(q) - sum_d (B_d dq_d + d_d F_d)),  rhs = r dt + Ww.
-// Assumes sm.q holds the current iterate; leaves the lane's dq in dqs.  With
+// Assumes sm.q holds the current iterate when PRED_DQ; leaves the lane's dq
+// in dqs (zero when no term reads it).  With
 // NTF = 1, F of one time node at a time passes through LDS (two barriers per
 // time node); every (t, v) result is the same either way.
 DEV void node_rhs(const NodeCtx& c, PredLDS& sm, const double (&q)[N][V], const double* w,
@@ -234,7 +242,12 @@ DEV void node_rhs(const NodeCtx& c, PredLDS& sm, const double (&q)[N][V], const 
   auto eval_f = [&](int t, int slot_t) {
     static_for<0, D>([&](auto dc) {
       constexpr int d = decltype(dc)::value;
-      lds_deriv<d>(c, sm.q, t, 1.0 / g.dX[d], dqs[t][d]);
+      if constexpr (PRED_DQ) {
+        lds_deriv<d>(c, sm.q, t, 1.0 / g.dX[d], dqs[t][d]);
+      } else {
+#pragma unroll
+        for (int v = 0; v < V; v++) dqs[t][d][v] = 0.0;
+      }
     });
 #if PDE_USE_F
 #pragma unroll
@@ -312,7 +325,12 @@ DEV void node_rhs_const(const NodeCtx& c, PredLDS& sm, const double (&q)[N][V], 
   if (active) {
     static_for<0, D>([&](auto dc) {
       constexpr int d = decltype(dc)::value;
-      lds_deriv<d>(c, sm.q, 0, 1.0 / g.dX[d], dqs[0][d]);
+      if constexpr (PRED_DQ) {
+        lds_deriv<d>(c, sm.q, 0, 1.0 / g.dX[d], dqs[0][d]);
+      } else {
+#pragma unroll
+        for (int v = 0; v < V; v++) dqs[0][d][v] = 0.0;
+      }
     });
 #if PDE_USE_F
 #pragma unroll
@@ -628,7 +646,7 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
   }
 
   while (__ballot(active) != 0ull) {
-    if (active) {
+    if (PRED_DQ && active) {
 #pragma unroll
       for (int t = 0; t < N; t++)
 #pragma unroll
@@ -911,7 +929,7 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
   // (the stiff cold pass then needs 208 registers instead of 234).
   auto iterate = [&](auto firstc) {
     constexpr bool FIRST = decltype(firstc)::value;
-    if (active) {
+    if (PRED_DQ && active) {
 #pragma unroll
       for (int t = 0; t < N; t++)
 #pragma unroll
