@@ -9,6 +9,18 @@
 constexpr int CPW = 64 / ND;
 static_assert(CPW >= 1, "N^D must not exceed 64");
 
+// Every predictor workgroup is one wavefront (64 lanes: __launch_bounds__(64)
+// on each predictor kernel, 64-thread blocks at every launch).  The LDS
+// operations of one wavefront are performed in issue order, so exchanging
+// values between its lanes through LDS needs only that the compiler keep the
+// accesses in program order across this point; a workgroup barrier would also
+// wait for every outstanding global load and store of the wavefront.
+DEV void pred_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 #ifndef PDE_GUESS
 #define PDE_GUESS 0
 #endif
@@ -114,7 +126,7 @@ DEV void node_ctx(NodeCtx& c, int lane, NodeTab* nt) {
 #pragma unroll
     for (int t = 0; t < N; t++) row[NT_ISC + t] = 1.0 / (T_W[t] * p);
   }
-  __syncthreads();
+  pred_sync();
 #else
   (void)nt;
 #pragma unroll
@@ -201,7 +213,7 @@ DEV void grad_traces(const NodeCtx& c, PredLDS& sm, const Geom& g, long cell, bo
       }
     });
   }
-  __syncthreads();
+  pred_sync();
   if (on) {
     static_for<0, D>([&](auto dc) {
       constexpr int d = decltype(dc)::value;
@@ -226,7 +238,7 @@ DEV void grad_traces(const NodeCtx& c, PredLDS& sm, const Geom& g, long cell, bo
       }
     });
   }
-  __syncthreads();
+  pred_sync();
 }
 #endif
 
@@ -294,7 +306,7 @@ DEV void node_rhs(const NodeCtx& c, PredLDS& sm, const double (&q)[N][V], const 
       for (int t = 0; t < N; t++) eval_f(t, t);
     }
 #if PDE_USE_F
-    __syncthreads();
+    pred_sync();
 #endif
     if (active) {
 #pragma unroll
@@ -305,11 +317,11 @@ DEV void node_rhs(const NodeCtx& c, PredLDS& sm, const double (&q)[N][V], const 
     for (int t = 0; t < N; t++) {
       if (active) eval_f(t, 0);
 #if PDE_USE_F
-      __syncthreads();
+      pred_sync();
 #endif
       if (active) finish(t, 0);
 #if PDE_USE_F
-      if (t + 1 < N) __syncthreads();
+      if (t + 1 < N) pred_sync();
 #endif
     }
   }
@@ -343,7 +355,7 @@ DEV void node_rhs_const(const NodeCtx& c, PredLDS& sm, const double (&q)[N][V], 
 #endif
   }
 #if PDE_USE_F
-  __syncthreads();
+  pred_sync();
 #endif
   if (active) {
     double r[V];
@@ -652,7 +664,7 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
 #pragma unroll
         for (int v = 0; v < V; v++) sm.q[c.slot][t][c.idx][v] = q[t][v];
     }
-    __syncthreads();
+    pred_sync();
     node_rhs(c, sm, q, w, dt, g, active, rhs, dqs);
     bool more = false;
     bool skip = false;
@@ -676,9 +688,9 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
     // while a cell is still on its learned guess, a residual that is not finite
     // or larger than at the previous iterate sends it back to the plain guess
     if (__ballot(active && learned) != 0ull) {
-      __syncthreads();
+      pred_sync();
       sm.red[lane] = rmax;
-      __syncthreads();
+      pred_sync();
       double cm = 0.0;
 #pragma unroll
       for (int i = 0; i < ND; i++) {
@@ -740,7 +752,7 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
         fin = true;
       }
     }
-    __syncthreads();
+    pred_sync();
     if (__ballot(fin) == 0ull) continue;
 
     // ---- finished slots: traces, volume term, optional full qh
@@ -750,7 +762,7 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
 #pragma unroll
         for (int v = 0; v < V; v++) sm.q[c.slot][t][c.idx][v] = q[t][v];
     }
-    __syncthreads();
+    pred_sync();
     if (fin) {
       static_for<0, D>([&](auto dc) {
         constexpr int d = decltype(dc)::value;
@@ -808,12 +820,12 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
         for (int v = 0; v < V; v++) part[v] = (t == 0) ? tmp * sv[v] : dfma(tmp, sv[v], part[v]);
       }
     }
-    __syncthreads();
+    pred_sync();
     if (fin) {
 #pragma unroll
       for (int v = 0; v < V; v++) sm.q[c.slot][0][c.idx][v] = part[v];
     }
-    __syncthreads();
+    pred_sync();
     if (fin && c.idx == 0) {
 #pragma unroll
       for (int v = 0; v < V; v++) {
@@ -823,7 +835,7 @@ k_predictor(const double* __restrict__ wh, const double* __restrict__ guess,
         vol[(long)v * g.nwc + cell] = acc;
       }
     }
-    __syncthreads();
+    pred_sync();
 #endif
     if (fin && c.idx == 0) {
       s_iters += (unsigned long long)it;
@@ -935,7 +947,7 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
 #pragma unroll
         for (int v = 0; v < V; v++) sm.q[c.slot][t][c.idx][v] = q[t][v];
     }
-    __syncthreads();
+    pred_sync();
     if constexpr (FIRST)
       node_rhs_const(c, sm, q, w, dt, g, active, rhs, dqs);
     else
@@ -962,9 +974,9 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
     // while a cell is still on its learned guess, a residual that is not finite
     // or larger than at the previous iterate sends it back to the plain guess
     if (__ballot(active && learned) != 0ull) {
-      __syncthreads();
+      pred_sync();
       sm.red[lane] = rmax;
-      __syncthreads();
+      pred_sync();
       double cm = 0.0;
 #pragma unroll
       for (int i = 0; i < ND; i++) {
@@ -1036,7 +1048,7 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
       it++;
       if ((mb & cellmask) == 0ull || it >= maxit) active = false;
     }
-    __syncthreads();
+    pred_sync();
   };
   if (!(PDE_GUESS && guess) && __ballot(active) != 0ull) iterate(BConst<true>{});
   while (__ballot(active) != 0ull) iterate(BConst<false>{});
@@ -1058,7 +1070,7 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
 #pragma unroll
       for (int v = 0; v < V; v++) sm.q[c.slot][t][c.idx][v] = q[t][v];
   }
-  __syncthreads();
+  pred_sync();
   if (done_here) {
     // traces with every lane busy: the N ND1 = ND lanes of a cell take one
     // (time node t, transverse node j) each and write both ends e of every
@@ -1120,12 +1132,12 @@ DEV void predictor_static_body(PredLDS& sm, const double* __restrict__ wh,
       for (int v = 0; v < V; v++) part[v] = (t == 0) ? tmp * s[v] : dfma(tmp, s[v], part[v]);
     }
   }
-  __syncthreads();
+  pred_sync();
   if (done_here) {
 #pragma unroll
     for (int v = 0; v < V; v++) sm.q[c.slot][0][c.idx][v] = part[v];
   }
-  __syncthreads();
+  pred_sync();
   // the cell's lane idx sums variable v = idx (+ ND, ...) over the nodes in order
   if (done_here) {
 #pragma unroll
@@ -1304,7 +1316,7 @@ DEV void node_rhs32(const NodeCtx& c, PredLDS32& sm, const float (&q)[N][V], con
     }
   }
 #if PDE_USE_F
-  __syncthreads();
+  pred_sync();
 #endif
   if (active) {
     float r[NTE][V];
@@ -1365,7 +1377,7 @@ k_predictor_f32(const double* __restrict__ wh, const double* __restrict__ gin,
 #pragma unroll
         for (int v = 0; v < V; v++) sm.q[c.slot][t][c.idx][v] = q[t][v];
     }
-    __syncthreads();
+    pred_sync();
     if (it == 0 && !gin)
       node_rhs32<true>(c, sm, q, w, dt, rdx, active, rhs);
     else
@@ -1417,7 +1429,7 @@ k_predictor_f32(const double* __restrict__ wh, const double* __restrict__ gin,
       it++;
       if ((mb & cellmask) == 0ull || it >= maxit32) active = false;
     }
-    __syncthreads();
+    pred_sync();
   }
   // hand over: the FP32 iterate, or the plain guess where it is not finite
   bool bad = false;
