@@ -372,6 +372,7 @@ struct pypde_ctx {
   hipModule_t mod_guess = nullptr;  // learned-guess variant of the predictor
   hipFunction_t fn_plain[K_COUNT] = {}, fn_guess[K_COUNT] = {};
   bool use_model = false;
+  int dt_lanes = 1;  // k_dt threads per w-cell (the kernels' DT_LANES)
   MLPW model;
   double* gbuf = nullptr;   // per-step MLP guesses
   void* mlp_pack = nullptr;  // the attached model's BF16 weight fragments
@@ -523,7 +524,8 @@ int enqueue_step(pypde_ctx* c, bool guess_now, std::vector<hipEvent_t>* evs) {
     StepState* st = c->st;
     std::vector<void*> a = {&w, &st, &g};
     // its last workgroup also turns the wave-speed maximum into dt
-    if ((rc = launch(c, K_DT, dim3((unsigned)cdiv(g.nwc, 256)), dim3(256), a))) return rc;
+    if ((rc = launch(c, K_DT, dim3((unsigned)cdiv(g.nwc * c->dt_lanes, 256)), dim3(256), a)))
+      return rc;
   }
   mark();
   if (c->use_model) {
@@ -789,6 +791,7 @@ static pypde_ctx* create_ctx(const char* src, bool useF, bool useB, bool useS, c
   for (int d = 0; d < ndim; d++) c->ND *= N;
   c->NT = c->ND * N;
   c->CPW = 64 / c->ND;
+  c->dt_lanes = (ndim == 2 && V <= 5 && !secondOrder) ? 2 : 1;  // DT_PAIR in ader_kernels.h
   c->device = opts.device;
   // Predictor schedule: the autotuner picks among the static one (one
   // wavefront per group of CPW cells), the persistent work-queued kernel and
@@ -1350,7 +1353,8 @@ static int batch_setup(pypde_ensemble* e, long cap) {
     for (int s = 0; s < BS_COUNT; s++) {
       unsigned long total = 0;
       for (int i = 0; i < gr.count; i++) {
-        const Geom& g = e->m[order[gr.first + i]]->g;
+        const pypde_ctx* mc = e->m[order[gr.first + i]];
+        const Geom& g = mc->g;
         long gx = 1, gy = 1, gz = 1;
         switch (s) {
           case BS_WENO:
@@ -1362,7 +1366,7 @@ static int batch_setup(pypde_ensemble* e, long cap) {
               gz = V;
             }
             break;
-          case BS_DT: gx = cdiv(g.nwc, 256); break;
+          case BS_DT: gx = cdiv(g.nwc * mc->dt_lanes, 256); break;
           case BS_PRED: gx = cdiv(g.nwc, CPW); break;
           case BS_FLUX0:
           case BS_FLUX1: {
