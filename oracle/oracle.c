@@ -617,8 +617,13 @@ double oracle_max_abs_eig(const double *A, int n) {
   if (!(scale >= DBL_MIN)) return (scale != scale) ? scale : 0.0;
   int e2;
   (void)frexp(scale, &e2);                 /* scale = m 2^e2, m in [0.5, 1) */
+  /* up to n = 5, with the largest |entry| within 2^+-64, the matrix stays
+   * unscaled: the reduction and the polynomial path then stay in the normal
+   * range and commute exactly with scaling by a power of two; the QR
+   * fallback, whose shifts are absolute, is scaled */
+  int scaled = n > 5 || e2 < -64 || e2 > 64;
   double pw = ldexp(1.0, -e2);
-  for (int i = 0; i < n * n; i++) H[i] = A[i] * pw;
+  for (int i = 0; i < n * n; i++) H[i] = scaled ? A[i] * pw : A[i];
   double v[3], x[MAXV];
   for (int k = 0; k + 2 < n; k++) {
     int m = n - k - 1;
@@ -636,8 +641,10 @@ double oracle_max_abs_eig(const double *A, int n) {
   }
   {
     double r;
-    if (n >= 3 && charpoly_max_abs(H, n, &r)) return ldexp(r, e2);
+    if (n >= 3 && charpoly_max_abs(H, n, &r)) return scaled ? ldexp(r, e2) : r;
   }
+  if (!scaled)
+    for (int i = 0; i < n * n; i++) H[i] = H[i] * pw;
   double anorm = 0.0;
   for (int i = 0; i < n * n; i++) anorm = anorm + fabs(H[i]);
   double best = 0.0, ex = 0.0;

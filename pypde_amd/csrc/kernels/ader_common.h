@@ -446,9 +446,10 @@ DEV bool charpoly_certify(const double* c, double l1, double l2, double d1, doub
 #endif
 
 // Scale A (destroyed) by the power of two 2^-e2 at or above its largest
-// |entry| and reduce it to upper Hessenberg form in place.  False when A is
-// zero, subnormal or NaN; *early is then max |lambda|.
-DEV bool eig_prepare(double* H, int* e2out, double* early) {
+// |entry| above V = 5 or when that lies outside 2^+-64 (*scaled), and reduce
+// it to upper Hessenberg form in place.  False when A is zero, subnormal or
+// NaN; *early is then max |lambda|.
+DEV bool eig_prepare(double* H, int* e2out, bool* scaled, double* early) {
   double scale = 0.0;
 #pragma unroll
   for (int i = 0; i < V * V; i++) scale = dmax(scale, dabs(H[i]));
@@ -459,9 +460,18 @@ DEV bool eig_prepare(double* H, int* e2out, double* early) {
   int e2;
   (void)__builtin_frexp(scale, &e2);  // scale = m 2^e2, m in [0.5, 1)
   *e2out = e2;
-  const double pw = __builtin_ldexp(1.0, -e2);
+  // Up to V = 5, with the largest |entry| within 2^+-64, the matrix is left
+  // unscaled: every operation of the reduction and of the polynomial path
+  // then stays in the normal range (products of degree up to 2 V - 1 < 16)
+  // and commutes exactly with scaling by a power of two, so only the QR
+  // fallback, whose shifts are absolute, takes the scaled matrix (max_abs_eig
+  // scales it).
+  *scaled = V > 5 || e2 < -64 || e2 > 64;
+  if (*scaled) {
+    const double pw = __builtin_ldexp(1.0, -e2);
 #pragma unroll
-  for (int i = 0; i < V * V; i++) H[i] = H[i] * pw;
+    for (int i = 0; i < V * V; i++) H[i] = H[i] * pw;
+  }
   static_for<0, (V >= 3 ? V - 2 : 0)>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
     constexpr int m = V - k - 1;
@@ -629,8 +639,9 @@ DEV double eig_qr(double* H) {
 // indices are compile-time constants; run-time bounds only predicate.
 DEV double max_abs_eig(double* H) {
   int e2 = 0;
+  bool scaled = true;
   double early;
-  if (!eig_prepare(H, &e2, &early)) return early;
+  if (!eig_prepare(H, &e2, &scaled, &early)) return early;
 #if PDE_V >= 3
   {
     double c[2][V + 1], x0[2], root[2], dp[2], r;
@@ -640,9 +651,14 @@ DEV double max_abs_eig(double* H) {
     for (int j = 0; j <= V; j++) c[1][j] = c[0][j];
     real_roots<2>(c, x0, root, dp, ok);
     if (ok[0] && ok[1] && charpoly_certify(c[0], root[0], root[1], dp[0], dp[1], &r))
-      return __builtin_ldexp(r, e2);
+      return scaled ? __builtin_ldexp(r, e2) : r;
   }
 #endif
+  if (!scaled) {
+    const double pw = __builtin_ldexp(1.0, -e2);
+#pragma unroll
+    for (int i = 0; i < V * V; i++) H[i] = H[i] * pw;
+  }
   return __builtin_ldexp(eig_qr(H), e2);
 }
 

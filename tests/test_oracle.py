@@ -162,6 +162,26 @@ def test_max_abs_eig_numpy():
     assert abs(cc.oracle().oracle_max_abs_eig(cc.dp(A), 7) - want) <= 1e-10 * want
 
 
+def test_max_abs_eig_scales_exactly():
+    """max |lambda| is homogeneous: scaling a matrix by 2^k scales it by 2^k
+    bit for bit, whether the largest entry lies inside 2^+-64 (the matrix is
+    reduced unscaled) or outside (scaled first), on random, real-spectrum and
+    degree-17 matrices."""
+    rng = np.random.default_rng(12)
+    O = cc.oracle()
+    for V in (3, 4, 5, 17):
+        for t in range(300):
+            if t % 2:
+                P = rng.standard_normal((V, V))
+                A = P @ np.diag(rng.uniform(-3, 3, V)) @ np.linalg.inv(P)
+            else:
+                A = rng.standard_normal((V, V))
+            base = O.oracle_max_abs_eig(cc.dp(np.ascontiguousarray(A)), V)
+            for k in (-200, -70, -30, 10, 40, 63, 90, 300):
+                got = O.oracle_max_abs_eig(cc.dp(np.ascontiguousarray(A * 2.0 ** k)), V)
+                assert got == base * 2.0 ** k, (V, t, k)
+
+
 @pytest.mark.ref
 def test_max_abs_eig_reference():
     rng = np.random.default_rng(1)
