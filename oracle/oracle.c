@@ -1667,9 +1667,12 @@ static int newton_cell(const Ctx *C, const double *w, double *q, double *rhs,
       } else {
         for (int i = 0; i < V * V; i++) JS[i] = 0.0;
       }
-      for (int i = 0; i < V * V; i++) A[i] = C->dt * JS[i];
+      /* max |dt JS| as |dt| max |JS| (rounding is monotone: the same bits;
+       * 0 for a NaN dt, as the NaN-skipping maximum of the products gives) */
       double amax = 0.0;
-      for (int i = 0; i < V * V; i++) amax = fmax(amax, fabs(A[i]));
+      for (int i = 0; i < V * V; i++) amax = fmax(amax, fabs(JS[i]));
+      amax = fmax(0.0, fabs(C->dt) * amax);
+      for (int i = 0; i < V * V; i++) A[i] = C->dt * JS[i];
       if (amax < NEWTON_A_TINY) {
         /* negligible source Jacobian: (Tw - A)^-1 = Tw^-1, i.e. dq_t =
          * -(sum_s M_N[t][s] g_s) (1 / c_0) (the A = 0 case of the general step) */

@@ -293,7 +293,6 @@ DEV void real_roots(const double (&c)[K][V + 1], const double (&x0)[K], double (
     x[k] = x0[k];
     prev[k] = 0.0;
     state[k] = 0;
-    root[k] = 0.0;
     dp[k] = 0.0;
   }
   for (int it = 0; it < 40; it++) {
@@ -337,14 +336,18 @@ DEV void real_roots(const double (&c)[K][V + 1], const double (&x0)[K], double (
       const bool step_ok = run && !exact && (den != 0.0);
       const int ns = exact ? 1 : !(den != 0.0) ? 2 : conv ? (ax <= 1e300 ? 1 : 2) : 0;
       dp[k] = run ? d : dp[k];
-      root[k] = run ? (exact ? xk : xn) : root[k];
       x[k] = step_ok ? xn : xk;
       prev[k] = step_ok ? as : prev[k];
       state[k] = run ? ns : state[k];
     }
   }
+  // a root converges where its iterate stops: at xk for an exact zero, at
+  // the last step's xn otherwise (a failed root is not used)
 #pragma unroll
-  for (int k = 0; k < K; k++) ok[k] = state[k] == 1;
+  for (int k = 0; k < K; k++) {
+    ok[k] = state[k] == 1;
+    root[k] = x[k];
+  }
 }
 
 // Characteristic polynomial c of the scaled upper Hessenberg H and the

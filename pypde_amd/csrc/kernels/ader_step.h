@@ -440,11 +440,13 @@ DEV bool newton_node(const NodeCtx& c, const double (&q)[N][V], double (&G)[N][V
 #pragma unroll
   for (int i = 0; i < V * V; i++) A[i] = 0.0;
 #endif
-#pragma unroll
-  for (int i = 0; i < V * V; i++) A[i] = dt * A[i];
+  // max |dt dS/dQ| as |dt| max |dS/dQ|: rounding is monotone, so this is the
+  // maximum of the products, bit for bit (and 0 for a NaN dt, as the maximum
+  // that skips NaN entries gives); dt A itself is formed in the full path only
   double amax = 0.0;
 #pragma unroll
   for (int i = 0; i < V * V; i++) amax = dmax(amax, dabs(A[i]));
+  amax = dmax(0.0, dabs(dt) * amax);
   if (amax < NEWTON_A_TINY) {
     // negligible source Jacobian: dq_t = -(sum_s M_N[t][s] g_s) (1 / c_0)
 #pragma unroll
@@ -459,6 +461,8 @@ DEV bool newton_node(const NodeCtx& c, const double (&q)[N][V], double (&G)[N][V
     return false;
   }
   if constexpr (!FULL) return true;
+#pragma unroll
+  for (int i = 0; i < V * V; i++) A[i] = dt * A[i];
   // h_t (Horner in A); result left in G
 #pragma unroll
   for (int t = 0; t < N; t++) {
